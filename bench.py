@@ -1,0 +1,206 @@
+#!/usr/bin/env python3
+"""MISA-MD EAM step throughput on one MI355X.
+
+Workload (default): 128x128x128 BCC cells (4,194,304 atoms), a = 2.855 A,
+3 atom types drawn 90/5/5 from the seed with 9 synthetic pair tables (A, B, C
+jittered +-10%, phi symmetric), 5000-point splines on r in [0.5, 5.6] A,
+Gaussian displacement 0.05 A, Maxwell velocities at 800 K, velocity Verlet
+with dt = 1 fs, fully device-resident (ghost refresh, rho + df, ghost df,
+force, integration all on the GPU).  One step = one full velocity-Verlet step
+over every atom; the metric is atom-steps/s of the FP64 path.  The same run
+times the mixed-precision path on the same box and reports its force/energy
+error against FP64, and times the CPU oracle (a restatement of the
+reference's EAM algorithm) on the host cores.
+
+Inputs are resident in HBM before the timed region starts.  Per-kernel
+durations come from HIP events recorded on the context's stream around every
+launch.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "misa-md-hip_amd"), os.path.join(ROOT, "oracle")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import mmd  # noqa: E402
+
+A, RC, SKIN, DT = 2.855, 5.6, 0.3, 1.0
+PEAK_HBM_GBPS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_workload(cells, seed=2024):
+    spec = mmd.lattice_spec((cells, cells, cells), ghost_cells=3, a=A, n_types=3, sigma=0.05, temperature=800.0,
+                            seed=seed)
+    dom, box, st, _ = mmd.lattice_init(spec)
+    tab = mmd.synth_tables(n_types=3, rc=RC, jitter=0.1, seed=seed)
+    st3 = mmd.stencil_build(A, RC + SKIN, 0)
+    return dict(dom=dom, box=box, st=st, tab=tab, st3=st3, even=mmd.stencil_linear(dom, st3),
+                odd=mmd.stencil_linear(dom, -st3))
+
+
+def pairs_in_cutoff(w, sample=20000, seed=0):
+    """Mean number of in-cutoff neighbours per atom (N_c) over a sample."""
+    st, dom = w["st"], w["dom"]
+    li = st.local_index()
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(li, size=min(sample, len(li)), replace=False)
+    X = (idx % dom.ext[0]) - dom.ghost[0]
+    par = X % 2
+    tot = 0
+    for p, offs in ((0, w["even"]), (1, w["odd"])):
+        i = idx[par == p]
+        j = i[:, None] + offs[None, :]
+        d2 = (st.x[i, None] - st.x[j]) ** 2 + (st.y[i, None] - st.y[j]) ** 2 + (st.z[i, None] - st.z[j]) ** 2
+        tot += ((d2 < RC * RC) & (st.type[j] >= 0)).sum()
+    return tot / len(idx)
+
+
+def run_gpu(w, prec, steps, warmup, energy_every):
+    ctx = mmd.Context(0)
+    ctx.set_domain(w["dom"])
+    ctx.set_stencil(w["even"], w["odd"])
+    ctx.set_tables(w["tab"], mmd.DEFAULT_MASSES)
+    ctx.set_precision(prec)
+    ctx.upload(w["st"], mmd.F_X | mmd.F_V | mmd.F_TYPE)
+    ctx.md_setup(w["box"], A, DT, RC, energy_every=energy_every)
+    ctx.md_prime()
+    f0 = w["st"].copy()
+    ctx.download(f0, mmd.F_F)
+    e0 = ctx.energy()
+    if warmup:
+        ctx.md_run(warmup)
+    ctx.sync()
+    ctx.timing(True)
+    ctx.timing_reset()
+    en = np.zeros((steps, 2))
+    t0 = time.perf_counter()
+    ctx.md_run(steps, en)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ms, n = ctx.timing_get()
+    ctx.timing(False)
+    ctx.close()
+    return dict(seconds=t1 - t0, kernel_ms=ms, launches=n, energies=en, f0=f0, e0=e0)
+
+
+def cpu_baseline(w, steps):
+    import oracle as O
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    st = w["st"].copy()
+    T = O.Tables(w["tab"], mmd.DEFAULT_MASSES)
+    O.rho_df(w["dom"], w["even"], w["odd"], T, RC, st, nthreads=nthreads)
+    O.ghost_df(w["dom"], st)
+    O.force(w["dom"], w["even"], w["odd"], T, RC, st, nthreads=nthreads)
+    n_atoms = int((st.type[st.local_index()] >= 0).sum())
+    t0 = time.perf_counter()
+    O.md_run(w["dom"], w["even"], w["odd"], T, RC, w["box"], DT, steps, st, nthreads=nthreads, energies=False)
+    dt = time.perf_counter() - t0
+    return dict(value=n_atoms * steps / dt, unit="atom-steps/s", cores=nthreads, kind="port",
+                sample=f"CPU oracle (FP64 restatement of the reference EAM step, OpenMP) on the same "
+                       f"{n_atoms}-atom box for {steps} velocity-Verlet steps, {dt:.1f} s")
+
+
+def load_traffic():
+    """HBM bytes per force launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), if profiled."""
+    p = os.path.join(ROOT, "profiles", "force_fp64_128cube_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cells", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs one sub-domain per GPU and has no multi-GPU "
+              f"path; --gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    log(f"bench: building {args.cells}^3-cell 3-type BCC workload")
+    w = make_workload(args.cells)
+    li = w["st"].local_index()
+    n_atoms = int((w["st"].type[li] >= 0).sum())
+    S = len(w["st3"])
+    Nc = pairs_in_cutoff(w)
+
+    log(f"bench: FP64 {args.steps} steps (+{args.warmup} warmup), {n_atoms} atoms")
+    r64 = run_gpu(w, mmd.PREC_FP64, args.steps, args.warmup, energy_every=10)
+    value = n_atoms * args.steps / r64["seconds"]
+    ms, n = r64["kernel_ms"], r64["launches"]
+    t_force = ms[mmd.K_FORCE] / max(n[mmd.K_FORCE], 1) * 1e-3
+    t_rho = ms[mmd.K_RHO] / max(n[mmd.K_RHO], 1) * 1e-3
+    flops_force = (8 * S + 40 * Nc) * n_atoms
+    flops_rho = (8 * S + 10 * Nc + 10) * n_atoms
+    bytes_force = 57 * n_atoms
+    traffic = load_traffic()
+    achieved = flops_force / t_force / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+                "kernel": "k_force<FP64>", "kernel_ms": round(t_force * 1e3, 4),
+                "hbm_achieved_GBps": round(bytes_force / t_force / 1e9, 1),
+                "hbm_frac": round(bytes_force / t_force / 1e9 / PEAK_HBM_GBPS, 4),
+                "flops_per_atom": round(8 * S + 40 * Nc, 1), "alg_bytes_per_atom": 57,
+                "rho_df_kernel_ms": round(t_rho * 1e3, 4),
+                "rho_df_tflops": round(flops_rho / t_rho / 1e12, 3)}
+    out = {"metric": "atom-steps/s (FP64 EAM velocity-Verlet step)", "value": round(value, 1),
+           "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(r64["seconds"] / args.steps * 1e3, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"BCC {args.cells}^3 cells, 3-type alloy 90/5/5, 800 K, dt 1 fs, FP64",
+                      "atoms": n_atoms, "stencil_entries": S, "pairs_in_cutoff": round(Nc, 2),
+                      "energy_every": 10},
+           "roofline": roofline,
+           "kernel_ms_per_step": {k: round(ms[i] / max(args.steps, 1), 4) for i, k in
+                                  enumerate(("rho_df", "force", "integrate", "ghost", "interstitial"))}}
+
+    if not args.no_mixed:
+        log("bench: mixed precision on the same box")
+        rmx = run_gpu(w, mmd.PREC_MIXED, args.steps, args.warmup, energy_every=10)
+        vmx = n_atoms * args.steps / rmx["seconds"]
+        F64 = np.stack([r64["f0"].fx[li], r64["f0"].fy[li], r64["f0"].fz[li]], 1)
+        FMX = np.stack([rmx["f0"].fx[li], rmx["f0"].fy[li], rmx["f0"].fz[li]], 1)
+        d = np.linalg.norm(FMX - F64, axis=1)
+        rms = np.sqrt((np.linalg.norm(F64, axis=1) ** 2).mean())
+        e64 = r64["energies"][9::10].sum(1)
+        emx = rmx["energies"][9::10].sum(1)
+        drift = np.abs((emx - e64) - (rmx["e0"][0] + rmx["e0"][1] - r64["e0"][0] - r64["e0"][1]))
+        steps_at = np.arange(10, args.steps + 1, 10)
+        mk = rmx["kernel_ms"]
+        mn = rmx["launches"]
+        out["mixed"] = {"value": round(vmx, 1), "speedup_vs_fp64": round(vmx / value, 3),
+                        "force_rel_err_normwise": float(np.linalg.norm(FMX - F64) / np.linalg.norm(F64)),
+                        "force_err_max_over_rms": float(d.max() / rms),
+                        "energy_diff_per_atom_per_step": float((drift / steps_at).max() / n_atoms)
+                        if len(steps_at) else None,
+                        "force_kernel_ms": round(mk[mmd.K_FORCE] / max(mn[mmd.K_FORCE], 1), 4),
+                        "rho_df_kernel_ms": round(mk[mmd.K_RHO] / max(mn[mmd.K_RHO], 1), 4)}
+
+    if not args.no_cpu:
+        log(f"bench: CPU oracle baseline, {args.cpu_steps} steps")
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,864 @@
+// C ABI of the device-resident EAM path: context, domain, stencil, tables,
+// data movement, the three hook passes, the drop-in host-array forms, the MD
+// step and kernel timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "mmd_eam.h"
+#include "ctx.h"
+#include "host_helpers.h"
+#include "kernels.h"
+
+using namespace mmd;
+
+#define HIP_TRY(expr)                                                                                       \
+  do {                                                                                                      \
+    hipError_t e_ = (expr);                                                                                 \
+    if (e_ != hipSuccess)                                                                                   \
+      return fail(MMD_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+#define GUARD(...)                                                         \
+  try {                                                                    \
+    __VA_ARGS__                                                            \
+  } catch (const std::bad_alloc &) {                                       \
+    return fail(MMD_E_NOMEM, "%s: host allocation failed", __func__);      \
+  } catch (...) {                                                          \
+    return fail(MMD_E_INVALID, "%s: unexpected C++ exception", __func__);  \
+  }
+
+namespace mmd {
+
+int ctx_bind(mmd_ctx *c) {
+  if (!c) return fail(MMD_E_INVALID, "null context");
+  HIP_TRY(hipSetDevice(c->device));
+  return MMD_OK;
+}
+
+template <typename T>
+static int dalloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) return MMD_OK;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(MMD_E_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+  }
+  return MMD_OK;
+}
+
+template <typename T>
+static void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+static void free_state(mmd_ctx *c) {
+  dfree(c->x), dfree(c->y), dfree(c->z), dfree(c->vx), dfree(c->vy), dfree(c->vz);
+  dfree(c->fx), dfree(c->fy), dfree(c->fz), dfree(c->rho), dfree(c->df), dfree(c->type);
+  dfree(c->stage), dfree(c->e_rho), dfree(c->e_force), dfree(c->e_ke), dfree(c->e_itl);
+  dfree(c->aos), c->aos_bytes = 0;
+}
+
+void ctx_drop_graphs(mmd_ctx *c) {
+  for (auto &g : c->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// --------------------------------------------------------------- timing
+
+int timed_begin(mmd_ctx *c, int cls, hipEvent_t *ev) {
+  *ev = nullptr;
+  if (!c->timing || c->capturing) return MMD_OK;
+  if (c->event_pool.empty()) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    c->event_pool.push_back(e);
+  }
+  *ev = c->event_pool.back();
+  c->event_pool.pop_back();
+  HIP_TRY(hipEventRecord(*ev, c->stream));
+  (void)cls;
+  return MMD_OK;
+}
+
+int timed_end(mmd_ctx *c, int cls, hipEvent_t start) {
+  if (!start) return MMD_OK;
+  if (c->event_pool.empty()) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    c->event_pool.push_back(e);
+  }
+  hipEvent_t stop = c->event_pool.back();
+  c->event_pool.pop_back();
+  HIP_TRY(hipEventRecord(stop, c->stream));
+  c->pending.push_back({cls, start, stop});
+  return MMD_OK;
+}
+
+static int timing_collect(mmd_ctx *c) {
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (auto &p : c->pending) {
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, p.start, p.stop));
+    c->t_ms[p.cls] += ms;
+    c->t_n[p.cls] += 1;
+    c->event_pool.push_back(p.start);
+    c->event_pool.push_back(p.stop);
+  }
+  c->pending.clear();
+  return MMD_OK;
+}
+
+// ---------------------------------------------------------- pass helpers
+
+int run_rho_df(mmd_ctx *c, double cutoff, bool want_e) {
+  hipEvent_t ev;
+  int rc = timed_begin(c, MMD_K_RHO, &ev);
+  if (rc) return rc;
+  HIP_TRY(launch_rho_df(c->stream, c->g, c->d_off, c->nk_even, c->nk_odd, c->x, c->y, c->z, c->type, c->rho, c->df,
+                        c->tf, c->tm, cutoff * cutoff, c->prec, want_e ? c->e_rho : nullptr));
+  rc = timed_end(c, MMD_K_RHO, ev);
+  if (rc) return rc;
+  if (c->n_itl > 0) return itl_rho_df(c, cutoff, want_e);
+  return MMD_OK;
+}
+
+int run_force(mmd_ctx *c, double cutoff, bool want_e) {
+  hipEvent_t ev;
+  int rc = timed_begin(c, MMD_K_FORCE, &ev);
+  if (rc) return rc;
+  HIP_TRY(launch_force(c->stream, c->g, c->d_off, c->nk_even, c->nk_odd, c->x, c->y, c->z, c->type, c->df, c->fx,
+                       c->fy, c->fz, c->tf, c->tm, cutoff * cutoff, c->prec, want_e ? c->e_force : nullptr));
+  rc = timed_end(c, MMD_K_FORCE, ev);
+  if (rc) return rc;
+  if (c->n_itl > 0) return itl_force(c, cutoff, want_e);
+  return MMD_OK;
+}
+
+int run_ghost(mmd_ctx *c, uint32_t fields) {
+  hipEvent_t ev;
+  int rc = timed_begin(c, MMD_K_GHOST, &ev);
+  if (rc) return rc;
+  HIP_TRY(launch_ghost(c->stream, c->g, c->md.box_len[0], c->md.box_len[1], c->md.box_len[2], c->x, c->y, c->z,
+                       c->type, c->df, fields));
+  return timed_end(c, MMD_K_GHOST, ev);
+}
+
+static int run_kick(mmd_ctx *c, double h, bool drift, bool want_ke) {
+  hipEvent_t ev;
+  int rc = timed_begin(c, MMD_K_INTEGRATE, &ev);
+  if (rc) return rc;
+  HIP_TRY(launch_kick(c->stream, c->g, c->masses, h, c->md.dt, drift, c->x, c->y, c->z, c->vx, c->vy, c->vz, c->fx,
+                      c->fy, c->fz, c->type, want_ke ? c->e_ke : nullptr));
+  rc = timed_end(c, MMD_K_INTEGRATE, ev);
+  if (rc) return rc;
+  if (c->n_itl > 0) return itl_kick(c, h, drift, want_ke);
+  return MMD_OK;
+}
+
+static int run_finalize(mmd_ctx *c, double *out, int32_t *counter) {
+  const int64_t nb = site_blocks(c->g);
+  HIP_TRY(launch_energy_finalize(c->stream, c->e_rho, nb, c->e_force, nb, c->n_itl > 0 ? c->e_itl : nullptr,
+                                 c->n_itl > 0 ? c->itl_part_n : 0, c->e_ke, nb, out, counter));
+  return MMD_OK;
+}
+
+// one velocity-Verlet step; energy recorded into d_energy[(*d_counter)++]
+static int md_step(mmd_ctx *c, bool want_e) {
+  int rc;
+  if ((rc = run_kick(c, 0.5 * c->md.dt, true, false))) return rc;
+  if ((rc = run_ghost(c, MMD_F_X))) return rc;
+  if ((rc = run_rho_df(c, c->md.cutoff, want_e))) return rc;
+  if ((rc = run_ghost(c, MMD_F_DF))) return rc;
+  if ((rc = run_force(c, c->md.cutoff, want_e))) return rc;
+  if ((rc = run_kick(c, 0.5 * c->md.dt, false, want_e))) return rc;
+  if (want_e && (rc = run_finalize(c, c->d_energy, c->d_counter))) return rc;
+  return MMD_OK;
+}
+
+static int ensure_energy_cap(mmd_ctx *c, int64_t n) {
+  if (n <= c->energy_cap) return MMD_OK;
+  dfree(c->d_energy);
+  int rc = dalloc(&c->d_energy, (size_t)(2 * n));
+  if (rc) return rc;
+  c->energy_cap = n;
+  ctx_drop_graphs(c);  // graphs hold the old pointer
+  return MMD_OK;
+}
+
+}  // namespace mmd
+
+// ================================================================== C ABI
+
+extern "C" int mmd_ctx_create(int32_t device, mmd_ctx **out) {
+  GUARD({
+    clear_error();
+    if (!out) return fail(MMD_E_INVALID, "mmd_ctx_create: null out");
+    *out = nullptr;
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(MMD_E_INVALID, "mmd_ctx_create: device %d of %d", device, n);
+    HIP_TRY(hipSetDevice(device));
+    mmd_ctx *c = new mmd_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      return fail(MMD_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    int rc = dalloc(&c->d_counter, 4);
+    if (rc) {
+      (void)hipStreamDestroy(c->stream);
+      delete c;
+      return rc;
+    }
+    *out = c;
+    return MMD_OK;
+  })
+}
+
+extern "C" int mmd_ctx_destroy(mmd_ctx *c) {
+  GUARD({
+    if (!c) return MMD_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    ctx_drop_graphs(c);
+    free_state(c);
+    itl_free(c);
+    dfree(c->d_off), dfree(c->d_tab), dfree(c->d_mixed), dfree(c->d_energy), dfree(c->d_counter);
+    for (auto &p : c->pending) c->event_pool.push_back(p.start), c->event_pool.push_back(p.stop);
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MMD_OK;
+  })
+}
+
+extern "C" int mmd_set_domain(mmd_ctx *c, const mmd_domain *d) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc) return rc;
+    if (!d) return fail(MMD_E_INVALID, "mmd_set_domain: null domain");
+    for (int k = 0; k < 3; k++) {
+      if (d->box[k] < 1 || d->ghost[k] < 0 || d->ext[k] != d->box[k] + 2 * d->ghost[k])
+        return fail(MMD_E_INVALID, "mmd_set_domain: dim %d needs box>=1, ghost>=0, ext=box+2*ghost", k);
+      if (d->ghost[k] > d->box[k])
+        return fail(MMD_E_INVALID, "mmd_set_domain: ghost wider than the box in dim %d", k);
+    }
+    if ((d->box[0] | d->ghost[0]) & 1)
+      return fail(MMD_E_INVALID, "mmd_set_domain: box and ghost along x must be even (two sites per cell)");
+    if (((d->start[0] - d->ghost[0]) % 2 + 2) % 2 != 0)
+      return fail(MMD_E_INVALID, "mmd_set_domain: start_x - ghost_x must be even");
+    const int64_t n_ext = (int64_t)d->ext[0] * d->ext[1] * d->ext[2];
+    if (n_ext >= ((int64_t)1 << 31)) return fail(MMD_E_RANGE, "mmd_set_domain: %lld ext sites exceed 2^31", (long long)n_ext);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    ctx_drop_graphs(c);
+    free_state(c);
+    c->has_domain = false;
+    c->has_stencil = false;
+    c->n_itl = 0;
+    Geom &g = c->g;
+    g.ext_x = d->ext[0], g.ext_y = d->ext[1], g.ext_z = d->ext[2];
+    g.exh = d->ext[0] / 2;
+    g.gxh = d->ghost[0] / 2, g.gy = d->ghost[1], g.gz = d->ghost[2];
+    g.bxh = d->box[0] / 2, g.by = d->box[1], g.bz = d->box[2];
+    g.chunks = (g.bxh + kWave - 1) / kWave;
+    g.n_ext = n_ext;
+    g.n_waves = (int64_t)2 * g.by * g.bz * g.chunks;
+    c->dom = *d;
+    const size_t n = (size_t)n_ext;
+    double **f64[] = {&c->x, &c->y, &c->z, &c->vx, &c->vy, &c->vz, &c->fx, &c->fy, &c->fz, &c->rho, &c->df, &c->stage};
+    for (double **p : f64) {
+      if ((rc = dalloc(p, n))) {
+        free_state(c);
+        return rc;
+      }
+      HIP_TRY(launch_fill_f64(c->stream, *p, (int64_t)n, 0.0));
+    }
+    if ((rc = dalloc(&c->type, n))) {
+      free_state(c);
+      return rc;
+    }
+    HIP_TRY(launch_fill_i8(c->stream, c->type, (int64_t)n, (int8_t)-1));
+    const size_t nb = (size_t)site_blocks(g);
+    if ((rc = dalloc(&c->e_rho, nb)) || (rc = dalloc(&c->e_force, nb)) || (rc = dalloc(&c->e_ke, nb))) {
+      free_state(c);
+      return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->has_domain = true;
+    return MMD_OK;
+  })
+}
+
+static int64_t centred_mod(int64_t v, int64_t n) {
+  int64_t r = ((v % n) + n) % n;
+  if (r >= n / 2) r -= n;
+  return r;
+}
+
+static int64_t floor_div2(int64_t v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); }
+
+extern "C" int mmd_set_stencil(mmd_ctx *c, const int64_t *even, int32_t n_even, const int64_t *odd, int32_t n_odd) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc) return rc;
+    if (!c->has_domain) return fail(MMD_E_STATE, "mmd_set_stencil: set the domain first");
+    if (n_even < 0 || n_odd < 0 || n_even > kMaxStencil || n_odd > kMaxStencil || (n_even && !even) ||
+        (n_odd && !odd))
+      return fail(MMD_E_INVALID, "mmd_set_stencil: sizes must be in [0, %d]", kMaxStencil);
+    const mmd_domain &d = c->dom;
+    std::vector<int32_t> host(2 * kMaxStencil, 0);
+    for (int p = 0; p < 2; p++) {
+      const int64_t *src = p ? odd : even;
+      const int32_t n = p ? n_odd : n_even;
+      for (int32_t k = 0; k < n; k++) {
+        const int64_t o = src[k];
+        const int64_t dX = centred_mod(o, d.ext[0]);
+        const int64_t o1 = (o - dX) / d.ext[0];
+        const int64_t dY = centred_mod(o1, d.ext[1]);
+        const int64_t dZ = (o1 - dY) / d.ext[1];
+        if (std::llabs(dX) > d.ghost[0] || std::llabs(dY) > d.ghost[1] || std::llabs(dZ) > d.ghost[2] ||
+            (!dX && !dY && !dZ))
+          return fail(MMD_E_RANGE, "mmd_set_stencil: parity %d entry %d (offset %lld) leaves the ghost shell", p, k,
+                      (long long)o);
+        const int64_t s1 = (p + dX) & 1;
+        const int64_t dev = ((dZ * d.ext[1] + dY) * 2 + (s1 - p)) * (d.ext[0] / 2) + floor_div2(p + dX);
+        host[(size_t)p * kMaxStencil + k] = (int32_t)dev;
+      }
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!c->d_off && (rc = dalloc(&c->d_off, 2 * kMaxStencil))) return rc;
+    HIP_TRY(hipMemcpy(c->d_off, host.data(), host.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    c->nk_even = n_even;
+    c->nk_odd = n_odd;
+    c->has_stencil = true;
+    ctx_drop_graphs(c);
+    return MMD_OK;
+  })
+}
+
+// Mixed-precision r^2-keyed tables resampled from the FP64 r-keyed spline:
+// knot values and exact derivatives of g(sqrt(s)) at uniform s, cubic Hermite
+// in s; FP64 leading terms, FP32 correction polynomials.
+static void build_mixed(const double *c7, int n_r, double r0, double inv_dr, int n_s, double s0, double ds,
+                        double *vrec /*[n_s][4]*/, double *drec /*[n_s][2]*/) {
+  std::vector<double> h(n_s), hp(n_s);
+  for (int i = 0; i < n_s; i++) {
+    const double s = s0 + ds * (double)i;
+    const double r = std::sqrt(s);
+    h[i] = spline7_value(c7, n_r, r0, inv_dr, r);
+    hp[i] = spline7_deriv(c7, n_r, r0, inv_dr, r) / (2.0 * r);
+  }
+  for (int m = 0; m < n_s; m++) {
+    double a1 = 0, a2 = 0, a3 = 0;
+    if (m < n_s - 1) {
+      const double D = h[m + 1] - h[m];
+      a1 = hp[m] * ds;
+      const double a1n = hp[m + 1] * ds;
+      a2 = 3.0 * D - 2.0 * a1 - a1n;
+      a3 = a1 + a1n - 2.0 * D;
+    }
+    float fa[4] = {(float)a1, (float)a2, (float)a3, 0.0f};
+    vrec[4 * m + 0] = h[m];
+    std::memcpy(&vrec[4 * m + 1], &fa[0], 8);
+    std::memcpy(&vrec[4 * m + 2], &fa[2], 8);
+    vrec[4 * m + 3] = 0.0;
+    float fb[2] = {(float)(2.0 * a2 / ds), (float)(3.0 * a3 / ds)};
+    drec[2 * m + 0] = hp[m];
+    std::memcpy(&drec[2 * m + 1], fb, 8);
+  }
+}
+
+extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc) return rc;
+    if (!t || t->n_types < 1 || t->n_types > kMaxTypes || t->n_r < 5 || t->n_rho < 5 || !(t->dr > 0.0) ||
+        !(t->drho > 0.0) || !(t->r0 > 0.0) || !t->rho_r || !t->emb || !t->phi_r || !t->mass)
+      return fail(MMD_E_INVALID, "mmd_set_tables: bad table descriptor");
+    const int nt = t->n_types, nr = t->n_r, nrho = t->n_rho;
+    for (int a = 0; a < nt; a++) {
+      if (!(t->mass[a] > 0.0)) return fail(MMD_E_INVALID, "mmd_set_tables: mass of type %d must be > 0", a);
+      for (int b = 0; b < nt; b++)
+        if (std::memcmp(t->phi_r + ((size_t)a * nt + b) * nr, t->phi_r + ((size_t)b * nt + a) * nr,
+                        sizeof(double) * nr) != 0)
+          return fail(MMD_E_INVALID, "mmd_set_tables: phi_%d%d != phi_%d%d (pair tables must be symmetric)", a, b, b,
+                      a);
+    }
+    const size_t n_fun_r = (size_t)nt + (size_t)nt * nt;
+    const size_t words = (n_fun_r * nr + (size_t)nt * nrho) * 8;
+    std::vector<double> host(words, 0.0), c7((size_t)std::max(nr, nrho) * 7);
+    auto pack = [&](const double *f, int n, double dx, double *dst) {
+      spline7(f, n, dx, c7.data());
+      for (int m = 0; m < n; m++) {
+        const double *k = &c7[(size_t)m * 7];
+        double *o = dst + (size_t)m * 8;
+        o[0] = k[3], o[1] = k[4], o[2] = k[5], o[3] = k[6];
+        o[4] = k[0], o[5] = k[1], o[6] = k[2], o[7] = 0.0;
+      }
+    };
+    double *h_rho = host.data();
+    double *h_emb = h_rho + (size_t)nt * nr * 8;
+    double *h_phi = h_emb + (size_t)nt * nrho * 8;
+    for (int a = 0; a < nt; a++) pack(t->rho_r + (size_t)a * nr, nr, t->dr, h_rho + (size_t)a * nr * 8);
+    for (int a = 0; a < nt; a++) pack(t->emb + (size_t)a * nrho, nrho, t->drho, h_emb + (size_t)a * nrho * 8);
+    for (int p = 0; p < nt * nt; p++) pack(t->phi_r + (size_t)p * nr, nr, t->dr, h_phi + (size_t)p * nr * 8);
+
+    // mixed: r^2-keyed, n_s knots over [r0^2, rmax^2]
+    const int n_s = c->mixed_knots > 0 ? c->mixed_knots : nr;
+    const double rmax = t->r0 + t->dr * (double)(nr - 1);
+    const double s0 = t->r0 * t->r0, ds = (rmax * rmax - s0) / (double)(n_s - 1);
+    const size_t vwords = (size_t)n_s * 4, dwords = (size_t)n_s * 2;
+    std::vector<double> mixed(n_fun_r * (vwords + dwords), 0.0);
+    double *m_rho_v = mixed.data();
+    double *m_phi_v = m_rho_v + (size_t)nt * vwords;
+    double *m_rho_d = m_phi_v + (size_t)nt * nt * vwords;
+    double *m_phi_d = m_rho_d + (size_t)nt * dwords;
+    const double inv_dr = 1.0 / t->dr;
+    for (int a = 0; a < nt; a++) {
+      spline7(t->rho_r + (size_t)a * nr, nr, t->dr, c7.data());
+      build_mixed(c7.data(), nr, t->r0, inv_dr, n_s, s0, ds, m_rho_v + a * vwords, m_rho_d + a * dwords);
+    }
+    for (int p = 0; p < nt * nt; p++) {
+      spline7(t->phi_r + (size_t)p * nr, nr, t->dr, c7.data());
+      build_mixed(c7.data(), nr, t->r0, inv_dr, n_s, s0, ds, m_phi_v + p * vwords, m_phi_d + p * dwords);
+    }
+
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    ctx_drop_graphs(c);
+    dfree(c->d_tab), dfree(c->d_mixed);
+    if ((rc = dalloc(&c->d_tab, words))) return rc;
+    if ((rc = dalloc(&c->d_mixed, mixed.size()))) return rc;
+    HIP_TRY(hipMemcpy(c->d_tab, host.data(), words * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_mixed, mixed.data(), mixed.size() * sizeof(double), hipMemcpyHostToDevice));
+    TabF64 &tf = c->tf;
+    tf.rho = c->d_tab;
+    tf.emb = c->d_tab + (h_emb - h_rho);
+    tf.phi = c->d_tab + (h_phi - h_rho);
+    tf.nt = nt, tf.n_r = nr, tf.n_rho = nrho;
+    tf.r0 = t->r0, tf.inv_dr = inv_dr, tf.rho0 = t->rho0, tf.inv_drho = 1.0 / t->drho;
+    TabMixed &tm = c->tm;
+    tm.rho_v = reinterpret_cast<const double4 *>(c->d_mixed);
+    tm.phi_v = reinterpret_cast<const double4 *>(c->d_mixed + (m_phi_v - m_rho_v));
+    tm.rho_d = reinterpret_cast<const double2 *>(c->d_mixed + (m_rho_d - m_rho_v));
+    tm.phi_d = reinterpret_cast<const double2 *>(c->d_mixed + (m_phi_d - m_rho_v));
+    tm.nt = nt, tm.n_s = n_s, tm.s0 = s0, tm.inv_ds = 1.0 / ds;
+    for (int a = 0; a < kMaxTypes; a++) {
+      c->masses.m[a] = a < nt ? t->mass[a] : 1.0;
+      c->masses.inv_m[a] = 1.0 / c->masses.m[a];
+    }
+    c->n_types = nt;
+    c->r_max = rmax;
+    c->has_tables = true;
+    return MMD_OK;
+  })
+}
+
+extern "C" int mmd_set_precision(mmd_ctx *c, int32_t mode) {
+  int rc = ctx_bind(c);
+  if (rc) return rc;
+  if (mode != MMD_PREC_FP64 && mode != MMD_PREC_MIXED) return fail(MMD_E_INVALID, "mmd_set_precision: mode %d", mode);
+  if (c->prec != mode) ctx_drop_graphs(c);
+  c->prec = mode;
+  return MMD_OK;
+}
+
+static int check_ready(mmd_ctx *c, const char *fn, bool need_tables = true) {
+  if (!c->has_domain) return fail(MMD_E_STATE, "%s: domain not set", fn);
+  if (!c->has_stencil) return fail(MMD_E_STATE, "%s: stencil not set", fn);
+  if (need_tables && !c->has_tables) return fail(MMD_E_STATE, "%s: tables not set", fn);
+  if (need_tables && c->max_type >= c->n_types)
+    return fail(MMD_E_INVALID, "%s: atom type %d has no table (n_types = %d)", fn, c->max_type, c->n_types);
+  return MMD_OK;
+}
+
+static int scan_types_i8(const int8_t *t, size_t n, int32_t *mx) {
+  int32_t m = -1;
+  for (size_t i = 0; i < n; i++) m = std::max<int32_t>(m, t[i]);
+  *mx = m;
+  return MMD_OK;
+}
+
+static int check_cutoff(mmd_ctx *c, double rc, const char *fn) {
+  if (!(rc > 0.0) || !(rc <= c->r_max * (1.0 + 1e-12)))
+    return fail(MMD_E_INVALID, "%s: cutoff %g outside (0, %g] covered by the tables", fn, rc, c->r_max);
+  return MMD_OK;
+}
+
+// ------------------------------------------------------------ data movement
+
+extern "C" int mmd_upload(mmd_ctx *c, uint32_t fields, const mmd_host_soa *h) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc) return rc;
+    if (!c->has_domain) return fail(MMD_E_STATE, "mmd_upload: domain not set");
+    if (!h || (fields & ~MMD_F_ALL)) return fail(MMD_E_INVALID, "mmd_upload: bad arguments");
+    const size_t n = (size_t)c->g.n_ext;
+    struct F {
+      uint32_t bit;
+      const double *src;
+      double *dst;
+    } fs[] = {{MMD_F_X, h->x, c->x},     {MMD_F_X, h->y, c->y},     {MMD_F_X, h->z, c->z},
+              {MMD_F_V, h->vx, c->vx},   {MMD_F_V, h->vy, c->vy},   {MMD_F_V, h->vz, c->vz},
+              {MMD_F_F, h->fx, c->fx},   {MMD_F_F, h->fy, c->fy},   {MMD_F_F, h->fz, c->fz},
+              {MMD_F_RHO, h->rho, c->rho}, {MMD_F_DF, h->df, c->df}};
+    for (auto &f : fs) {
+      if (!(fields & f.bit)) continue;
+      if (!f.src) return fail(MMD_E_INVALID, "mmd_upload: host pointer missing for field mask 0x%x", f.bit);
+      HIP_TRY(hipMemcpyAsync(c->stage, f.src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HIP_TRY(launch_ref2dev_f64(c->stream, c->g, c->stage, f.dst));
+    }
+    if (fields & MMD_F_TYPE) {
+      if (!h->type) return fail(MMD_E_INVALID, "mmd_upload: type pointer missing");
+      int32_t mx;
+      scan_types_i8(h->type, n, &mx);
+      if (mx >= kMaxTypes) return fail(MMD_E_INVALID, "mmd_upload: atom type %d beyond %d", mx, kMaxTypes - 1);
+      c->max_type = mx;
+      int8_t *st = reinterpret_cast<int8_t *>(c->stage);
+      HIP_TRY(hipMemcpyAsync(st, h->type, n, hipMemcpyHostToDevice, c->stream));
+      HIP_TRY(launch_ref2dev_i8(c->stream, c->g, st, c->type));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MMD_OK;
+  })
+}
+
+extern "C" int mmd_download(mmd_ctx *c, uint32_t fields, const mmd_host_soa *h) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc) return rc;
+    if (!c->has_domain) return fail(MMD_E_STATE, "mmd_download: domain not set");
+    if (!h || (fields & ~MMD_F_ALL)) return fail(MMD_E_INVALID, "mmd_download: bad arguments");
+    const size_t n = (size_t)c->g.n_ext;
+    struct F {
+      uint32_t bit;
+      double *dst;
+      const double *src;
+    } fs[] = {{MMD_F_X, h->x, c->x},     {MMD_F_X, h->y, c->y},     {MMD_F_X, h->z, c->z},
+              {MMD_F_V, h->vx, c->vx},   {MMD_F_V, h->vy, c->vy},   {MMD_F_V, h->vz, c->vz},
+              {MMD_F_F, h->fx, c->fx},   {MMD_F_F, h->fy, c->fy},   {MMD_F_F, h->fz, c->fz},
+              {MMD_F_RHO, h->rho, c->rho}, {MMD_F_DF, h->df, c->df}};
+    for (auto &f : fs) {
+      if (!(fields & f.bit)) continue;
+      if (!f.dst) return fail(MMD_E_INVALID, "mmd_download: host pointer missing for field mask 0x%x", f.bit);
+      HIP_TRY(launch_dev2ref_f64(c->stream, c->g, f.src, c->stage));
+      HIP_TRY(hipMemcpyAsync(f.dst, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    if (fields & MMD_F_TYPE) {
+      if (!h->type) return fail(MMD_E_INVALID, "mmd_download: type pointer missing");
+      int8_t *st = reinterpret_cast<int8_t *>(c->stage);
+      HIP_TRY(launch_dev2ref_i8(c->stream, c->g, c->type, st));
+      HIP_TRY(hipMemcpyAsync(h->type, st, n, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MMD_OK;
+  })
+}
+
+// ------------------------------------------------------------ hook passes
+
+extern "C" int mmd_eam_rho_df(mmd_ctx *c, double cutoff) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc || (rc = check_ready(c, "mmd_eam_rho_df")) || (rc = check_cutoff(c, cutoff, "mmd_eam_rho_df"))) return rc;
+    return run_rho_df(c, cutoff, false);
+  })
+}
+
+extern "C" int mmd_eam_df(mmd_ctx *c) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc || (rc = check_ready(c, "mmd_eam_df"))) return rc;
+    HIP_TRY(launch_df(c->stream, c->g, c->type, c->rho, c->df, c->tf));
+    if (c->n_itl > 0) return itl_df(c);
+    return MMD_OK;
+  })
+}
+
+extern "C" int mmd_eam_force(mmd_ctx *c, double cutoff) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc || (rc = check_ready(c, "mmd_eam_force")) || (rc = check_cutoff(c, cutoff, "mmd_eam_force"))) return rc;
+    return run_force(c, cutoff, false);
+  })
+}
+
+extern "C" int mmd_sync(mmd_ctx *c) {
+  int rc = ctx_bind(c);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MMD_OK;
+}
+
+// ------------------------------------------------- drop-in host AoS forms
+
+static int aos_desc(const mmd_aos_layout *lay, AosDesc *d) {
+  if (!lay || lay->stride < 8 || (lay->type_bytes != 1 && lay->type_bytes != 2 && lay->type_bytes != 4))
+    return fail(MMD_E_INVALID, "AoS layout: stride >= 8 and type_bytes in {1,2,4} required");
+  const size_t offs[] = {lay->off_x + 24, lay->off_v + 24, lay->off_f + 24, lay->off_rho + 8, lay->off_df + 8,
+                         lay->off_type + (size_t)lay->type_bytes};
+  for (size_t o : offs)
+    if (o > lay->stride) return fail(MMD_E_INVALID, "AoS layout: field offset beyond the record stride");
+  d->stride = (int64_t)lay->stride, d->off_type = (int64_t)lay->off_type, d->off_x = (int64_t)lay->off_x;
+  d->off_v = (int64_t)lay->off_v, d->off_f = (int64_t)lay->off_f, d->off_rho = (int64_t)lay->off_rho;
+  d->off_df = (int64_t)lay->off_df, d->type_bytes = lay->type_bytes;
+  return MMD_OK;
+}
+
+// upload the whole ext record array, unpack `in` fields, run, pack `out`
+// fields into the local records and copy the local z-layers back.
+static int aos_pass(mmd_ctx *c, void *atoms, const mmd_aos_layout *lay, uint32_t in, uint32_t out, int pass,
+                    double cutoff) {
+  if (!atoms) return fail(MMD_E_INVALID, "null atom array");
+  AosDesc d;
+  int rc = aos_desc(lay, &d);
+  if (rc) return rc;
+  const size_t bytes = (size_t)c->g.n_ext * lay->stride;
+  if (in & MMD_F_TYPE) {  // validate types on the host before they index device tables
+    const char *base = static_cast<const char *>(atoms) + lay->off_type;
+    int32_t mx = -1;
+    for (int64_t i = 0; i < c->g.n_ext; i++) {
+      const char *p = base + (size_t)i * lay->stride;
+      int32_t t;
+      if (lay->type_bytes == 1)
+        t = *reinterpret_cast<const int8_t *>(p);
+      else if (lay->type_bytes == 2)
+        t = *reinterpret_cast<const int16_t *>(p);
+      else
+        t = *reinterpret_cast<const int32_t *>(p);
+      mx = std::max(mx, t);
+    }
+    if (mx >= c->n_types) return fail(MMD_E_INVALID, "atom type %d has no table (n_types = %d)", mx, c->n_types);
+    c->max_type = mx;
+  }
+  if (bytes > c->aos_bytes) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    dfree(c->aos);
+    c->aos_bytes = 0;
+    if ((rc = dalloc(&c->aos, bytes))) return rc;
+    c->aos_bytes = bytes;
+  }
+  HIP_TRY(hipMemcpyAsync(c->aos, atoms, bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(launch_aos_unpack(c->stream, c->g, c->aos, d, in, c->x, c->y, c->z, c->type, c->df, c->rho));
+  if (pass == 0) rc = run_rho_df(c, cutoff, false);
+  if (pass == 1) {
+    HIP_TRY(launch_df(c->stream, c->g, c->type, c->rho, c->df, c->tf));
+  }
+  if (pass == 2) rc = run_force(c, cutoff, false);
+  if (rc) return rc;
+  HIP_TRY(launch_aos_pack(c->stream, c->g, c->aos, d, out, c->type, c->rho, c->df, c->fx, c->fy, c->fz));
+  const size_t layer = (size_t)c->g.ext_x * c->g.ext_y * lay->stride;
+  const size_t first = (size_t)c->g.gz * layer, count = (size_t)c->g.bz * layer;
+  HIP_TRY(hipMemcpyAsync(static_cast<char *>(atoms) + first, c->aos + first, count, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MMD_OK;
+}
+
+extern "C" int mmd_eam_rho_calc_aos(mmd_ctx *c, void *atoms, const mmd_aos_layout *lay, double cutoff) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc || (rc = check_ready(c, "mmd_eam_rho_calc_aos")) || (rc = check_cutoff(c, cutoff, "mmd_eam_rho_calc_aos")))
+      return rc;
+    return aos_pass(c, atoms, lay, MMD_F_X | MMD_F_TYPE, MMD_F_RHO | MMD_F_DF, 0, cutoff);
+  })
+}
+
+extern "C" int mmd_eam_df_calc_aos(mmd_ctx *c, void *atoms, const mmd_aos_layout *lay) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc || (rc = check_ready(c, "mmd_eam_df_calc_aos"))) return rc;
+    return aos_pass(c, atoms, lay, MMD_F_TYPE | MMD_F_RHO, MMD_F_DF, 1, 0.0);
+  })
+}
+
+extern "C" int mmd_eam_force_calc_aos(mmd_ctx *c, void *atoms, const mmd_aos_layout *lay, double cutoff) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc || (rc = check_ready(c, "mmd_eam_force_calc_aos")) ||
+        (rc = check_cutoff(c, cutoff, "mmd_eam_force_calc_aos")))
+      return rc;
+    return aos_pass(c, atoms, lay, MMD_F_X | MMD_F_TYPE | MMD_F_DF, MMD_F_F, 2, cutoff);
+  })
+}
+
+extern "C" int mmd_host_alloc(void **out, size_t bytes) {
+  if (!out) return fail(MMD_E_INVALID, "mmd_host_alloc: null out");
+  *out = nullptr;
+  if (bytes == 0) return MMD_OK;
+  HIP_TRY(hipHostMalloc(out, bytes, hipHostMallocDefault));
+  return MMD_OK;
+}
+
+extern "C" int mmd_host_free(void *p) {
+  if (!p) return MMD_OK;
+  HIP_TRY(hipHostFree(p));
+  return MMD_OK;
+}
+
+// --------------------------------------------------------------- MD step
+
+extern "C" int mmd_md_setup(mmd_ctx *c, const mmd_md_params *p) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc || (rc = check_ready(c, "mmd_md_setup"))) return rc;
+    if (!p || !(p->dt > 0.0) || p->energy_every < 0 || !(p->lattice_const > 0.0))
+      return fail(MMD_E_INVALID, "mmd_md_setup: bad parameters");
+    for (int k = 0; k < 3; k++)
+      if (!(p->box_len[k] > 0.0)) return fail(MMD_E_INVALID, "mmd_md_setup: box length %d must be > 0", k);
+    if ((rc = check_cutoff(c, p->cutoff, "mmd_md_setup"))) return rc;
+    for (int k = 0; k < 3; k++)
+      if (c->dom.start[k] != 0 || c->dom.ghost[k] > c->dom.box[k])
+        return fail(MMD_E_INVALID, "mmd_md_setup: the periodic single-domain step needs start = 0");
+    c->md = *p;
+    c->md_ready = true;
+    ctx_drop_graphs(c);
+    return MMD_OK;
+  })
+}
+
+extern "C" int mmd_ghost_refresh(mmd_ctx *c, uint32_t fields) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc) return rc;
+    if (!c->md_ready) return fail(MMD_E_STATE, "mmd_ghost_refresh: call mmd_md_setup first");
+    if (fields & ~(MMD_F_X | MMD_F_DF)) return fail(MMD_E_INVALID, "mmd_ghost_refresh: only MMD_F_X and MMD_F_DF");
+    return run_ghost(c, fields);
+  })
+}
+
+extern "C" int mmd_md_prime(mmd_ctx *c) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc) return rc;
+    if (!c->md_ready) return fail(MMD_E_STATE, "mmd_md_prime: call mmd_md_setup first");
+    if ((rc = run_ghost(c, MMD_F_X)) || (rc = run_rho_df(c, c->md.cutoff, false)) || (rc = run_ghost(c, MMD_F_DF)) ||
+        (rc = run_force(c, c->md.cutoff, false)))
+      return rc;
+    return MMD_OK;
+  })
+}
+
+extern "C" int mmd_md_run(mmd_ctx *c, int32_t nsteps, double *energies, int32_t use_graph) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc) return rc;
+    if (!c->md_ready) return fail(MMD_E_STATE, "mmd_md_run: call mmd_md_setup first");
+    if (nsteps < 0) return fail(MMD_E_INVALID, "mmd_md_run: nsteps < 0");
+    const int every = c->md.energy_every;
+    const int64_t n_rec = every > 0 ? nsteps / every : 0;
+    if ((rc = ensure_energy_cap(c, std::max<int64_t>(n_rec, 1)))) return rc;
+    HIP_TRY(launch_fill_i32(c->stream, c->d_counter, 1, 0));
+    for (int32_t s = 0; s < nsteps; s++) {
+      const bool want_e = every > 0 && (s + 1) % every == 0;
+      if (use_graph) {
+        hipGraphExec_t &ge = c->graph[want_e ? 1 : 0];
+        if (!ge) {
+          hipGraph_t gr;
+          HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+          c->capturing = true;
+          rc = md_step(c, want_e);
+          c->capturing = false;
+          hipError_t ee = hipStreamEndCapture(c->stream, &gr);
+          if (rc) {
+            if (ee == hipSuccess) (void)hipGraphDestroy(gr);
+            return rc;
+          }
+          HIP_TRY(ee);
+          hipError_t ie = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(gr);
+          HIP_TRY(ie);
+        }
+        HIP_TRY(hipGraphLaunch(ge, c->stream));
+      } else {
+        if ((rc = md_step(c, want_e))) return rc;
+      }
+    }
+    if (n_rec > 0 && energies) {
+      std::vector<double> buf((size_t)(2 * n_rec));
+      HIP_TRY(hipMemcpyAsync(buf.data(), c->d_energy, buf.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      for (int64_t r = 0; r < n_rec; r++) {
+        const int64_t step = (r + 1) * every - 1;
+        energies[2 * step] = buf[2 * r];
+        energies[2 * step + 1] = buf[2 * r + 1];
+      }
+    }
+    return MMD_OK;
+  })
+}
+
+extern "C" int mmd_energy(mmd_ctx *c, double *epot, double *ekin) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc) return rc;
+    if (!c->md_ready) return fail(MMD_E_STATE, "mmd_energy: call mmd_md_setup first");
+    if ((rc = ensure_energy_cap(c, 1))) return rc;
+    HIP_TRY(launch_fill_i32(c->stream, c->d_counter, 1, 0));
+    if ((rc = run_ghost(c, MMD_F_X)) || (rc = run_rho_df(c, c->md.cutoff, true)) || (rc = run_ghost(c, MMD_F_DF)) ||
+        (rc = run_force(c, c->md.cutoff, true)) || (rc = run_kick(c, 0.0, false, true)) ||
+        (rc = run_finalize(c, c->d_energy, c->d_counter)))
+      return rc;
+    double e[2];
+    HIP_TRY(hipMemcpyAsync(e, c->d_energy, sizeof(e), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (epot) *epot = e[0];
+    if (ekin) *ekin = e[1];
+    return MMD_OK;
+  })
+}
+
+// ---------------------------------------------------------------- timing
+
+extern "C" int mmd_timing_enable(mmd_ctx *c, int32_t on) {
+  int rc = ctx_bind(c);
+  if (rc) return rc;
+  c->timing = on != 0;
+  return MMD_OK;
+}
+
+extern "C" int mmd_timing_get(mmd_ctx *c, double *ms_total, int64_t *launches) {
+  int rc = ctx_bind(c);
+  if (rc) return rc;
+  if ((rc = timing_collect(c))) return rc;
+  for (int k = 0; k < MMD_K_COUNT; k++) {
+    if (ms_total) ms_total[k] = c->t_ms[k];
+    if (launches) launches[k] = c->t_n[k];
+  }
+  return MMD_OK;
+}
+
+extern "C" int mmd_timing_reset(mmd_ctx *c) {
+  int rc = ctx_bind(c);
+  if (rc) return rc;
+  if ((rc = timing_collect(c))) return rc;
+  for (int k = 0; k < MMD_K_COUNT; k++) c->t_ms[k] = 0.0, c->t_n[k] = 0;
+  return MMD_OK;
+}

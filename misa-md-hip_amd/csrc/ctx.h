@@ -1,0 +1,109 @@
+// The library context (one sub-domain on one device) and the internal entry
+// points shared between translation units.
+#ifndef MMD_CTX_H
+#define MMD_CTX_H
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "device_common.h"
+#include "mmd_eam.h"
+
+struct mmd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool has_domain = false, has_stencil = false, has_tables = false, md_ready = false;
+  mmd_domain dom{};
+  mmd::Geom g{};
+
+  // device-resident SoA state over the ghost-extended lattice (device layout)
+  double *x = nullptr, *y = nullptr, *z = nullptr;
+  double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+  double *rho = nullptr, *df = nullptr;
+  int8_t *type = nullptr;
+  double *stage = nullptr;  // n_ext doubles: reference-layout staging for upload/download
+
+  // stencil: [2][kMaxStencil] device-layout offsets
+  int32_t *d_off = nullptr;
+  int32_t nk_even = 0, nk_odd = 0;
+
+  // tables
+  double *d_tab = nullptr, *d_mixed = nullptr;
+  mmd::TabF64 tf{};
+  mmd::TabMixed tm{};
+  mmd::Masses masses{};
+  int32_t n_types = 0;
+  int32_t max_type = -1;  // largest atom type uploaded (checked against n_types before every pass)
+  double r_max = 0.0;
+  int32_t prec = MMD_PREC_FP64;
+  int32_t mixed_knots = 0;  // 0: as many r^2 knots as r knots
+
+  // energy partials (one per site block) and the per-step record
+  double *e_rho = nullptr, *e_force = nullptr, *e_ke = nullptr;
+  double *d_energy = nullptr;
+  int64_t energy_cap = 0;
+  int32_t *d_counter = nullptr;
+
+  // MD
+  mmd_md_params md{};
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  bool capturing = false;
+
+  // drop-in AoS staging
+  char *aos = nullptr;
+  size_t aos_bytes = 0;
+
+  // timing
+  struct Pending {
+    int cls;
+    hipEvent_t start, stop;
+  };
+  bool timing = false;
+  std::vector<hipEvent_t> event_pool;
+  std::vector<Pending> pending;
+  double t_ms[MMD_K_COUNT] = {};
+  int64_t t_n[MMD_K_COUNT] = {};
+
+  // interstitials (off-lattice atoms), see itl.hip
+  int32_t n_itl = 0, itl_cap = 0, n_wide = 0;
+  double *ix = nullptr, *iy = nullptr, *iz = nullptr;
+  double *ivx = nullptr, *ivy = nullptr, *ivz = nullptr;
+  double *ifx = nullptr, *ify = nullptr, *ifz = nullptr;
+  double *irho = nullptr, *idf = nullptr;
+  int8_t *itype = nullptr;
+  int32_t *d_wide = nullptr;       // [n_wide][3] wide stencil, parity 0
+  int64_t *li_site = nullptr;      // [cap * n_wide] lattice site of each candidate pair (device index), -1 = none
+  int64_t *li_key_sorted = nullptr;
+  int32_t *li_val = nullptr, *li_val_sorted = nullptr;
+  double *li_c = nullptr;          // per-pair lattice-side contributions (rho or force x,y,z)
+  int32_t *li_count = nullptr;     // number of valid pairs
+  void *sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  int32_t *cell_start = nullptr;   // interstitial bins
+  int32_t *itl_order = nullptr;
+  double *e_itl = nullptr;
+  int64_t itl_part_n = 0;
+};
+
+namespace mmd {
+
+int ctx_bind(mmd_ctx *c);
+void ctx_drop_graphs(mmd_ctx *c);
+int timed_begin(mmd_ctx *c, int cls, hipEvent_t *ev);
+int timed_end(mmd_ctx *c, int cls, hipEvent_t start);
+int run_rho_df(mmd_ctx *c, double cutoff, bool want_e);
+int run_force(mmd_ctx *c, double cutoff, bool want_e);
+int run_ghost(mmd_ctx *c, uint32_t fields);
+
+// interstitial passes (itl.hip): added on top of the lattice passes
+int itl_rho_df(mmd_ctx *c, double cutoff, bool want_e);
+int itl_df(mmd_ctx *c);
+int itl_force(mmd_ctx *c, double cutoff, bool want_e);
+int itl_kick(mmd_ctx *c, double h, bool drift, bool want_ke);
+void itl_free(mmd_ctx *c);
+
+}  // namespace mmd
+
+#endif

@@ -1,0 +1,287 @@
+// gfx950 kernels around the EAM passes: layout conversion, host AoS record
+// pack/unpack, periodic ghost-shell refresh, velocity Verlet and the
+// deterministic energy reduction.  All are HBM-bound streaming kernels.
+#include "device_common.h"
+#include "kernels.h"
+
+namespace mmd {
+
+namespace {
+
+constexpr double kFtm2v = 9.64853321e-3;  // (eV/A/amu) -> A/fs^2
+constexpr double kMvv2e = 1.0 / kFtm2v;   // amu A^2/fs^2 -> eV
+
+struct ExtPos {
+  int32_t s, xh, ye, ze;
+};
+
+// device index -> ext coordinates (32-bit arithmetic: n_ext < 2^31 is checked at set-up)
+__device__ __forceinline__ ExtPos ext_of(const Geom &g, uint32_t idx) {
+  ExtPos p;
+  const uint32_t row = idx / (uint32_t)g.exh;
+  p.xh = (int32_t)(idx - row * (uint32_t)g.exh);
+  p.s = (int32_t)(row & 1u);
+  const uint32_t yz = row >> 1;
+  const uint32_t ze = yz / (uint32_t)g.ext_y;
+  p.ye = (int32_t)(yz - ze * (uint32_t)g.ext_y);
+  p.ze = (int32_t)ze;
+  return p;
+}
+
+__device__ __forceinline__ int64_t ref_of(const Geom &g, const ExtPos &p) {
+  return ((int64_t)p.ze * g.ext_y + p.ye) * g.ext_x + 2 * p.xh + p.s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_ref2dev(Geom g, const T *__restrict__ ref, T *__restrict__ dev) {
+  const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= (uint32_t)g.n_ext) return;
+  dev[idx] = ref[ref_of(g, ext_of(g, idx))];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dev2ref(Geom g, const T *__restrict__ dev, T *__restrict__ ref) {
+  const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= (uint32_t)g.n_ext) return;
+  ref[ref_of(g, ext_of(g, idx))] = dev[idx];
+}
+
+__global__ __launch_bounds__(kBlock) void k_aos_unpack(Geom g, const char *__restrict__ aos, AosDesc d,
+                                                         uint32_t fields, double *__restrict__ X,
+                                                         double *__restrict__ Y, double *__restrict__ Z,
+                                                         int8_t *__restrict__ T, double *__restrict__ DF,
+                                                         double *__restrict__ RHO) {
+  const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= (uint32_t)g.n_ext) return;
+  const char *rec = aos + ref_of(g, ext_of(g, idx)) * d.stride;
+  if (fields & 0x20u) {
+    int32_t t;
+    if (d.type_bytes == 1)
+      t = *reinterpret_cast<const int8_t *>(rec + d.off_type);
+    else if (d.type_bytes == 2)
+      t = *reinterpret_cast<const int16_t *>(rec + d.off_type);
+    else
+      t = *reinterpret_cast<const int32_t *>(rec + d.off_type);
+    T[idx] = (int8_t)(t < 0 ? -1 : t);
+  }
+  if (fields & 0x01u) {
+    const double *x = reinterpret_cast<const double *>(rec + d.off_x);
+    X[idx] = x[0];
+    Y[idx] = x[1];
+    Z[idx] = x[2];
+  }
+  if (fields & 0x10u) DF[idx] = *reinterpret_cast<const double *>(rec + d.off_df);
+  if (fields & 0x08u) RHO[idx] = *reinterpret_cast<const double *>(rec + d.off_rho);
+}
+
+// writes results of local sites with type >= 0 only (vacancies untouched)
+__global__ __launch_bounds__(kBlock) void k_aos_pack(Geom g, char *__restrict__ aos, AosDesc d, uint32_t fields,
+                                                       const int8_t *__restrict__ T, const double *__restrict__ RHO,
+                                                       const double *__restrict__ DF, const double *__restrict__ FX,
+                                                       const double *__restrict__ FY, const double *__restrict__ FZ) {
+  Site st;
+  if (!map_site(g, st) || !st.active) return;
+  const int64_t i = st.i;
+  if (T[i] < 0) return;
+  ExtPos p;
+  {
+    const int64_t row = i / g.exh;
+    p.xh = (int32_t)(i - row * g.exh);
+    p.s = (int32_t)(row & 1);
+    const int64_t yz = row >> 1;
+    p.ze = (int32_t)(yz / g.ext_y);
+    p.ye = (int32_t)(yz - (int64_t)p.ze * g.ext_y);
+  }
+  char *rec = aos + ref_of(g, p) * d.stride;
+  if (fields & 0x08u) *reinterpret_cast<double *>(rec + d.off_rho) = RHO[i];
+  if (fields & 0x10u) *reinterpret_cast<double *>(rec + d.off_df) = DF[i];
+  if (fields & 0x04u) {
+    double *f = reinterpret_cast<double *>(rec + d.off_f);
+    f[0] = FX[i];
+    f[1] = FY[i];
+    f[2] = FZ[i];
+  }
+}
+
+__device__ __forceinline__ int32_t wrap_img(int32_t v, int32_t n, int32_t &img) {
+  if (v < 0) {
+    img = -1;
+    return v + n;
+  }
+  if (v >= n) {
+    img = 1;
+    return v - n;
+  }
+  img = 0;
+  return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_ghost(Geom g, double Lx, double Ly, double Lz, double *__restrict__ X,
+                                                    double *__restrict__ Y, double *__restrict__ Z,
+                                                    int8_t *__restrict__ T, double *__restrict__ DF,
+                                                    uint32_t fields) {
+  const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= (uint32_t)g.n_ext) return;
+  const ExtPos p = ext_of(g, idx);
+  const int32_t xl = p.xh - g.gxh, yl = p.ye - g.gy, zl = p.ze - g.gz;
+  if (xl >= 0 && xl < g.bxh && yl >= 0 && yl < g.by && zl >= 0 && zl < g.bz) return;
+  int32_t ix, iy, iz;
+  const int32_t sx = wrap_img(xl, g.bxh, ix), sy = wrap_img(yl, g.by, iy), sz = wrap_img(zl, g.bz, iz);
+  const int64_t src = dev_index(g, p.s, sx + g.gxh, sy + g.gy, sz + g.gz);
+  if (fields & 0x01u) {
+    T[idx] = T[src];
+    X[idx] = X[src] + (double)ix * Lx;
+    Y[idx] = Y[src] + (double)iy * Ly;
+    Z[idx] = Z[src] + (double)iz * Lz;
+  }
+  if (fields & 0x10u) DF[idx] = DF[src];
+}
+
+template <bool DRIFT, bool WANT_KE>
+__global__ __launch_bounds__(kBlock) void k_kick(Geom g, Masses M, double h, double dt, double *__restrict__ X,
+                                                   double *__restrict__ Y, double *__restrict__ Z,
+                                                   double *__restrict__ VX, double *__restrict__ VY,
+                                                   double *__restrict__ VZ, const double *__restrict__ FX,
+                                                   const double *__restrict__ FY, const double *__restrict__ FZ,
+                                                   const int8_t *__restrict__ T, double *__restrict__ ke_part) {
+  __shared__ double lds[kWavesPerBlock];
+  double ke = 0.0;
+  Site st;
+  if (map_site(g, st) && st.active) {
+    const int64_t i = st.i;
+    const int t = T[i];
+    if (t >= 0) {
+      const double sc = h * kFtm2v * M.inv_m[t];
+      const double vx = VX[i] + sc * FX[i], vy = VY[i] + sc * FY[i], vz = VZ[i] + sc * FZ[i];
+      VX[i] = vx;
+      VY[i] = vy;
+      VZ[i] = vz;
+      if (DRIFT) {
+        X[i] += dt * vx;
+        Y[i] += dt * vy;
+        Z[i] += dt * vz;
+      }
+      if (WANT_KE) ke = 0.5 * M.m[t] * (vx * vx + vy * vy + vz * vz) * kMvv2e;
+    }
+  }
+  if (WANT_KE) {
+    const double b = block_sum(ke, lds);
+    if (threadIdx.x == 0) ke_part[xcd_block()] = b;
+  }
+}
+
+__device__ double ordered_sum(const double *__restrict__ p, int64_t n) {
+  double s = 0.0;
+  for (int64_t k = threadIdx.x; k < n; k += kBlock) s += p[k];
+  return s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_energy_finalize(const double *__restrict__ ep_a, int64_t na,
+                                                              const double *__restrict__ ep_b, int64_t nb,
+                                                              const double *__restrict__ ep_c, int64_t nc,
+                                                              const double *__restrict__ ke, int64_t nk,
+                                                              double *__restrict__ out, int32_t *__restrict__ counter) {
+  __shared__ double lds[kWavesPerBlock];
+  double pe = 0.0;
+  if (ep_a) pe += ordered_sum(ep_a, na);
+  if (ep_b) pe += ordered_sum(ep_b, nb);
+  if (ep_c) pe += ordered_sum(ep_c, nc);
+  const double kin = ke ? ordered_sum(ke, nk) : 0.0;
+  const double PE = block_sum(pe, lds);
+  const double KE = block_sum(kin, lds);
+  if (threadIdx.x == 0) {
+    const int32_t slot = counter[0];
+    out[2 * slot + 0] = PE;
+    out[2 * slot + 1] = KE;
+    counter[0] = slot + 1;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill(T *__restrict__ p, int64_t n, T v) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+inline dim3 grid_ext(const Geom &g) { return dim3((unsigned)((g.n_ext + kBlock - 1) / kBlock)); }
+
+}  // namespace
+
+hipError_t launch_ref2dev_f64(hipStream_t st, const Geom &g, const double *ref, double *dev) {
+  k_ref2dev<double><<<grid_ext(g), kBlock, 0, st>>>(g, ref, dev);
+  return hipGetLastError();
+}
+hipError_t launch_dev2ref_f64(hipStream_t st, const Geom &g, const double *dev, double *ref) {
+  k_dev2ref<double><<<grid_ext(g), kBlock, 0, st>>>(g, dev, ref);
+  return hipGetLastError();
+}
+hipError_t launch_ref2dev_i8(hipStream_t st, const Geom &g, const int8_t *ref, int8_t *dev) {
+  k_ref2dev<int8_t><<<grid_ext(g), kBlock, 0, st>>>(g, ref, dev);
+  return hipGetLastError();
+}
+hipError_t launch_dev2ref_i8(hipStream_t st, const Geom &g, const int8_t *dev, int8_t *ref) {
+  k_dev2ref<int8_t><<<grid_ext(g), kBlock, 0, st>>>(g, dev, ref);
+  return hipGetLastError();
+}
+
+hipError_t launch_aos_unpack(hipStream_t st, const Geom &g, const char *aos, AosDesc d, uint32_t fields, double *X,
+                             double *Y, double *Z, int8_t *T, double *DF, double *RHO) {
+  k_aos_unpack<<<grid_ext(g), kBlock, 0, st>>>(g, aos, d, fields, X, Y, Z, T, DF, RHO);
+  return hipGetLastError();
+}
+
+hipError_t launch_aos_pack(hipStream_t st, const Geom &g, char *aos, AosDesc d, uint32_t fields, const int8_t *T,
+                           const double *RHO, const double *DF, const double *FX, const double *FY,
+                           const double *FZ) {
+  k_aos_pack<<<dim3((unsigned)site_blocks(g)), kBlock, 0, st>>>(g, aos, d, fields, T, RHO, DF, FX, FY, FZ);
+  return hipGetLastError();
+}
+
+hipError_t launch_ghost(hipStream_t st, const Geom &g, double Lx, double Ly, double Lz, double *X, double *Y,
+                        double *Z, int8_t *T, double *DF, uint32_t fields) {
+  k_ghost<<<grid_ext(g), kBlock, 0, st>>>(g, Lx, Ly, Lz, X, Y, Z, T, DF, fields);
+  return hipGetLastError();
+}
+
+hipError_t launch_kick(hipStream_t st, const Geom &g, const Masses &M, double h, double dt, bool drift, double *X,
+                       double *Y, double *Z, double *VX, double *VY, double *VZ, const double *FX, const double *FY,
+                       const double *FZ, const int8_t *T, double *ke_part) {
+  const dim3 grid((unsigned)site_blocks(g));
+  if (drift) {
+    if (ke_part)
+      k_kick<true, true><<<grid, kBlock, 0, st>>>(g, M, h, dt, X, Y, Z, VX, VY, VZ, FX, FY, FZ, T, ke_part);
+    else
+      k_kick<true, false><<<grid, kBlock, 0, st>>>(g, M, h, dt, X, Y, Z, VX, VY, VZ, FX, FY, FZ, T, ke_part);
+  } else {
+    if (ke_part)
+      k_kick<false, true><<<grid, kBlock, 0, st>>>(g, M, h, dt, X, Y, Z, VX, VY, VZ, FX, FY, FZ, T, ke_part);
+    else
+      k_kick<false, false><<<grid, kBlock, 0, st>>>(g, M, h, dt, X, Y, Z, VX, VY, VZ, FX, FY, FZ, T, ke_part);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_energy_finalize(hipStream_t st, const double *ep_a, int64_t na, const double *ep_b, int64_t nb,
+                                  const double *ep_c, int64_t nc, const double *ke, int64_t nk, double *out,
+                                  int32_t *counter) {
+  k_energy_finalize<<<1, kBlock, 0, st>>>(ep_a, na, ep_b, nb, ep_c, nc, ke, nk, out, counter);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(hipStream_t st, int32_t *p, int64_t n, int32_t v) {
+  if (n <= 0) return hipSuccess;
+  k_fill<int32_t><<<dim3((unsigned)((n + kBlock - 1) / kBlock)), kBlock, 0, st>>>(p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_fill_f64(hipStream_t st, double *p, int64_t n, double v) {
+  if (n <= 0) return hipSuccess;
+  k_fill<double><<<dim3((unsigned)((n + kBlock - 1) / kBlock)), kBlock, 0, st>>>(p, n, v);
+  return hipGetLastError();
+}
+hipError_t launch_fill_i8(hipStream_t st, int8_t *p, int64_t n, int8_t v) {
+  if (n <= 0) return hipSuccess;
+  k_fill<int8_t><<<dim3((unsigned)((n + kBlock - 1) / kBlock)), kBlock, 0, st>>>(p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace mmd

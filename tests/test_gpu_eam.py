@@ -1,0 +1,300 @@
+"""GPU parity of the EAM hot path (rho + fused df, df, force), the drop-in
+host-array hooks, the ghost refresh and the device MD step against the CPU
+oracle, through the C ABI.
+
+Tolerances: the FP64 path reproduces the oracle's arithmetic (same spline,
+same stencil order); the device contracts multiply-adds into FMAs, so results
+agree to rounding: rho, df relative <= 1e-12, forces absolute <= 1e-11 of the
+largest force component.  The mixed path is held to the stated target: force
+error <= 1e-6 of the RMS force (per atom, max) and normwise.  Integer fields
+(types, vacancy flags) are compared bit-exactly.
+"""
+import numpy as np
+import pytest
+
+import mmd
+import oracle as O
+import workload as W
+
+pytestmark = pytest.mark.gpu
+
+A, RC = W.A, W.RC
+
+
+def _ctx(c, prec=mmd.PREC_FP64, fields=mmd.F_X | mmd.F_TYPE | mmd.F_V):
+    ctx = mmd.Context(0)
+    ctx.set_domain(c["dom"])
+    ctx.set_stencil(c["even"], c["odd"])
+    ctx.set_tables(c["tab"], c["masses"])
+    ctx.set_precision(prec)
+    ctx.upload(c["st"], fields)
+    return ctx
+
+
+def _oracle_pass(c):
+    ref = c["st"].copy()
+    T = O.Tables(c["tab"], c["masses"])
+    O.rho_df(c["dom"], c["even"], c["odd"], T, RC, ref)
+    O.ghost_df(c["dom"], ref)
+    O.force(c["dom"], c["even"], c["odd"], T, RC, ref)
+    return ref, T
+
+
+CASES = {
+    "bcc20_1type": dict(cells=(20, 20, 20), n_types=1, sigma=0.03, temperature=600.0, seed=1),
+    "bcc12_3type_vac": dict(cells=(12, 12, 12), n_types=3, sigma=0.05, temperature=800.0, vacancy_frac=0.01,
+                            seed=2),
+    "ragged_7x9x5": dict(cells=(7, 9, 5), n_types=3, sigma=0.05, temperature=1200.0, vacancy_frac=0.02, seed=3),
+    "wide_row_70x3x4": dict(cells=(70, 3, 4), n_types=2, sigma=0.05, temperature=800.0, seed=4, ghost_cells=2),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_rho_df_force_fp64_match_oracle(name):
+    c = W.case(**CASES[name])
+    ref, _ = _oracle_pass(c)
+    with _ctx(c) as ctx:
+        ctx.rho_df(RC)
+        out = c["st"].copy()
+        ctx.download(out, mmd.F_RHO | mmd.F_DF | mmd.F_TYPE)
+        # force from the device's own df with a device ghost refresh of df
+        ctx.md_setup(c["box"], A, 1.0, RC)
+        ctx.ghost_refresh(mmd.F_DF)
+        ctx.force(RC)
+        ctx.download(out, mmd.F_F | mmd.F_DF)
+    li = out.local_index()
+    assert np.array_equal(out.type, c["st"].type)
+    occ = li[out.type[li] >= 0]
+    assert np.allclose(out.rho[occ], ref.rho[occ], rtol=1e-12, atol=0)
+    assert np.allclose(out.df[occ], ref.df[occ], rtol=1e-12, atol=1e-15)
+    assert np.allclose(out.df, ref.df, rtol=1e-12, atol=1e-15)  # ghost df refreshed like the oracle's
+    F = np.stack([out.fx[occ], out.fy[occ], out.fz[occ]], 1)
+    Fr = np.stack([ref.fx[occ], ref.fy[occ], ref.fz[occ]], 1)
+    assert np.abs(F - Fr).max() <= 1e-11 * np.abs(Fr).max()
+    vac = li[out.type[li] < 0]
+    assert np.all(out.rho[vac] == 0) and np.all(out.fx[vac] == 0)  # vacancies untouched
+
+
+def test_standalone_df_matches_oracle():
+    c = W.case(**CASES["bcc12_3type_vac"])
+    ref, T = _oracle_pass(c)
+    st = c["st"].copy()
+    st.rho[:] = ref.rho
+    with _ctx(c, fields=mmd.F_TYPE | mmd.F_RHO) as ctx:
+        ctx.upload(st, mmd.F_RHO)
+        ctx.df()
+        out = st.copy()
+        ctx.download(out, mmd.F_DF)
+    li = out.local_index()
+    occ = li[out.type[li] >= 0]
+    assert np.allclose(out.df[occ], ref.df[occ], rtol=1e-13, atol=1e-15)
+
+
+def test_perfect_lattice_on_device():
+    c = W.case(cells=(16, 16, 16), sigma=0.0, temperature=0.0)
+    with _ctx(c) as ctx:
+        ctx.rho_df(RC)
+        ctx.md_setup(c["box"], A, 1.0, RC)
+        ctx.ghost_refresh(mmd.F_DF)
+        ctx.force(RC)
+        out = c["st"].copy()
+        ctx.download(out, mmd.F_RHO | mmd.F_F)
+    li = out.local_index()
+    shells = [(8, np.sqrt(3) / 2 * A), (6, A), (12, np.sqrt(2) * A), (24, np.sqrt(11) / 2 * A), (8, np.sqrt(3) * A)]
+    rho_cf = sum(n * (RC - r) ** 2 * np.exp(-r) for n, r in shells)
+    assert np.allclose(out.rho[li], rho_cf, rtol=1e-12)
+    assert np.abs(np.stack([out.fx[li], out.fy[li], out.fz[li]])).max() < 1e-12
+
+
+def test_results_bit_reproducible():
+    c = W.case(**CASES["bcc12_3type_vac"])
+    outs = []
+    with _ctx(c) as ctx:
+        for _ in range(2):
+            ctx.rho_df(RC)
+            ctx.force(RC)
+            o = c["st"].copy()
+            ctx.download(o, mmd.F_RHO | mmd.F_F)
+            outs.append(o)
+    assert np.array_equal(outs[0].rho, outs[1].rho) and np.array_equal(outs[0].fz, outs[1].fz)
+
+
+def test_dropin_aos_hooks_match_oracle():
+    """hip_eam_rho_calc / hip_eam_force_calc contract: host AoS records in,
+    rho/df (or f) of local records out, everything else untouched."""
+    c = W.case(**CASES["ragged_7x9x5"])
+    ref, _ = _oracle_pass(c)
+    st = c["st"]
+    n = st.x.size
+    recs = np.zeros(n, mmd.REF_AOS_DTYPE)
+    recs["id"] = np.arange(n)
+    recs["type"] = st.type
+    recs["x"] = np.stack([st.x, st.y, st.z], 1)
+    recs["v"] = np.stack([st.vx, st.vy, st.vz], 1)
+    recs["rho"] = -7.0
+    recs["df"] = -7.0
+    before = recs.copy()
+    with _ctx(c) as ctx:
+        ctx.rho_calc_aos(recs, RC)
+        li = st.local_index()
+        occ = li[st.type[li] >= 0]
+        assert np.allclose(recs["rho"][occ], ref.rho[occ], rtol=1e-12)
+        assert np.allclose(recs["df"][occ], ref.df[occ], rtol=1e-12)
+        mask = np.ones(n, bool)
+        mask[occ] = False
+        assert np.all(recs["rho"][mask] == -7.0)  # ghosts and vacancies untouched
+        for f in ("id", "type", "x", "v", "f"):
+            assert np.array_equal(recs[f], before[f])
+        # the caller refreshes ghost df between the hooks (MISA-MD core does it over MPI)
+        recs["df"] = ref.df
+        ctx.force_calc_aos(recs, RC)
+        F = recs["f"][occ]
+        Fr = np.stack([ref.fx[occ], ref.fy[occ], ref.fz[occ]], 1)
+        assert np.abs(F - Fr).max() <= 1e-11 * np.abs(Fr).max()
+        # standalone df hook on the records' rho
+        recs["df"] = 0.0
+        ctx.df_calc_aos(recs)
+        assert np.allclose(recs["df"][occ], ref.df[occ], rtol=1e-13)
+
+
+def test_ghost_refresh_matches_oracle():
+    c = W.case(**CASES["ragged_7x9x5"])
+    st = c["st"].copy()
+    li = st.local_index()
+    scrambled = st.copy()
+    mask = np.ones(st.x.size, bool)
+    mask[li] = False
+    scrambled.x[mask] = 99.0
+    scrambled.type[mask] = 5
+    with _ctx(c) as ctx:
+        ctx.upload(scrambled, mmd.F_X | mmd.F_TYPE)
+        ctx.md_setup(c["box"], A, 1.0, RC)
+        ctx.ghost_refresh(mmd.F_X)
+        out = st.copy()
+        ctx.download(out, mmd.F_X | mmd.F_TYPE)
+    ref = scrambled.copy()
+    O.ghost_xt(c["dom"], c["box"], ref)
+    assert np.array_equal(out.type, ref.type)
+    assert np.array_equal(out.x, ref.x) and np.array_equal(out.y, ref.y) and np.array_equal(out.z, ref.z)
+
+
+def _prime(ctx, c, energy_every=0):
+    ctx.md_setup(c["box"], A, 1.0, RC, energy_every=energy_every)
+    ctx.md_prime()
+
+
+def test_md_steps_match_oracle():
+    c = W.case(cells=(10, 10, 10), n_types=3, sigma=0.05, temperature=800.0, vacancy_frac=0.01, seed=9)
+    ref, T = _oracle_pass(c)
+    nsteps = 10
+    en_ref = O.md_run(c["dom"], c["even"], c["odd"], T, RC, c["box"], 1.0, nsteps, ref)
+    with _ctx(c) as ctx:
+        _prime(ctx, c, energy_every=1)
+        en = np.zeros((nsteps, 2))
+        ctx.md_run(nsteps, en)
+        out = c["st"].copy()
+        ctx.download(out)
+    li = out.local_index()
+    occ = li[out.type[li] >= 0]
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        assert np.abs(getattr(out, f)[occ] - getattr(ref, f)[occ]).max() < 1e-10, f
+    assert np.allclose(en, en_ref, rtol=1e-12, atol=0)
+    n_at = len(occ)
+    assert np.abs(en - en_ref).max() / n_at < 1e-12
+
+
+def test_graph_replay_equals_eager():
+    c = W.case(cells=(8, 9, 10), n_types=3, sigma=0.05, temperature=1200.0, vacancy_frac=0.01, seed=12)
+    outs = []
+    for use_graph in (False, True):
+        with _ctx(c) as ctx:
+            _prime(ctx, c, energy_every=2)
+            en = np.zeros((6, 2))
+            ctx.md_run(3, en[:3], use_graph=use_graph)
+            ctx.sync()
+            ctx.md_run(3, en[3:], use_graph=use_graph)  # second call replays the captured graphs
+            o = c["st"].copy()
+            ctx.download(o)
+            outs.append((o, en))
+    (a, ea), (b, eb) = outs
+    for f in ("x", "vx", "fx", "rho", "df"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    assert np.array_equal(ea, eb)
+
+
+def test_energy_entry_point_matches_oracle():
+    c = W.case(**CASES["bcc12_3type_vac"])
+    st = c["st"]
+    T = O.Tables(c["tab"], c["masses"])
+    ref = st.copy()
+    e1 = O.rho_df(c["dom"], c["even"], c["odd"], T, RC, ref, want_e=True)
+    O.ghost_df(c["dom"], ref)
+    e2 = O.force(c["dom"], c["even"], c["odd"], T, RC, ref, want_e=True)
+    ke = O.kinetic(c["dom"], T, ref)
+    with _ctx(c) as ctx:
+        ctx.md_setup(c["box"], A, 1.0, RC)
+        ep, ek = ctx.energy()
+    assert abs(ep - (e1 + e2)) < 1e-12 * abs(e1 + e2)
+    assert abs(ek - ke) < 1e-12 * ke
+
+
+def _force_err(F, Fr):
+    d = np.linalg.norm(F - Fr, axis=1)
+    rms = np.sqrt((np.linalg.norm(Fr, axis=1) ** 2).mean())
+    return np.linalg.norm(F - Fr) / np.linalg.norm(Fr), d.max() / rms
+
+
+@pytest.mark.parametrize("name", ["bcc12_3type_vac", "ragged_7x9x5"])
+def test_mixed_precision_within_target(name):
+    c = W.case(**CASES[name])
+    res = {}
+    for prec in (mmd.PREC_FP64, mmd.PREC_MIXED):
+        with _ctx(c, prec=prec) as ctx:
+            ctx.md_setup(c["box"], A, 1.0, RC)
+            ctx.rho_df(RC)
+            ctx.ghost_refresh(mmd.F_DF)
+            ctx.force(RC)
+            o = c["st"].copy()
+            ctx.download(o, mmd.F_RHO | mmd.F_DF | mmd.F_F)
+            res[prec] = (o, ctx.energy())
+    a, b = res[mmd.PREC_FP64][0], res[mmd.PREC_MIXED][0]
+    li = a.local_index()
+    occ = li[a.type[li] >= 0]
+    assert np.allclose(b.rho[occ], a.rho[occ], rtol=1e-9)
+    Fa = np.stack([a.fx[occ], a.fy[occ], a.fz[occ]], 1)
+    Fb = np.stack([b.fx[occ], b.fy[occ], b.fz[occ]], 1)
+    normwise, per_atom = _force_err(Fb, Fa)
+    assert normwise <= 1e-6 and per_atom <= 1e-6, (normwise, per_atom)
+    (ea, _), (eb, _) = res[mmd.PREC_FP64][1], res[mmd.PREC_MIXED][1]
+    assert abs(ea - eb) / len(occ) < 1e-9
+
+
+def test_error_paths():
+    c = W.case(cells=(4, 4, 4))
+    ctx = mmd.Context(0)
+    with pytest.raises(mmd.MmdError) as e:
+        ctx.set_stencil(c["even"], c["odd"])
+    assert e.value.code == mmd.E_STATE
+    ctx.set_domain(c["dom"])
+    with pytest.raises(mmd.MmdError) as e:
+        ctx.rho_df(RC)
+    assert e.value.code == mmd.E_STATE
+    bad = c["even"].copy()
+    bad[0] = 10 ** 7
+    with pytest.raises(mmd.MmdError) as e:
+        ctx.set_stencil(bad, c["odd"])
+    assert e.value.code == mmd.E_RANGE
+    ctx.set_stencil(c["even"], c["odd"])
+    tab = dict(c["tab"])
+    tab["phi_r"] = tab["phi_r"].copy()
+    ctx.set_tables(tab, c["masses"])
+    with pytest.raises(mmd.MmdError) as e:
+        ctx.rho_df(7.0)  # beyond the tabulated range
+    assert e.value.code == mmd.E_INVALID
+    t3 = mmd.synth_tables(n_types=2, jitter=0.1)
+    t3["phi_r"] = t3["phi_r"].copy()
+    t3["phi_r"][0, 1, 10] += 1.0  # asymmetric pair table
+    with pytest.raises(mmd.MmdError) as e:
+        ctx.set_tables(t3, c["masses"])
+    assert e.value.code == mmd.E_INVALID
+    ctx.close()

@@ -1,0 +1,125 @@
+"""Shared workload builders for the tests (lattice, tables, stencils).
+
+Everything here calls the product's host helpers (stencil generator, synthetic
+tables, initial conditions) and numpy; device calls live in the GPU tests.
+"""
+import numpy as np
+
+import mmd
+
+A = 2.855
+RC = 5.6
+SKIN = 0.3
+
+
+def case(cells=(6, 6, 6), n_types=1, sigma=0.03, temperature=600.0, vacancy_frac=0.0, interstitial_frac=0.0,
+         seed=7, ghost_cells=3, jitter=None, n_r=5000, n_rho=5000, n_blobs=0, blob_sigma=8.0, skin=SKIN):
+    spec = mmd.lattice_spec(cells, ghost_cells=ghost_cells, a=A, n_types=n_types, sigma=sigma,
+                            temperature=temperature, vacancy_frac=vacancy_frac, interstitial_frac=interstitial_frac,
+                            n_blobs=n_blobs, blob_sigma=blob_sigma, seed=seed)
+    cap = int(interstitial_frac * 2 * np.prod(cells)) + 16
+    dom, box, st, itl = mmd.lattice_init(spec, itl_cap=cap)
+    tab = mmd.synth_tables(n_types=n_types, rc=RC, jitter=(0.1 if n_types > 1 else 0.0) if jitter is None else jitter,
+                           seed=seed, n_r=n_r, n_rho=n_rho)
+    st3 = mmd.stencil_build(A, RC + skin, 0)
+    even = mmd.stencil_linear(dom, st3)
+    odd = mmd.stencil_linear(dom, -st3)
+    wide3 = mmd.wide_stencil(A, RC, skin)
+    return dict(dom=dom, box=box, st=st, itl=itl, tab=tab, st3=st3, even=even, odd=odd, wide3=wide3,
+                masses=mmd.DEFAULT_MASSES, spec=spec)
+
+
+def local_vectors(st, idx=None):
+    li = st.local_index() if idx is None else idx
+    return li, np.stack([st.fx[li], st.fy[li], st.fz[li]], 1)
+
+
+# -------------------------------------------------- numpy restatement of the spline
+
+def np_spline(f, dx):
+    """Cubic Hermite with 4th-order finite-difference slopes (LAMMPS pair_eam
+    interpolate), vectorised independently of the C oracle."""
+    f = np.asarray(f, float)
+    n = len(f)
+    s = np.empty(n)
+    s[0] = f[1] - f[0]
+    s[1] = 0.5 * (f[2] - f[0])
+    s[n - 2] = 0.5 * (f[n - 1] - f[n - 3])
+    s[n - 1] = f[n - 1] - f[n - 2]
+    s[2:n - 2] = ((f[0:n - 4] - f[4:n]) + 8.0 * (f[3:n - 1] - f[1:n - 3])) / 12.0
+    d = f[1:] - f[:-1]
+    c4 = np.zeros(n)
+    c3 = np.zeros(n)
+    c4[:-1] = 3.0 * d - 2.0 * s[:-1] - s[1:]
+    c3[:-1] = s[:-1] + s[1:] - 2.0 * d
+    return dict(f=f, s=s, c4=c4, c3=c3, d2=s / dx, d1=2.0 * c4 / dx, d0=3.0 * c3 / dx, n=n)
+
+
+def np_seg(sp, x0, inv_dx, x):
+    p = (np.asarray(x, float) - x0) * inv_dx
+    p = np.maximum(p, 0.0)
+    m = np.minimum(p.astype(np.int64), sp["n"] - 2)
+    p = np.minimum(p - m, 1.0)
+    return m, p
+
+
+def np_value(sp, x0, inv_dx, x):
+    m, p = np_seg(sp, x0, inv_dx, x)
+    return ((sp["c3"][m] * p + sp["c4"][m]) * p + sp["s"][m]) * p + sp["f"][m]
+
+
+def np_deriv(sp, x0, inv_dx, x):
+    m, p = np_seg(sp, x0, inv_dx, x)
+    return (sp["d0"][m] * p + sp["d1"][m]) * p + sp["d2"][m]
+
+
+class NpTables:
+    def __init__(self, tab):
+        nt = tab["n_types"]
+        self.nt = nt
+        self.r0, self.idr = tab["r0"], 1.0 / tab["dr"]
+        self.p0, self.idp = tab["rho0"], 1.0 / tab["drho"]
+        self.rho = [np_spline(tab["rho_r"][t], tab["dr"]) for t in range(nt)]
+        self.emb = [np_spline(tab["emb"][t], tab["drho"]) for t in range(nt)]
+        self.phi = [[np_spline(tab["phi_r"][a][b], tab["dr"]) for b in range(nt)] for a in range(nt)]
+
+
+def brute_force(pos, types, L, nt_tab, rc):
+    """O(N^2) minimum-image EAM: rho, df, forces, total potential energy."""
+    T = nt_tab
+    pos = np.asarray(pos, float)
+    types = np.asarray(types, int)
+    N = len(pos)
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= L * np.round(d / L)
+    r2 = (d ** 2).sum(-1)
+    mask = (r2 < rc * rc) & ~np.eye(N, dtype=bool)
+    r = np.sqrt(np.where(mask, r2, 1.0))
+    rho = np.zeros(N)
+    for t in range(T.nt):
+        sel = mask & (types[None, :] == t)
+        rho += np.where(sel, np_value(T.rho[t], T.r0, T.idr, r), 0.0).sum(1)
+    df = np.zeros(N)
+    Femb = np.zeros(N)
+    for t in range(T.nt):
+        s = types == t
+        df[s] = np_deriv(T.emb[t], T.p0, T.idp, rho[s])
+        Femb[s] = np_value(T.emb[t], T.p0, T.idp, rho[s])
+    rhop = np.zeros((T.nt, N, N))
+    for t in range(T.nt):
+        rhop[t] = np_deriv(T.rho[t], T.r0, T.idr, r)
+    ti, tj = types[:, None] * np.ones((1, N), int), np.ones((N, 1), int) * types[None, :]
+    phip = np.zeros((N, N))
+    phiv = np.zeros((N, N))
+    for a in range(T.nt):
+        for b in range(T.nt):
+            sel = (ti == a) & (tj == b)
+            phip = np.where(sel, np_deriv(T.phi[a][b], T.r0, T.idr, r), phip)
+            phiv = np.where(sel, np_value(T.phi[a][b], T.r0, T.idr, r), phiv)
+    rhojp = np.take_along_axis(rhop.transpose(1, 2, 0), tj[:, :, None], 2)[:, :, 0]
+    rhoip = np.take_along_axis(rhop.transpose(1, 2, 0), ti[:, :, None], 2)[:, :, 0]
+    psip = df[:, None] * rhojp + df[None, :] * rhoip + phip
+    fpair = np.where(mask, -psip / r, 0.0)
+    F = (d * fpair[:, :, None]).sum(1)
+    E = Femb.sum() + 0.5 * np.where(mask, phiv, 0.0).sum()
+    return rho, df, F, E
