@@ -352,9 +352,9 @@ extern "C" int mmd_set_stencil(mmd_ctx *c, const int64_t *even, int32_t n_even, 
 
 // Mixed-precision r^2-keyed tables resampled from the FP64 r-keyed spline:
 // knot values and exact derivatives of g(sqrt(s)) at uniform s, cubic Hermite
-// in s; FP64 leading terms, FP32 correction polynomials.
+// in s; FP64 base and linear terms, FP32 quadratic and cubic terms.
 static void build_mixed(const double *c7, int n_r, double r0, double inv_dr, int n_s, double s0, double ds,
-                        double *vrec /*[n_s][4]*/, double *drec /*[n_s][2]*/) {
+                        double *vrec /*[n_s][4]*/, double *drec /*[n_s][4]*/) {
   std::vector<double> h(n_s), hp(n_s);
   for (int i = 0; i < n_s; i++) {
     const double s = s0 + ds * (double)i;
@@ -371,14 +371,16 @@ static void build_mixed(const double *c7, int n_r, double r0, double inv_dr, int
       a2 = 3.0 * D - 2.0 * a1 - a1n;
       a3 = a1 + a1n - 2.0 * D;
     }
-    float fa[4] = {(float)a1, (float)a2, (float)a3, 0.0f};
+    const float fa[2] = {(float)a2, (float)a3};
     vrec[4 * m + 0] = h[m];
-    std::memcpy(&vrec[4 * m + 1], &fa[0], 8);
-    std::memcpy(&vrec[4 * m + 2], &fa[2], 8);
+    vrec[4 * m + 1] = a1;
+    std::memcpy(&vrec[4 * m + 2], fa, 8);
     vrec[4 * m + 3] = 0.0;
-    float fb[2] = {(float)(2.0 * a2 / ds), (float)(3.0 * a3 / ds)};
-    drec[2 * m + 0] = hp[m];
-    std::memcpy(&drec[2 * m + 1], fb, 8);
+    const float fb[2] = {(float)(3.0 * a3 / ds), 0.0f};
+    drec[4 * m + 0] = hp[m];
+    drec[4 * m + 1] = 2.0 * a2 / ds;
+    std::memcpy(&drec[4 * m + 2], fb, 8);
+    drec[4 * m + 3] = 0.0;
   }
 }
 
@@ -422,7 +424,7 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
     const int n_s = c->mixed_knots > 0 ? c->mixed_knots : nr;
     const double rmax = t->r0 + t->dr * (double)(nr - 1);
     const double s0 = t->r0 * t->r0, ds = (rmax * rmax - s0) / (double)(n_s - 1);
-    const size_t vwords = (size_t)n_s * 4, dwords = (size_t)n_s * 2;
+    const size_t vwords = (size_t)n_s * 4, dwords = (size_t)n_s * 4;
     std::vector<double> mixed(n_fun_r * (vwords + dwords), 0.0);
     double *m_rho_v = mixed.data();
     double *m_phi_v = m_rho_v + (size_t)nt * vwords;
@@ -454,9 +456,11 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
     TabMixed &tm = c->tm;
     tm.rho_v = reinterpret_cast<const double4 *>(c->d_mixed);
     tm.phi_v = reinterpret_cast<const double4 *>(c->d_mixed + (m_phi_v - m_rho_v));
-    tm.rho_d = reinterpret_cast<const double2 *>(c->d_mixed + (m_rho_d - m_rho_v));
-    tm.phi_d = reinterpret_cast<const double2 *>(c->d_mixed + (m_phi_d - m_rho_v));
+    tm.rho_d = reinterpret_cast<const double4 *>(c->d_mixed + (m_rho_d - m_rho_v));
+    tm.phi_d = reinterpret_cast<const double4 *>(c->d_mixed + (m_phi_d - m_rho_v));
     tm.nt = nt, tm.n_s = n_s, tm.s0 = s0, tm.inv_ds = 1.0 / ds;
+    const double r_head = t->r0 + 3.0 * t->dr, r_tail = t->r0 + (double)(nr - 4) * t->dr;
+    tm.s_head = r_head * r_head, tm.s_tail = r_tail * r_tail;
     for (int a = 0; a < kMaxTypes; a++) {
       c->masses.m[a] = a < nt ? t->mass[a] : 1.0;
       c->masses.inv_m[a] = 1.0 / c->masses.m[a];

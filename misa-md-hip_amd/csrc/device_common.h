@@ -44,19 +44,25 @@ struct TabF64 {
   double r0, inv_dr, rho0, inv_drho;
 };
 
-// Mixed-precision spline keyed by s = r^2: per knot an FP64 base value and the
-// FP32 correction polynomial in the segment fraction q:
-//   value = base + q * (a1 + q * (a2 + q * a3))          (FP32 bracket)
-//   d/ds  = dbase + q * (b1 + q * b2)                    (FP32 bracket)
-// Value record (32 B): {double base; float a1, a2; float a3, 0; double 0}
-// Deriv record (16 B): {double dbase; float b1, b2}
+// Mixed-precision spline keyed by s = r^2 (no square root or division per
+// pair).  Per knot, with q the fraction of the segment:
+//   value = base + q * (a1 + q * c),   c = a2 + q * a3   (c in FP32)
+//   d/ds  = dbase + q * (b1 + q * b2)                     (b2 stored as FP32)
+// The FP64 base and linear terms carry the leading digits; the FP32 terms are
+// O(ds^2) corrections, so their rounding stays below 1e-13 relative.
+// Value record (32 B): {double base; double a1; float a2, a3; double 0}
+// Deriv record (32 B): {double dbase; double b1; float b2, 0; double 0}
 struct TabMixed {
   const double4 *rho_v;   // [nt][n_s] value records (rho by source type)
-  const double2 *rho_d;   // [nt][n_s] derivative records
-  const double2 *phi_d;   // [nt*nt][n_s]
+  const double4 *rho_d;   // [nt][n_s] derivative records
+  const double4 *phi_d;   // [nt*nt][n_s]
   const double4 *phi_v;   // [nt*nt][n_s]
   int32_t nt, n_s;
   double s0, inv_ds;
+  // The reference spline's first and last three segments use low-order end
+  // slopes, which leave second-derivative kinks a uniform s grid cannot follow:
+  // pairs with s < s_head or s >= s_tail evaluate the FP64 r-spline instead.
+  double s_head, s_tail;
 };
 
 struct Masses {

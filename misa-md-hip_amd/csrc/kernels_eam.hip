@@ -42,7 +42,7 @@ __device__ __forceinline__ double der8(const double *__restrict__ tab, Seg s) {
 
 struct SegS {
   int m;
-  float q;
+  double q;
 };
 
 __device__ __forceinline__ SegS seg_s(double s2, const TabMixed &t) {
@@ -51,24 +51,21 @@ __device__ __forceinline__ SegS seg_s(double s2, const TabMixed &t) {
   int m = min((int)p, t.n_s - 2);
   SegS r;
   r.m = m;
-  r.q = (float)fmin(p - (double)m, 1.0);
+  r.q = fmin(p - (double)m, 1.0);
   return r;
 }
 
 __device__ __forceinline__ double valS(const double4 *__restrict__ tab, SegS s) {
   const double4 rec = tab[s.m];
-  // rec.x = FP64 base; rec.y, rec.z carry the three FP32 coefficients
-  const float2 a12 = __builtin_bit_cast(float2, rec.y);
-  const float2 a3p = __builtin_bit_cast(float2, rec.z);
-  const float corr = s.q * fmaf(s.q, fmaf(s.q, a3p.x, a12.y), a12.x);
-  return rec.x + (double)corr;
+  const float2 a23 = __builtin_bit_cast(float2, rec.z);
+  const float c = fmaf((float)s.q, a23.y, a23.x);  // FP32 quadratic/cubic correction
+  return fma(s.q, fma(s.q, (double)c, rec.y), rec.x);
 }
 
-__device__ __forceinline__ double derS(const double2 *__restrict__ tab, SegS s) {
-  const double2 rec = tab[s.m];
-  const float2 b = __builtin_bit_cast(float2, rec.y);
-  const float corr = s.q * fmaf(s.q, b.y, b.x);
-  return rec.x + (double)corr;
+__device__ __forceinline__ double derS(const double4 *__restrict__ tab, SegS s) {
+  const double4 rec = tab[s.m];
+  const float2 b2 = __builtin_bit_cast(float2, rec.z);
+  return fma(s.q, fma(s.q, (double)b2.x, rec.y), rec.x);
 }
 
 // ------------------------------------------------------------------ rho
@@ -79,6 +76,7 @@ __device__ __forceinline__ double rho_pair(const TabF64 &tf, const TabMixed &tm,
     const Seg sg = seg_f64(sqrt(d2), tf.r0, tf.inv_dr, tf.n_r);
     return val8(tf.rho + (size_t)tj * tf.n_r * 8, sg);
   } else {
+    if (d2 >= tm.s_tail || d2 < tm.s_head) return rho_pair<0>(tf, tm, tj, d2);
     const SegS sg = seg_s(d2, tm);
     return valS(tm.rho_v + (size_t)tj * tm.n_s, sg);
   }
@@ -151,6 +149,7 @@ __device__ __forceinline__ double fpair(const TabF64 &tf, const TabMixed &tm, in
     const double psip = dfi * rhojp + dfj * rhoip + phip;
     return -psip / r;
   } else {
+    if (d2 >= tm.s_tail || d2 < tm.s_head) return fpair<0, WANT_E>(tf, tm, ti, tj, d2, dfi, dfj, phi);
     const SegS sg = seg_s(d2, tm);
     const size_t pr = (size_t)(ti * tm.nt + tj) * tm.n_s;
     const double rhojp = derS(tm.rho_d + (size_t)tj * tm.n_s, sg);

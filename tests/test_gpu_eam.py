@@ -165,7 +165,7 @@ def test_ghost_refresh_matches_oracle():
     mask = np.ones(st.x.size, bool)
     mask[li] = False
     scrambled.x[mask] = 99.0
-    scrambled.type[mask] = 5
+    scrambled.type[mask] = -1
     with _ctx(c) as ctx:
         ctx.upload(scrambled, mmd.F_X | mmd.F_TYPE)
         ctx.md_setup(c["box"], A, 1.0, RC)
