@@ -168,8 +168,10 @@ static int run_kick(mmd_ctx *c, double h, bool drift, bool want_ke) {
 
 static int run_finalize(mmd_ctx *c, double *out, int32_t *counter) {
   const int64_t nb = site_blocks(c->g);
-  HIP_TRY(launch_energy_finalize(c->stream, c->e_rho, nb, c->e_force, nb, c->n_itl > 0 ? c->e_itl : nullptr,
-                                 c->n_itl > 0 ? c->itl_part_n : 0, c->e_ke, nb, out, counter));
+  const bool itl = c->n_itl > 0;
+  HIP_TRY(launch_energy_finalize(c->stream, c->e_rho, nb, c->e_force, nb, itl ? c->e_itl : nullptr,
+                                 itl ? c->itl_part_n : 0, c->e_ke, nb, itl ? c->e_itl + c->itl_part_n : nullptr,
+                                 itl ? c->itl_kick_n : 0, out, counter));
   return MMD_OK;
 }
 
@@ -316,9 +318,10 @@ extern "C" int mmd_set_stencil(mmd_ctx *c, const int64_t *even, int32_t n_even, 
     int rc = ctx_bind(c);
     if (rc) return rc;
     if (!c->has_domain) return fail(MMD_E_STATE, "mmd_set_stencil: set the domain first");
-    if (n_even < 0 || n_odd < 0 || n_even > kMaxStencil || n_odd > kMaxStencil || (n_even && !even) ||
-        (n_odd && !odd))
-      return fail(MMD_E_INVALID, "mmd_set_stencil: sizes must be in [0, %d]", kMaxStencil);
+    // rows are padded with zero offsets so unrolled pair loops may read past nk
+    const int32_t cap = kMaxStencil - 8;
+    if (n_even < 0 || n_odd < 0 || n_even > cap || n_odd > cap || (n_even && !even) || (n_odd && !odd))
+      return fail(MMD_E_RANGE, "mmd_set_stencil: sizes must be in [0, %d]", cap);
     const mmd_domain &d = c->dom;
     std::vector<int32_t> host(2 * kMaxStencil, 0);
     for (int p = 0; p < 2; p++) {
@@ -527,6 +530,7 @@ extern "C" int mmd_upload(mmd_ctx *c, uint32_t fields, const mmd_host_soa *h) {
       HIP_TRY(hipMemcpyAsync(c->stage, f.src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
       HIP_TRY(launch_ref2dev_f64(c->stream, c->g, c->stage, f.dst));
     }
+    if (fields & MMD_F_X) c->itl_stale = true;
     if (fields & MMD_F_TYPE) {
       if (!h->type) return fail(MMD_E_INVALID, "mmd_upload: type pointer missing");
       int32_t mx;
@@ -804,6 +808,7 @@ extern "C" int mmd_md_run(mmd_ctx *c, int32_t nsteps, double *energies, int32_t 
         if ((rc = md_step(c, want_e))) return rc;
       }
     }
+    if ((rc = itl_check(c))) return rc;
     if (n_rec > 0 && energies) {
       std::vector<double> buf((size_t)(2 * n_rec));
       HIP_TRY(hipMemcpyAsync(buf.data(), c->d_energy, buf.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -830,6 +835,7 @@ extern "C" int mmd_energy(mmd_ctx *c, double *epot, double *ekin) {
         (rc = run_force(c, c->md.cutoff, true)) || (rc = run_kick(c, 0.0, false, true)) ||
         (rc = run_finalize(c, c->d_energy, c->d_counter)))
       return rc;
+    if ((rc = itl_check(c))) return rc;
     double e[2];
     HIP_TRY(hipMemcpyAsync(e, c->d_energy, sizeof(e), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -73,18 +73,11 @@ struct mmd_ctx {
   double *ifx = nullptr, *ify = nullptr, *ifz = nullptr;
   double *irho = nullptr, *idf = nullptr;
   int8_t *itype = nullptr;
-  int32_t *d_wide = nullptr;       // [n_wide][3] wide stencil, parity 0
-  int64_t *li_site = nullptr;      // [cap * n_wide] lattice site of each candidate pair (device index), -1 = none
-  int64_t *li_key_sorted = nullptr;
-  int32_t *li_val = nullptr, *li_val_sorted = nullptr;
-  double *li_c = nullptr;          // per-pair lattice-side contributions (rho or force x,y,z)
-  int32_t *li_count = nullptr;     // number of valid pairs
-  void *sort_tmp = nullptr;
-  size_t sort_tmp_bytes = 0;
-  int32_t *cell_start = nullptr;   // interstitial bins
-  int32_t *itl_order = nullptr;
-  double *e_itl = nullptr;
-  int64_t itl_part_n = 0;
+  int32_t *d_wide = nullptr;  // [n_wide][3] wide stencil W, parity 0
+  double *e_itl = nullptr;    // [itl_part_n] energy partials, then kinetic partials
+  int64_t itl_part_n = 0, itl_kick_n = 0;
+  bool itl_stale = true;      // anchors / neighbour lists predate the current positions
+  void *itl_state = nullptr;  // device work arrays (itl.hip)
 };
 
 namespace mmd {
@@ -103,6 +96,7 @@ int itl_df(mmd_ctx *c);
 int itl_force(mmd_ctx *c, double cutoff, bool want_e);
 int itl_kick(mmd_ctx *c, double h, bool drift, bool want_ke);
 void itl_free(mmd_ctx *c);
+int itl_check(mmd_ctx *c);
 
 }  // namespace mmd
 

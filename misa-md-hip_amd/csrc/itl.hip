@@ -1,27 +1,639 @@
-// Interstitial (off-lattice atom) passes.  Placeholder: the list can be set
-// only empty until the device passes are in place.
+// Interstitial (off-lattice atom) passes on the device.
+//
+// The reference plugin skips every off-lattice atom (src/kernels/kernel_itl.hpp:44-46,
+// :52-54); MISA-MD core keeps them in a separate list.  Here the list is
+// device resident and the passes run on top of the lattice passes:
+//   * interstitial k sees the lattice sites of the wide stencil W around its
+//     anchor (nearest lattice site of its position, wrapped into the periodic
+//     box) and every other interstitial within rc (minimum image);
+//   * lattice<->interstitial pairs are {(i,k): site(i) in W(anchor k),
+//     type_i >= 0, r_ik < rc}; a ghost site stands for its periodic source.
+// Determinism without atomics on physical quantities: each affected lattice
+// site has one owner, the lowest-numbered interstitial interacting with it,
+// whose lane adds every interstitial contribution to that site in increasing
+// interstitial order.  Interstitial neighbour lists (pairs closer than 2 rc,
+// the only interstitials that can share a lattice site) come from periodic
+// bins filled with atomics and are then sorted, so their order is fixed.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "ctx.h"
 #include "host_helpers.h"
+#include "kernels.h"
 
 namespace mmd {
-int itl_rho_df(mmd_ctx *, double, bool) { return fail(MMD_E_STATE, "interstitial passes not available"); }
-int itl_df(mmd_ctx *) { return fail(MMD_E_STATE, "interstitial passes not available"); }
-int itl_force(mmd_ctx *, double, bool) { return fail(MMD_E_STATE, "interstitial passes not available"); }
-int itl_kick(mmd_ctx *, double, bool, bool) { return fail(MMD_E_STATE, "interstitial passes not available"); }
-void itl_free(mmd_ctx *) {}
-}  // namespace mmd
 
-extern "C" int mmd_itl_set(mmd_ctx *c, int32_t n, const double *, const double *, const double *, const double *,
-                           const double *, const double *, const int8_t *, const int32_t *, int32_t, int32_t) {
-  if (!c) return mmd::fail(MMD_E_INVALID, "null context");
-  if (n != 0) return mmd::fail(MMD_E_STATE, "mmd_itl_set: interstitial passes not available");
-  c->n_itl = 0;
+constexpr int kItlNbrCap = 32;  // interstitials within 2 rc of one interstitial
+constexpr int kBinCap = 64;     // interstitials per bin
+
+struct ItlDev {
+  const double *x, *y, *z;  // positions as stored (unwrapped)
+  double *fx, *fy, *fz, *rho, *df;
+  const int8_t *type;
+  int32_t *ax, *ay, *az;        // anchor site, local lattice coordinates (x doubled)
+  double *px, *py, *pz;         // positions shifted by the same periodic image as the anchor
+  int32_t *nbr, *nnb;           // [n][kItlNbrCap] sorted neighbour ids, counts
+  int32_t *bin_count, *bin_list;
+  int32_t *err;
+  const int32_t *wide;          // [n_wide][3], parity 0
+  int32_t n, n_wide, max_d2q;
+  int32_t nbx, nby, nbz;
+  double bsx, bsy, bsz;
+  double Lx, Ly, Lz, a, rc2, rc2x4;
+};
+
+namespace {
+
+constexpr double kFtm2v = 9.64853321e-3;
+constexpr double kMvv2e = 1.0 / kFtm2v;
+
+__device__ __forceinline__ double spl_val(const double *tab, int rec, double p) {
+  const double *c = tab + (size_t)rec * 8;
+  return ((c[0] * p + c[1]) * p + c[2]) * p + c[3];
+}
+
+__device__ __forceinline__ double spl_der(const double *tab, int rec, double p) {
+  const double *c = tab + (size_t)rec * 8 + 4;
+  return (c[0] * p + c[1]) * p + c[2];
+}
+
+__device__ __forceinline__ int seg(double x, double x0, double inv_dx, int n, double &p) {
+  double q = (x - x0) * inv_dx;
+  q = fmax(q, 0.0);
+  int m = min((int)q, n - 2);
+  p = fmin(q - (double)m, 1.0);
+  return m;
+}
+
+__device__ __forceinline__ int wrap_n(int v, int n) {
+  v %= n;
+  return v < 0 ? v + n : v;
+}
+
+// centred periodic difference in [-n/2, n/2)
+__device__ __forceinline__ int cdiff(int v, int n) {
+  v = wrap_n(v, n);
+  return v >= n - n / 2 ? v - n : v;
+}
+
+__device__ __forceinline__ double minimg(double d, double L) { return d - L * rint(d / L); }
+
+// Is lattice site (sx, sy, sz) (local, x doubled) in W of interstitial m and
+// closer than rc?  Returns the ext device index of the image seen from m.
+__device__ __forceinline__ bool li_pair(const Geom &g, const ItlDev &I, const double *X, const double *Y,
+                                        const double *Z, int m, int sx, int sy, int sz, int64_t &e, double &d2,
+                                        double &dx, double &dy, double &dz) {
+  const int aX = I.ax[m], aY = I.ay[m], aZ = I.az[m];
+  const int dX = cdiff(sx - aX, 2 * g.bxh), dY = cdiff(sy - aY, g.by), dZ = cdiff(sz - aZ, g.bz);
+  const int sa = aX & 1, s1 = (aX + dX) & 1;
+  const int hy = 2 * dY + s1 - sa, hz = 2 * dZ + s1 - sa;
+  if (dX * dX + hy * hy + hz * hz > I.max_d2q) return false;
+  const int xe = aX + dX + 2 * g.gxh, ye = aY + dY + g.gy, ze = aZ + dZ + g.gz;
+  e = dev_index(g, xe & 1, xe >> 1, ye, ze);
+  dx = X[e] - I.px[m];
+  dy = Y[e] - I.py[m];
+  dz = Z[e] - I.pz[m];
+  d2 = dx * dx + dy * dy + dz * dz;
+  return d2 < I.rc2;
+}
+
+// anchor (nearest lattice site), wrapped into the box, and bins
+__global__ __launch_bounds__(kBlock) void k_itl_prepare(Geom g, ItlDev I) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= I.n) return;
+  double px = I.x[k], py = I.y[k], pz = I.z[k];
+  const double ux = px / I.a, uy = py / I.a, uz = pz / I.a;
+  const double cx = floor(ux + 0.5), cy = floor(uy + 0.5), cz = floor(uz + 0.5);
+  const double bx = floor(ux) + 0.5, by = floor(uy) + 0.5, bz = floor(uz) + 0.5;
+  const double dc = (ux - cx) * (ux - cx) + (uy - cy) * (uy - cy) + (uz - cz) * (uz - cz);
+  const double db = (ux - bx) * (ux - bx) + (uy - by) * (uy - by) + (uz - bz) * (uz - bz);
+  long long X, Y, Z;
+  if (dc <= db) {
+    X = 2 * (long long)cx, Y = (long long)cy, Z = (long long)cz;
+  } else {
+    X = 2 * (long long)floor(ux) + 1, Y = (long long)floor(uy), Z = (long long)floor(uz);
+  }
+  const long long nX = 2 * g.bxh, nY = g.by, nZ = g.bz;
+  long long q = X / nX - (X % nX < 0 ? 1 : 0);
+  X -= q * nX, px -= (double)q * I.Lx;
+  q = Y / nY - (Y % nY < 0 ? 1 : 0);
+  Y -= q * nY, py -= (double)q * I.Ly;
+  q = Z / nZ - (Z % nZ < 0 ? 1 : 0);
+  Z -= q * nZ, pz -= (double)q * I.Lz;
+  I.ax[k] = (int)X, I.ay[k] = (int)Y, I.az[k] = (int)Z;
+  I.px[k] = px, I.py[k] = py, I.pz[k] = pz;
+  // bin of the wrapped position (positions may sit a little outside [0, L))
+  const double wx = px - I.Lx * floor(px / I.Lx), wy = py - I.Ly * floor(py / I.Ly), wz = pz - I.Lz * floor(pz / I.Lz);
+  const int ibx = min((int)(wx / I.bsx), I.nbx - 1), iby = min((int)(wy / I.bsy), I.nby - 1),
+            ibz = min((int)(wz / I.bsz), I.nbz - 1);
+  const int b = (ibz * I.nby + iby) * I.nbx + ibx;
+  const int slot = atomicAdd(&I.bin_count[b], 1);
+  if (slot < kBinCap)
+    I.bin_list[b * kBinCap + slot] = k;
+  else
+    atomicOr(I.err, 1);
+}
+
+__device__ __forceinline__ int bin_of(const ItlDev &I, double p, double L, double bs, int nb) {
+  const double w = p - L * floor(p / L);
+  return min((int)(w / bs), nb - 1);
+}
+
+// sorted list of interstitials closer than 2 rc (minimum image)
+__global__ __launch_bounds__(kBlock) void k_itl_nbr(ItlDev I) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= I.n) return;
+  const int bx = bin_of(I, I.px[k], I.Lx, I.bsx, I.nbx), by = bin_of(I, I.py[k], I.Ly, I.bsy, I.nby),
+            bz = bin_of(I, I.pz[k], I.Lz, I.bsz, I.nbz);
+  int list[kItlNbrCap];
+  int cnt = 0;
+  const int rx0 = I.nbx >= 3 ? -1 : 0, rx1 = I.nbx >= 2 ? 1 : 0;
+  const int ry0 = I.nby >= 3 ? -1 : 0, ry1 = I.nby >= 2 ? 1 : 0;
+  const int rz0 = I.nbz >= 3 ? -1 : 0, rz1 = I.nbz >= 2 ? 1 : 0;
+  for (int oz = rz0; oz <= rz1; oz++)
+    for (int oy = ry0; oy <= ry1; oy++)
+      for (int ox = rx0; ox <= rx1; ox++) {
+        const int b = (wrap_n(bz + oz, I.nbz) * I.nby + wrap_n(by + oy, I.nby)) * I.nbx + wrap_n(bx + ox, I.nbx);
+        const int nb = min(I.bin_count[b], kBinCap);
+        for (int q = 0; q < nb; q++) {
+          const int m = I.bin_list[b * kBinCap + q];
+          if (m == k) continue;
+          const double dx = minimg(I.px[k] - I.px[m], I.Lx), dy = minimg(I.py[k] - I.py[m], I.Ly),
+                       dz = minimg(I.pz[k] - I.pz[m], I.Lz);
+          if (dx * dx + dy * dy + dz * dz >= I.rc2x4) continue;
+          if (cnt == kItlNbrCap) {
+            atomicOr(I.err, 2);
+            continue;
+          }
+          int pos = cnt++;
+          while (pos > 0 && list[pos - 1] > m) {
+            list[pos] = list[pos - 1];
+            pos--;
+          }
+          list[pos] = m;
+        }
+      }
+  I.nnb[k] = cnt;
+  for (int q = 0; q < cnt; q++) I.nbr[k * kItlNbrCap + q] = list[q];
+}
+
+// local lattice coordinates (x doubled) of the periodic source of ext site (xe, ye, ze)
+__device__ __forceinline__ void src_local(const Geom &g, int xe, int ye, int ze, int &sx, int &sy, int &sz) {
+  sx = wrap_n(xe - 2 * g.gxh, 2 * g.bxh);
+  sy = wrap_n(ye - g.gy, g.by);
+  sz = wrap_n(ze - g.gz, g.bz);
+}
+
+// One wave per interstitial.  MODE 0: rho (+ owner-side lattice rho/df update),
+// MODE 1: force (+ owner-side lattice force update).
+template <int MODE, bool WANT_E>
+__global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const double *__restrict__ X,
+                                                       const double *__restrict__ Y, const double *__restrict__ Z,
+                                                       const int8_t *__restrict__ T, double *__restrict__ RHO,
+                                                       double *__restrict__ DF, double *__restrict__ FX,
+                                                       double *__restrict__ FY, double *__restrict__ FZ, TabF64 tf,
+                                                       double *__restrict__ e_part) {
+  __shared__ double lds[kWavesPerBlock];
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  double acc = 0.0, fxk = 0.0, fyk = 0.0, fzk = 0.0, e_loc = 0.0;
+  if (k < I.n) {
+    const int tk = I.type[k];
+    const int aX = I.ax[k], aY = I.ay[k], aZ = I.az[k];
+    const int sgn = (aX & 1) ? -1 : 1;
+    const int nn = I.nnb[k];
+    const int *nb = I.nbr + k * kItlNbrCap;
+    const double dfk = MODE == 1 ? I.df[k] : 0.0;
+    for (int w = lane; w < I.n_wide; w += kWave) {
+      const int xe = aX + sgn * I.wide[3 * w] + 2 * g.gxh, ye = aY + sgn * I.wide[3 * w + 1] + g.gy,
+                ze = aZ + sgn * I.wide[3 * w + 2] + g.gz;
+      const int64_t e = dev_index(g, xe & 1, xe >> 1, ye, ze);
+      const int ti = T[e];
+      if (ti < 0) continue;
+      const double dx = X[e] - I.px[k], dy = Y[e] - I.py[k], dz = Z[e] - I.pz[k];
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 >= I.rc2) continue;
+      const double r = sqrt(d2);
+      double p;
+      const int m = seg(r, tf.r0, tf.inv_dr, tf.n_r, p);
+      if (MODE == 0) {
+        acc += spl_val(tf.rho, ti * tf.n_r + m, p);
+      } else {
+        const double dfi = DF[e];
+        const double psip = dfi * spl_der(tf.rho, tk * tf.n_r + m, p) + dfk * spl_der(tf.rho, ti * tf.n_r + m, p) +
+                            spl_der(tf.phi, (ti * tf.nt + tk) * tf.n_r + m, p);
+        const double fp = -psip / r;  // force on the lattice atom along del = x_i - x_k
+        fxk -= dx * fp, fyk -= dy * fp, fzk -= dz * fp;
+      }
+      // owner: no lower-numbered neighbour interacts with this lattice site
+      int sx, sy, sz;
+      src_local(g, xe, ye, ze, sx, sy, sz);
+      bool owner = true;
+      for (int q = 0; q < nn && owner; q++) {
+        const int mm = nb[q];
+        if (mm > k) break;
+        int64_t e2;
+        double d2b, ex, ey, ez;
+        if (li_pair(g, I, X, Y, Z, mm, sx, sy, sz, e2, d2b, ex, ey, ez)) owner = false;
+      }
+      if (!owner) continue;
+      const int64_t src = dev_index(g, sx & 1, (sx >> 1) + g.gxh, sy + g.gy, sz + g.gz);
+      double rho_i = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, epair = 0.0;
+      if (MODE == 0) rho_i = RHO[src];
+      const double dfi = MODE == 1 ? DF[src] : 0.0;
+      // every interacting interstitial in increasing order (k merged into its neighbours)
+      int placed_k = 0;
+      for (int q = 0; q < nn + 1; q++) {
+        int mm;
+        if (!placed_k && (q - placed_k >= nn || nb[q - placed_k] > k)) {
+          mm = k;
+          placed_k = 1;
+        } else {
+          mm = nb[q - placed_k];
+        }
+        int64_t e2;
+        double d2b, ex, ey, ez;
+        if (!li_pair(g, I, X, Y, Z, mm, sx, sy, sz, e2, d2b, ex, ey, ez)) continue;
+        const int tm = I.type[mm];
+        const double rb = sqrt(d2b);
+        double pb;
+        const int mb = seg(rb, tf.r0, tf.inv_dr, tf.n_r, pb);
+        if (MODE == 0) {
+          rho_i += spl_val(tf.rho, tm * tf.n_r + mb, pb);
+        } else {
+          const double psip = dfi * spl_der(tf.rho, tm * tf.n_r + mb, pb) +
+                              I.df[mm] * spl_der(tf.rho, ti * tf.n_r + mb, pb) +
+                              spl_der(tf.phi, (ti * tf.nt + tm) * tf.n_r + mb, pb);
+          const double fp = -psip / rb;
+          gx += ex * fp, gy += ey * fp, gz += ez * fp;
+          if (WANT_E) epair += spl_val(tf.phi, (ti * tf.nt + tm) * tf.n_r + mb, pb);
+        }
+      }
+      if (MODE == 0) {
+        double pe;
+        const int me = seg(rho_i, tf.rho0, tf.inv_drho, tf.n_rho, pe);
+        if (WANT_E) {
+          double po;
+          const int mo = seg(RHO[src], tf.rho0, tf.inv_drho, tf.n_rho, po);
+          e_loc += spl_val(tf.emb, ti * tf.n_rho + me, pe) - spl_val(tf.emb, ti * tf.n_rho + mo, po);
+        }
+        RHO[src] = rho_i;
+        DF[src] = spl_der(tf.emb, ti * tf.n_rho + me, pe);
+      } else {
+        FX[src] += gx, FY[src] += gy, FZ[src] += gz;
+        if (WANT_E) e_loc += epair;
+      }
+    }
+    // interstitial <-> interstitial
+    for (int q = lane; q < nn; q += kWave) {
+      const int mm = nb[q];
+      const double dx = minimg(I.px[k] - I.px[mm], I.Lx), dy = minimg(I.py[k] - I.py[mm], I.Ly),
+                   dz = minimg(I.pz[k] - I.pz[mm], I.Lz);
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (d2 >= I.rc2) continue;
+      const int tm = I.type[mm];
+      const double r = sqrt(d2);
+      double p;
+      const int m = seg(r, tf.r0, tf.inv_dr, tf.n_r, p);
+      if (MODE == 0) {
+        acc += spl_val(tf.rho, tm * tf.n_r + m, p);
+      } else {
+        const double psip = dfk * spl_der(tf.rho, tm * tf.n_r + m, p) + I.df[mm] * spl_der(tf.rho, tk * tf.n_r + m, p) +
+                            spl_der(tf.phi, (tk * tf.nt + tm) * tf.n_r + m, p);
+        const double fp = -psip / r;
+        fxk += dx * fp, fyk += dy * fp, fzk += dz * fp;
+        if (WANT_E) e_loc += 0.5 * spl_val(tf.phi, (tk * tf.nt + tm) * tf.n_r + m, p);
+      }
+    }
+    if (MODE == 0) {
+      acc = wave_sum(acc);
+      if (lane == 0) {
+        double pe;
+        const int me = seg(acc, tf.rho0, tf.inv_drho, tf.n_rho, pe);
+        I.rho[k] = acc;
+        I.df[k] = spl_der(tf.emb, tk * tf.n_rho + me, pe);
+        if (WANT_E) e_loc += spl_val(tf.emb, tk * tf.n_rho + me, pe);
+      }
+    } else {
+      fxk = wave_sum(fxk), fyk = wave_sum(fyk), fzk = wave_sum(fzk);
+      if (lane == 0) I.fx[k] = fxk, I.fy[k] = fyk, I.fz[k] = fzk;
+    }
+  }
+  if (WANT_E) {
+    const double b = block_sum(e_loc, lds);
+    if (threadIdx.x == 0) e_part[blockIdx.x] = b;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_itl_df(ItlDev I, TabF64 tf) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= I.n) return;
+  double p;
+  const int m = seg(I.rho[k], tf.rho0, tf.inv_drho, tf.n_rho, p);
+  I.df[k] = spl_der(tf.emb, I.type[k] * tf.n_rho + m, p);
+}
+
+template <bool DRIFT, bool WANT_KE>
+__global__ __launch_bounds__(kBlock) void k_itl_kick(ItlDev I, double *vx, double *vy, double *vz, double *x,
+                                                       double *y, double *z, Masses M, double h, double dt,
+                                                       double *ke_part) {
+  __shared__ double lds[kWavesPerBlock];
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  double ke = 0.0;
+  if (k < I.n) {
+    const int t = I.type[k];
+    const double sc = h * kFtm2v * M.inv_m[t];
+    const double a = vx[k] + sc * I.fx[k], b = vy[k] + sc * I.fy[k], c = vz[k] + sc * I.fz[k];
+    vx[k] = a, vy[k] = b, vz[k] = c;
+    if (DRIFT) x[k] += dt * a, y[k] += dt * b, z[k] += dt * c;
+    if (WANT_KE) ke = 0.5 * M.m[t] * (a * a + b * b + c * c) * kMvv2e;
+  }
+  if (WANT_KE) {
+    const double s = block_sum(ke, lds);
+    if (threadIdx.x == 0) ke_part[blockIdx.x] = s;
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- host
+
+#define ITL_TRY(expr)                                                                                       \
+  do {                                                                                                      \
+    hipError_t e_ = (expr);                                                                                 \
+    if (e_ != hipSuccess)                                                                                   \
+      return fail(MMD_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+struct ItlState {
+  int32_t *ax = nullptr, *ay = nullptr, *az = nullptr, *nbr = nullptr, *nnb = nullptr;
+  double *px = nullptr, *py = nullptr, *pz = nullptr;
+  int32_t *bins = nullptr;  // counts followed by lists
+  int32_t *err = nullptr;
+  int32_t nbx = 0, nby = 0, nbz = 0, max_d2q = 0;
+};
+
+static ItlState &state(mmd_ctx *c) {
+  static_assert(sizeof(ItlState) > 0, "");
+  if (!c->itl_state) c->itl_state = new ItlState();
+  return *static_cast<ItlState *>(c->itl_state);
+}
+
+template <typename T>
+static void dfree_t(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+void itl_free(mmd_ctx *c) {
+  dfree_t(c->ix), dfree_t(c->iy), dfree_t(c->iz), dfree_t(c->ivx), dfree_t(c->ivy), dfree_t(c->ivz);
+  dfree_t(c->ifx), dfree_t(c->ify), dfree_t(c->ifz), dfree_t(c->irho), dfree_t(c->idf), dfree_t(c->itype);
+  dfree_t(c->d_wide), dfree_t(c->e_itl);
+  if (c->itl_state) {
+    ItlState &s = state(c);
+    dfree_t(s.ax), dfree_t(s.ay), dfree_t(s.az), dfree_t(s.nbr), dfree_t(s.nnb);
+    dfree_t(s.px), dfree_t(s.py), dfree_t(s.pz), dfree_t(s.bins), dfree_t(s.err);
+    delete static_cast<ItlState *>(c->itl_state);
+    c->itl_state = nullptr;
+  }
+  c->n_itl = c->itl_cap = c->n_wide = 0;
+  c->itl_part_n = 0;
+}
+
+static ItlDev make_dev(mmd_ctx *c) {
+  ItlState &s = state(c);
+  ItlDev I{};
+  I.x = c->ix, I.y = c->iy, I.z = c->iz;
+  I.fx = c->ifx, I.fy = c->ify, I.fz = c->ifz, I.rho = c->irho, I.df = c->idf;
+  I.type = c->itype;
+  I.ax = s.ax, I.ay = s.ay, I.az = s.az, I.px = s.px, I.py = s.py, I.pz = s.pz;
+  I.nbr = s.nbr, I.nnb = s.nnb;
+  const int nbins = s.nbx * s.nby * s.nbz;
+  I.bin_count = s.bins, I.bin_list = s.bins + nbins;
+  I.err = s.err;
+  I.wide = c->d_wide;
+  I.n = c->n_itl, I.n_wide = c->n_wide, I.max_d2q = s.max_d2q;
+  I.nbx = s.nbx, I.nby = s.nby, I.nbz = s.nbz;
+  I.Lx = c->md.box_len[0], I.Ly = c->md.box_len[1], I.Lz = c->md.box_len[2];
+  I.bsx = I.Lx / s.nbx, I.bsy = I.Ly / s.nby, I.bsz = I.Lz / s.nbz;
+  I.a = c->md.lattice_const;
+  I.rc2 = c->md.cutoff * c->md.cutoff;
+  I.rc2x4 = 4.0 * I.rc2;
+  return I;
+}
+
+// bins sized from the MD box; (re)allocated when the box changes
+static int ensure_bins(mmd_ctx *c) {
+  ItlState &s = state(c);
+  const double bs = 2.0 * c->md.cutoff;
+  const int nx = std::max(1, (int)(c->md.box_len[0] / bs)), ny = std::max(1, (int)(c->md.box_len[1] / bs)),
+            nz = std::max(1, (int)(c->md.box_len[2] / bs));
+  if (s.bins && nx == s.nbx && ny == s.nby && nz == s.nbz) return MMD_OK;
+  dfree_t(s.bins);
+  const size_t n = (size_t)nx * ny * nz;
+  ITL_TRY(hipMalloc((void **)&s.bins, n * (1 + kBinCap) * sizeof(int32_t)));
+  s.nbx = nx, s.nby = ny, s.nbz = nz;
+  ctx_drop_graphs(c);
   return MMD_OK;
 }
 
-extern "C" int mmd_itl_get(mmd_ctx *c, int32_t *n, double *, double *, double *, double *, double *, double *,
-                           double *, double *, double *, double *, double *, int8_t *) {
-  if (!c) return mmd::fail(MMD_E_INVALID, "null context");
+static int itl_prepare(mmd_ctx *c) {
+  if (!c->md_ready) return fail(MMD_E_STATE, "interstitial passes need mmd_md_setup (box, lattice constant, cutoff)");
+  int rc = ensure_bins(c);
+  if (rc) return rc;
+  ItlState &s = state(c);
+  ItlDev I = make_dev(c);
+  const int nbins = s.nbx * s.nby * s.nbz;
+  ITL_TRY(launch_fill_i32(c->stream, s.bins, nbins, 0));
+  const dim3 grid((c->n_itl + kBlock - 1) / kBlock);
+  k_itl_prepare<<<grid, kBlock, 0, c->stream>>>(c->g, I);
+  ITL_TRY(hipGetLastError());
+  k_itl_nbr<<<grid, kBlock, 0, c->stream>>>(I);
+  ITL_TRY(hipGetLastError());
+  c->itl_stale = false;
+  return MMD_OK;
+}
+
+int itl_rho_df(mmd_ctx *c, double cutoff, bool want_e) {
+  if (!c->md_ready) return fail(MMD_E_STATE, "interstitial passes need mmd_md_setup (box, lattice constant, cutoff)");
+  if (std::fabs(cutoff - c->md.cutoff) > 1e-12 * cutoff)
+    return fail(MMD_E_INVALID, "interstitial passes use the cutoff given to mmd_md_setup (%g), got %g", c->md.cutoff,
+                cutoff);
+  hipEvent_t ev;
+  int rc = timed_begin(c, MMD_K_ITL, &ev);
+  if (rc || (rc = itl_prepare(c))) return rc;
+  ItlDev I = make_dev(c);
+  const dim3 grid((c->n_itl + kWavesPerBlock - 1) / kWavesPerBlock);
+  if (want_e)
+    k_itl_pass<0, true><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->x, c->y, c->z, c->type, c->rho, c->df, c->fx,
+                                                          c->fy, c->fz, c->tf, c->e_itl);
+  else
+    k_itl_pass<0, false><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->x, c->y, c->z, c->type, c->rho, c->df, c->fx,
+                                                           c->fy, c->fz, c->tf, c->e_itl);
+  ITL_TRY(hipGetLastError());
+  return timed_end(c, MMD_K_ITL, ev);
+}
+
+int itl_df(mmd_ctx *c) {
+  ItlDev I = make_dev(c);
+  k_itl_df<<<dim3((c->n_itl + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(I, c->tf);
+  ITL_TRY(hipGetLastError());
+  return MMD_OK;
+}
+
+int itl_force(mmd_ctx *c, double cutoff, bool want_e) {
+  if (!c->md_ready) return fail(MMD_E_STATE, "interstitial passes need mmd_md_setup (box, lattice constant, cutoff)");
+  if (std::fabs(cutoff - c->md.cutoff) > 1e-12 * cutoff)
+    return fail(MMD_E_INVALID, "interstitial passes use the cutoff given to mmd_md_setup (%g), got %g", c->md.cutoff,
+                cutoff);
+  hipEvent_t ev;
+  int rc = timed_begin(c, MMD_K_ITL, &ev);
+  if (rc) return rc;
+  if (c->itl_stale && (rc = itl_prepare(c))) return rc;  // positions changed since the rho pass
+  ItlDev I = make_dev(c);
+  const dim3 grid((c->n_itl + kWavesPerBlock - 1) / kWavesPerBlock);
+  // the embedding-energy partials of the rho pass live in the first half of e_itl
+  double *ep = c->e_itl + c->itl_part_n / 2;
+  if (want_e)
+    k_itl_pass<1, true><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->x, c->y, c->z, c->type, c->rho, c->df, c->fx,
+                                                          c->fy, c->fz, c->tf, ep);
+  else
+    k_itl_pass<1, false><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->x, c->y, c->z, c->type, c->rho, c->df, c->fx,
+                                                           c->fy, c->fz, c->tf, ep);
+  ITL_TRY(hipGetLastError());
+  return timed_end(c, MMD_K_ITL, ev);
+}
+
+int itl_kick(mmd_ctx *c, double h, bool drift, bool want_ke) {
+  ItlDev I = make_dev(c);
+  const dim3 grid((c->n_itl + kBlock - 1) / kBlock);
+  double *kp = c->e_itl + c->itl_part_n;  // kinetic partials follow the energy partials
+  if (drift) {
+    if (want_ke)
+      k_itl_kick<true, true><<<grid, kBlock, 0, c->stream>>>(I, c->ivx, c->ivy, c->ivz, c->ix, c->iy, c->iz, c->masses,
+                                                               h, c->md.dt, kp);
+    else
+      k_itl_kick<true, false><<<grid, kBlock, 0, c->stream>>>(I, c->ivx, c->ivy, c->ivz, c->ix, c->iy, c->iz,
+                                                                c->masses, h, c->md.dt, kp);
+  } else {
+    if (want_ke)
+      k_itl_kick<false, true><<<grid, kBlock, 0, c->stream>>>(I, c->ivx, c->ivy, c->ivz, c->ix, c->iy, c->iz,
+                                                                c->masses, h, c->md.dt, kp);
+    else
+      k_itl_kick<false, false><<<grid, kBlock, 0, c->stream>>>(I, c->ivx, c->ivy, c->ivz, c->ix, c->iy, c->iz,
+                                                                 c->masses, h, c->md.dt, kp);
+  }
+  ITL_TRY(hipGetLastError());
+  if (drift) c->itl_stale = true;
+  return MMD_OK;
+}
+
+int itl_check(mmd_ctx *c) {
+  if (c->n_itl == 0 || !c->itl_state) return MMD_OK;
+  ItlState &s = state(c);
+  int32_t e = 0;
+  ITL_TRY(hipMemcpyAsync(&e, s.err, sizeof(e), hipMemcpyDeviceToHost, c->stream));
+  ITL_TRY(hipStreamSynchronize(c->stream));
+  if (e & 1) return fail(MMD_E_RANGE, "interstitial bin overflow (more than %d per bin)", kBinCap);
+  if (e & 2) return fail(MMD_E_RANGE, "more than %d interstitials within 2 rc of one interstitial", kItlNbrCap);
+  return MMD_OK;
+}
+
+}  // namespace mmd
+
+using namespace mmd;
+
+extern "C" int mmd_itl_set(mmd_ctx *c, int32_t n, const double *x, const double *y, const double *z, const double *vx,
+                           const double *vy, const double *vz, const int8_t *type, const int32_t *wide3,
+                           int32_t n_wide, int32_t cap) {
+  int rc = ctx_bind(c);
+  if (rc) return rc;
+  if (!c->has_domain) return fail(MMD_E_STATE, "mmd_itl_set: domain not set");
+  if (n < 0 || cap < n || cap > (1 << 24)) return fail(MMD_E_RANGE, "mmd_itl_set: need 0 <= n <= cap <= 2^24");
+  if (n > 0 && (!x || !y || !z || !vx || !vy || !vz || !type || !wide3 || n_wide < 1 || n_wide > 4096))
+    return fail(MMD_E_INVALID, "mmd_itl_set: arrays or wide stencil missing");
+  int32_t max_d2q = 0;
+  for (int32_t w = 0; w < n_wide; w++) {
+    const int dX = wide3[3 * w], dY = wide3[3 * w + 1], dZ = wide3[3 * w + 2];
+    if (std::abs(dX) > c->dom.ghost[0] || std::abs(dY) > c->dom.ghost[1] || std::abs(dZ) > c->dom.ghost[2])
+      return fail(MMD_E_RANGE, "mmd_itl_set: wide stencil entry %d leaves the ghost shell", w);
+    const int s1 = dX & 1;
+    max_d2q = std::max(max_d2q, dX * dX + (2 * dY + s1) * (2 * dY + s1) + (2 * dZ + s1) * (2 * dZ + s1));
+  }
+  for (int32_t k = 0; k < n; k++)
+    if (type[k] < 0 || type[k] >= kMaxTypes || (c->has_tables && type[k] >= c->n_types))
+      return fail(MMD_E_INVALID, "mmd_itl_set: interstitial %d has type %d", k, type[k]);
+  ITL_TRY(hipStreamSynchronize(c->stream));
+  ctx_drop_graphs(c);
+  const bool realloc = cap > c->itl_cap || n_wide > c->n_wide || !c->d_wide;
+  if (realloc) {
+    itl_free(c);
+    ItlState &s = state(c);
+    const size_t nc = (size_t)std::max(cap, 1);
+    double **d[] = {&c->ix, &c->iy, &c->iz, &c->ivx, &c->ivy, &c->ivz, &c->ifx, &c->ify, &c->ifz, &c->irho,
+                    &c->idf, &s.px, &s.py, &s.pz};
+    for (double **p : d) ITL_TRY(hipMalloc((void **)p, nc * sizeof(double)));
+    int32_t **ip[] = {&s.ax, &s.ay, &s.az, &s.nnb};
+    for (int32_t **p : ip) ITL_TRY(hipMalloc((void **)p, nc * sizeof(int32_t)));
+    ITL_TRY(hipMalloc((void **)&s.nbr, nc * kItlNbrCap * sizeof(int32_t)));
+    ITL_TRY(hipMalloc((void **)&c->itype, nc));
+    ITL_TRY(hipMalloc((void **)&s.err, sizeof(int32_t)));
+    ITL_TRY(hipMalloc((void **)&c->d_wide, (size_t)std::max(n_wide, 1) * 3 * sizeof(int32_t)));
+    // energy partials: rho-pass blocks, force-pass blocks, kinetic blocks
+    const int64_t nb_pass = (cap + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int64_t nb_kick = (cap + kBlock - 1) / kBlock;
+    c->itl_part_n = 2 * nb_pass;
+    c->itl_kick_n = nb_kick;
+    ITL_TRY(hipMalloc((void **)&c->e_itl, (size_t)(2 * nb_pass + nb_kick) * sizeof(double)));
+    c->itl_cap = cap;
+  }
+  ItlState &s = state(c);
+  s.max_d2q = max_d2q;
+  c->n_wide = n_wide;
+  c->n_itl = n;
+  c->itl_stale = true;
+  if (n_wide > 0) ITL_TRY(hipMemcpy(c->d_wide, wide3, (size_t)n_wide * 3 * sizeof(int32_t), hipMemcpyHostToDevice));
+  ITL_TRY(launch_fill_i32(c->stream, s.err, 1, 0));
+  const int64_t nb_pass = (c->itl_cap + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int64_t nb_kick = (c->itl_cap + kBlock - 1) / kBlock;
+  ITL_TRY(launch_fill_f64(c->stream, c->e_itl, 2 * nb_pass + nb_kick, 0.0));
+  if (n > 0) {
+    const size_t b = (size_t)n * sizeof(double);
+    ITL_TRY(hipMemcpy(c->ix, x, b, hipMemcpyHostToDevice));
+    ITL_TRY(hipMemcpy(c->iy, y, b, hipMemcpyHostToDevice));
+    ITL_TRY(hipMemcpy(c->iz, z, b, hipMemcpyHostToDevice));
+    ITL_TRY(hipMemcpy(c->ivx, vx, b, hipMemcpyHostToDevice));
+    ITL_TRY(hipMemcpy(c->ivy, vy, b, hipMemcpyHostToDevice));
+    ITL_TRY(hipMemcpy(c->ivz, vz, b, hipMemcpyHostToDevice));
+    ITL_TRY(hipMemcpy(c->itype, type, (size_t)n, hipMemcpyHostToDevice));
+    ITL_TRY(launch_fill_f64(c->stream, c->ifx, n, 0.0));
+    ITL_TRY(launch_fill_f64(c->stream, c->ify, n, 0.0));
+    ITL_TRY(launch_fill_f64(c->stream, c->ifz, n, 0.0));
+    ITL_TRY(launch_fill_f64(c->stream, c->irho, n, 0.0));
+    ITL_TRY(launch_fill_f64(c->stream, c->idf, n, 0.0));
+  }
+  ITL_TRY(hipStreamSynchronize(c->stream));
+  return MMD_OK;
+}
+
+extern "C" int mmd_itl_get(mmd_ctx *c, int32_t *n, double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                           double *fx, double *fy, double *fz, double *rho, double *df, int8_t *type) {
+  int rc = ctx_bind(c);
+  if (rc) return rc;
+  if ((rc = itl_check(c))) return rc;
   if (n) *n = c->n_itl;
+  const size_t b = (size_t)c->n_itl * sizeof(double);
+  if (c->n_itl == 0) return MMD_OK;
+  struct P {
+    double *h;
+    const double *d;
+  } ps[] = {{x, c->ix}, {y, c->iy}, {z, c->iz}, {vx, c->ivx}, {vy, c->ivy}, {vz, c->ivz},
+            {fx, c->ifx}, {fy, c->ify}, {fz, c->ifz}, {rho, c->irho}, {df, c->idf}};
+  for (auto &p : ps)
+    if (p.h) ITL_TRY(hipMemcpyAsync(p.h, p.d, b, hipMemcpyDeviceToHost, c->stream));
+  if (type) ITL_TRY(hipMemcpyAsync(type, c->itype, (size_t)c->n_itl, hipMemcpyDeviceToHost, c->stream));
+  ITL_TRY(hipStreamSynchronize(c->stream));
   return MMD_OK;
 }

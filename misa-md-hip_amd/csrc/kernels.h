@@ -49,8 +49,8 @@ hipError_t launch_kick(hipStream_t st, const Geom &g, const Masses &M, double h,
 
 // energies: out[slot] = {sum(ep_a) + sum(ep_b), sum(ke)}, slot = (*counter)++
 hipError_t launch_energy_finalize(hipStream_t st, const double *ep_a, int64_t na, const double *ep_b, int64_t nb,
-                                  const double *ep_c, int64_t nc, const double *ke, int64_t nk, double *out,
-                                  int32_t *counter);
+                                  const double *ep_c, int64_t nc, const double *ke, int64_t nk, const double *ke_b,
+                                  int64_t nkb, double *out, int32_t *counter);
 hipError_t launch_fill_i32(hipStream_t st, int32_t *p, int64_t n, int32_t v);
 hipError_t launch_fill_f64(hipStream_t st, double *p, int64_t n, double v);
 hipError_t launch_fill_i8(hipStream_t st, int8_t *p, int64_t n, int8_t v);

@@ -12,6 +12,29 @@
 
 namespace mmd {
 
+#ifndef MMD_PAIR_UNROLL
+#define MMD_PAIR_UNROLL 4
+#endif
+constexpr int kUnroll = MMD_PAIR_UNROLL;  // stencil entries whose loads are in flight together
+
+// Loads through a uniform base pointer with a 32-bit element index, so the
+// address is an SGPR base + 32-bit VGPR offset (no 64-bit VALU arithmetic).
+template <typename T>
+__device__ __forceinline__ T ld(const T *__restrict__ base, uint32_t idx) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + idx * (uint32_t)sizeof(T));
+}
+
+// 1/sqrt(d2) for d2 > 0: hardware reciprocal-square-root seed refined by two
+// Newton steps to full FP64 accuracy (a few ulp), instead of the IEEE sqrt and
+// divide sequences.
+__device__ __forceinline__ double rsqrt64(double d2) {
+  double y = __builtin_amdgcn_rsq(d2);
+  double t = fma(-d2 * y, y, 1.0);
+  y = fma(0.5 * y, t, y);
+  t = fma(-d2 * y, y, 1.0);
+  return fma(0.5 * y, t, y);
+}
+
 // ------------------------------------------------------------ spline eval
 
 struct Seg {
@@ -30,14 +53,15 @@ __device__ __forceinline__ Seg seg_f64(double x, double x0, double inv_dx, int n
   return s;
 }
 
-__device__ __forceinline__ double val8(const double *__restrict__ tab, Seg s) {
-  const double4 c = *reinterpret_cast<const double4 *>(tab + (size_t)s.m * 8);
-  return ((c.x * s.p + c.y) * s.p + c.z) * s.p + c.w;
+// record = knot index (+ function offset); 8 doubles per record
+__device__ __forceinline__ double val8(const double *__restrict__ tab, uint32_t rec, double p) {
+  const double4 c = ld(reinterpret_cast<const double4 *>(tab), 2 * rec);
+  return ((c.x * p + c.y) * p + c.z) * p + c.w;
 }
 
-__device__ __forceinline__ double der8(const double *__restrict__ tab, Seg s) {
-  const double4 c = *reinterpret_cast<const double4 *>(tab + (size_t)s.m * 8 + 4);
-  return (c.x * s.p + c.y) * s.p + c.z;
+__device__ __forceinline__ double der8(const double *__restrict__ tab, uint32_t rec, double p) {
+  const double4 c = ld(reinterpret_cast<const double4 *>(tab), 2 * rec + 1);
+  return (c.x * p + c.y) * p + c.z;
 }
 
 struct SegS {
@@ -55,62 +79,120 @@ __device__ __forceinline__ SegS seg_s(double s2, const TabMixed &t) {
   return r;
 }
 
-__device__ __forceinline__ double valS(const double4 *__restrict__ tab, SegS s) {
-  const double4 rec = tab[s.m];
-  const float2 a23 = __builtin_bit_cast(float2, rec.z);
-  const float c = fmaf((float)s.q, a23.y, a23.x);  // FP32 quadratic/cubic correction
-  return fma(s.q, fma(s.q, (double)c, rec.y), rec.x);
+__device__ __forceinline__ double valS(const double4 *__restrict__ tab, uint32_t rec, double q) {
+  const double4 c = ld(tab, rec);
+  const float2 a23 = __builtin_bit_cast(float2, c.z);
+  const float cc = fmaf((float)q, a23.y, a23.x);  // FP32 quadratic/cubic correction
+  return fma(q, fma(q, (double)cc, c.y), c.x);
 }
 
-__device__ __forceinline__ double derS(const double4 *__restrict__ tab, SegS s) {
-  const double4 rec = tab[s.m];
-  const float2 b2 = __builtin_bit_cast(float2, rec.z);
-  return fma(s.q, fma(s.q, (double)b2.x, rec.y), rec.x);
+__device__ __forceinline__ double derS(const double4 *__restrict__ tab, uint32_t rec, double q) {
+  const double4 c = ld(tab, rec);
+  const float2 b2 = __builtin_bit_cast(float2, c.z);
+  return fma(q, fma(q, (double)b2.x, c.y), c.x);
 }
 
-// ------------------------------------------------------------------ rho
+// ------------------------------------------------------------- pair terms
 
+// rho_{tj}(r) for r^2 = d2
 template <int PREC>
 __device__ __forceinline__ double rho_pair(const TabF64 &tf, const TabMixed &tm, int tj, double d2) {
-  if (PREC == 0) {
-    const Seg sg = seg_f64(sqrt(d2), tf.r0, tf.inv_dr, tf.n_r);
-    return val8(tf.rho + (size_t)tj * tf.n_r * 8, sg);
-  } else {
-    if (d2 >= tm.s_tail || d2 < tm.s_head) return rho_pair<0>(tf, tm, tj, d2);
+  if (PREC == 1 && d2 < tm.s_tail && d2 >= tm.s_head) {
     const SegS sg = seg_s(d2, tm);
-    return valS(tm.rho_v + (size_t)tj * tm.n_s, sg);
+    return valS(tm.rho_v, (uint32_t)(tj * tm.n_s + sg.m), sg.q);
   }
+  const double r = d2 * rsqrt64(d2);
+  const Seg sg = seg_f64(r, tf.r0, tf.inv_dr, tf.n_r);
+  return val8(tf.rho, (uint32_t)(tj * tf.n_r + sg.m), sg.p);
 }
 
+// fpair = -(df_i rho'_{tj} + df_j rho'_{ti} + phi'_{ti tj}) / r (the scalar that
+// multiplies del = x_i - x_j); phi receives phi_{ti tj}(r) when WANT_E.
 template <int PREC, bool WANT_E>
-__global__ __launch_bounds__(kBlock) void k_rho_df(Geom g, const int32_t *__restrict__ off, int32_t nk_even, int32_t nk_odd,
-                                                     const double *__restrict__ X, const double *__restrict__ Y,
-                                                     const double *__restrict__ Z, const int8_t *__restrict__ T,
-                                                     double *__restrict__ RHO, double *__restrict__ DF, TabF64 tf,
-                                                     TabMixed tm, double rc2, double *__restrict__ e_part) {
+__device__ __forceinline__ double fpair(const TabF64 &tf, const TabMixed &tm, int ti, int tj, double d2, double dfi,
+                                        double dfj, double &phi) {
+  if (PREC == 1 && d2 < tm.s_tail && d2 >= tm.s_head) {
+    // r^2-keyed: d/ds tables, -(dpsi/dr)/r = -2 dpsi/ds
+    const SegS sg = seg_s(d2, tm);
+    const uint32_t rj = (uint32_t)(tj * tm.n_s + sg.m);
+    const uint32_t rp = (uint32_t)((ti * tm.nt + tj) * tm.n_s + sg.m);
+    const double rhojp = derS(tm.rho_d, rj, sg.q);
+    double rhoip = rhojp;
+    if (ti != tj) rhoip = derS(tm.rho_d, (uint32_t)(ti * tm.n_s + sg.m), sg.q);
+    const double phip = derS(tm.phi_d, rp, sg.q);
+    if (WANT_E) phi = valS(tm.phi_v, rp, sg.q);
+    return -2.0 * (dfi * rhojp + dfj * rhoip + phip);
+  }
+  const double rinv = rsqrt64(d2);
+  const double r = d2 * rinv;
+  const Seg sg = seg_f64(r, tf.r0, tf.inv_dr, tf.n_r);
+  const uint32_t rj = (uint32_t)(tj * tf.n_r + sg.m);
+  const uint32_t rp = (uint32_t)((ti * tf.nt + tj) * tf.n_r + sg.m);
+  const double rhojp = der8(tf.rho, rj, sg.p);
+  double rhoip = rhojp;
+  if (ti != tj) rhoip = der8(tf.rho, (uint32_t)(ti * tf.n_r + sg.m), sg.p);
+  const double phip = der8(tf.phi, rp, sg.p);
+  if (WANT_E) phi = val8(tf.phi, rp, sg.p);
+  return -(dfi * rhojp + dfj * rhoip + phip) * rinv;
+}
+
+// --------------------------------------------------------------- kernels
+//
+// One thread per local site, kUnroll stencil entries per trip: all neighbour
+// loads of a trip are issued before any is used, pairs are evaluated without
+// per-pair branches (out-of-cutoff and vacant neighbours are masked), and a
+// trip whose pairs are all outside the cutoff for the whole wave is skipped.
+// The stencil rows are padded with zero offsets past nk.
+
+template <int PREC, bool WANT_E>
+__global__ __launch_bounds__(kBlock) void k_rho_df(Geom g, const int32_t *__restrict__ off, int32_t nk_even,
+                                                     int32_t nk_odd, const double *__restrict__ X,
+                                                     const double *__restrict__ Y, const double *__restrict__ Z,
+                                                     const int8_t *__restrict__ T, double *__restrict__ RHO,
+                                                     double *__restrict__ DF, TabF64 tf, TabMixed tm, double rc2,
+                                                     double *__restrict__ e_part) {
   __shared__ double lds[kWavesPerBlock];
   double e_loc = 0.0;
   Site st;
   if (map_site(g, st) && st.active) {
-    const int64_t i = st.i;
-    const int t0 = T[i];
+    const uint32_t i = (uint32_t)st.i;
+    const int t0 = ld(T, i);
     if (t0 >= 0) {
       const int32_t *o = off + st.s * kMaxStencil;
       const int nk = st.s ? nk_odd : nk_even;
-      const double xi = X[i], yi = Y[i], zi = Z[i];
+      const double xi = ld(X, i), yi = ld(Y, i), zi = ld(Z, i);
+      const double d2_safe = 0.25 * rc2;
       double acc = 0.0;
-      for (int k = 0; k < nk; k++) {
-        const int64_t j = i + o[k];
-        const int tj = T[j];
-        const double dx = xi - X[j], dy = yi - Y[j], dz = zi - Z[j];
-        const double d2 = dx * dx + dy * dy + dz * dz;
-        if (tj >= 0 && d2 < rc2) acc += rho_pair<PREC>(tf, tm, tj, d2);
+      for (int k = 0; k < nk; k += kUnroll) {
+        int tj[kUnroll];
+        double d2[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+          const uint32_t j = i + (uint32_t)o[k + u];
+          tj[u] = ld(T, j);
+          const double dx = xi - ld(X, j), dy = yi - ld(Y, j), dz = zi - ld(Z, j);
+          d2[u] = dx * dx + dy * dy + dz * dz;
+        }
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+          const bool in = (k + u < nk) && tj[u] >= 0 && d2[u] < rc2;
+          if (!in) d2[u] = -1.0;
+          any |= in;
+        }
+        if (__ballot(any) == 0) continue;
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+          const bool in = d2[u] >= 0.0;
+          const double v = rho_pair<PREC>(tf, tm, in ? tj[u] : 0, in ? d2[u] : d2_safe);
+          acc += in ? v : 0.0;
+        }
       }
       RHO[i] = acc;
       const Seg se = seg_f64(acc, tf.rho0, tf.inv_drho, tf.n_rho);
-      const double *et = tf.emb + (size_t)t0 * tf.n_rho * 8;
-      DF[i] = der8(et, se);
-      if (WANT_E) e_loc = val8(et, se);
+      const uint32_t re = (uint32_t)(t0 * tf.n_rho + se.m);
+      DF[i] = der8(tf.emb, re, se.p);
+      if (WANT_E) e_loc = val8(tf.emb, re, se.p);
     }
   }
   if (WANT_E) {
@@ -124,73 +206,64 @@ __global__ __launch_bounds__(kBlock) void k_df(Geom g, const int8_t *__restrict_
                                                  double *__restrict__ DF, TabF64 tf) {
   Site st;
   if (!map_site(g, st) || !st.active) return;
-  const int t0 = T[st.i];
+  const uint32_t i = (uint32_t)st.i;
+  const int t0 = ld(T, i);
   if (t0 < 0) return;
-  const Seg se = seg_f64(RHO[st.i], tf.rho0, tf.inv_drho, tf.n_rho);
-  DF[st.i] = der8(tf.emb + (size_t)t0 * tf.n_rho * 8, se);
-}
-
-// ------------------------------------------------------------------ force
-
-// fpair of one pair (centre type ti, neighbour tj); phi receives the pair
-// energy when WANT_E.
-template <int PREC, bool WANT_E>
-__device__ __forceinline__ double fpair(const TabF64 &tf, const TabMixed &tm, int ti, int tj, double d2, double dfi,
-                                        double dfj, double &phi) {
-  if (PREC == 0) {
-    const double r = sqrt(d2);
-    const Seg sg = seg_f64(r, tf.r0, tf.inv_dr, tf.n_r);
-    const size_t stride = (size_t)tf.n_r * 8;
-    const double *pt = tf.phi + (size_t)(ti * tf.nt + tj) * stride;
-    const double rhojp = der8(tf.rho + (size_t)tj * stride, sg);
-    const double rhoip = der8(tf.rho + (size_t)ti * stride, sg);
-    const double phip = der8(pt, sg);
-    if (WANT_E) phi = val8(pt, sg);
-    const double psip = dfi * rhojp + dfj * rhoip + phip;
-    return -psip / r;
-  } else {
-    if (d2 >= tm.s_tail || d2 < tm.s_head) return fpair<0, WANT_E>(tf, tm, ti, tj, d2, dfi, dfj, phi);
-    const SegS sg = seg_s(d2, tm);
-    const size_t pr = (size_t)(ti * tm.nt + tj) * tm.n_s;
-    const double rhojp = derS(tm.rho_d + (size_t)tj * tm.n_s, sg);
-    const double rhoip = derS(tm.rho_d + (size_t)ti * tm.n_s, sg);
-    const double phip = derS(tm.phi_d + pr, sg);
-    if (WANT_E) phi = valS(tm.phi_v + pr, sg);
-    // derivatives are d/ds with s = r^2:  -(dpsi/dr)/r = -2 dpsi/ds
-    return -2.0 * (dfi * rhojp + dfj * rhoip + phip);
-  }
+  const Seg se = seg_f64(ld(RHO, i), tf.rho0, tf.inv_drho, tf.n_rho);
+  DF[i] = der8(tf.emb, (uint32_t)(t0 * tf.n_rho + se.m), se.p);
 }
 
 template <int PREC, bool WANT_E>
-__global__ __launch_bounds__(kBlock) void k_force(Geom g, const int32_t *__restrict__ off, int32_t nk_even, int32_t nk_odd,
-                                                    const double *__restrict__ X, const double *__restrict__ Y,
-                                                    const double *__restrict__ Z, const int8_t *__restrict__ T,
-                                                    const double *__restrict__ DF, double *__restrict__ FX,
-                                                    double *__restrict__ FY, double *__restrict__ FZ, TabF64 tf,
-                                                    TabMixed tm, double rc2, double *__restrict__ e_part) {
+__global__ __launch_bounds__(kBlock) void k_force(Geom g, const int32_t *__restrict__ off, int32_t nk_even,
+                                                    int32_t nk_odd, const double *__restrict__ X,
+                                                    const double *__restrict__ Y, const double *__restrict__ Z,
+                                                    const int8_t *__restrict__ T, const double *__restrict__ DF,
+                                                    double *__restrict__ FX, double *__restrict__ FY,
+                                                    double *__restrict__ FZ, TabF64 tf, TabMixed tm, double rc2,
+                                                    double *__restrict__ e_part) {
   __shared__ double lds[kWavesPerBlock];
   double e_loc = 0.0;
   Site st;
   if (map_site(g, st) && st.active) {
-    const int64_t i = st.i;
-    const int t0 = T[i];
+    const uint32_t i = (uint32_t)st.i;
+    const int t0 = ld(T, i);
     if (t0 >= 0) {
       const int32_t *o = off + st.s * kMaxStencil;
       const int nk = st.s ? nk_odd : nk_even;
-      const double xi = X[i], yi = Y[i], zi = Z[i], dfi = DF[i];
+      const double xi = ld(X, i), yi = ld(Y, i), zi = ld(Z, i), dfi = ld(DF, i);
+      const double d2_safe = 0.25 * rc2;
       double ax = 0.0, ay = 0.0, az = 0.0, ep = 0.0;
-      for (int k = 0; k < nk; k++) {
-        const int64_t j = i + o[k];
-        const int tj = T[j];
-        const double dx = xi - X[j], dy = yi - Y[j], dz = zi - Z[j];
-        const double d2 = dx * dx + dy * dy + dz * dz;
-        if (tj >= 0 && d2 < rc2) {
+      for (int k = 0; k < nk; k += kUnroll) {
+        int tj[kUnroll];
+        double dx[kUnroll], dy[kUnroll], dz[kUnroll], d2[kUnroll], dfj[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+          const uint32_t j = i + (uint32_t)o[k + u];
+          tj[u] = ld(T, j);
+          dx[u] = xi - ld(X, j);
+          dy[u] = yi - ld(Y, j);
+          dz[u] = zi - ld(Z, j);
+          dfj[u] = ld(DF, j);
+          d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+        }
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+          const bool in = (k + u < nk) && tj[u] >= 0 && d2[u] < rc2;
+          if (!in) d2[u] = -1.0;
+          any |= in;
+        }
+        if (__ballot(any) == 0) continue;
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+          const bool in = d2[u] >= 0.0;
           double phi = 0.0;
-          const double fp = fpair<PREC, WANT_E>(tf, tm, t0, tj, d2, dfi, DF[j], phi);
-          ax += dx * fp;
-          ay += dy * fp;
-          az += dz * fp;
-          if (WANT_E) ep += phi;
+          double fp = fpair<PREC, WANT_E>(tf, tm, t0, in ? tj[u] : t0, in ? d2[u] : d2_safe, dfi, dfj[u], phi);
+          fp = in ? fp : 0.0;
+          ax = fma(dx[u], fp, ax);
+          ay = fma(dy[u], fp, ay);
+          az = fma(dz[u], fp, az);
+          if (WANT_E) ep += in ? phi : 0.0;
         }
       }
       FX[i] = ax;

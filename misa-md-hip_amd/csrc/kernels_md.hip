@@ -180,13 +180,15 @@ __global__ __launch_bounds__(kBlock) void k_energy_finalize(const double *__rest
                                                               const double *__restrict__ ep_b, int64_t nb,
                                                               const double *__restrict__ ep_c, int64_t nc,
                                                               const double *__restrict__ ke, int64_t nk,
+                                                              const double *__restrict__ ke_b, int64_t nkb,
                                                               double *__restrict__ out, int32_t *__restrict__ counter) {
   __shared__ double lds[kWavesPerBlock];
   double pe = 0.0;
   if (ep_a) pe += ordered_sum(ep_a, na);
   if (ep_b) pe += ordered_sum(ep_b, nb);
   if (ep_c) pe += ordered_sum(ep_c, nc);
-  const double kin = ke ? ordered_sum(ke, nk) : 0.0;
+  double kin = ke ? ordered_sum(ke, nk) : 0.0;
+  if (ke_b) kin += ordered_sum(ke_b, nkb);
   const double PE = block_sum(pe, lds);
   const double KE = block_sum(kin, lds);
   if (threadIdx.x == 0) {
@@ -262,9 +264,9 @@ hipError_t launch_kick(hipStream_t st, const Geom &g, const Masses &M, double h,
 }
 
 hipError_t launch_energy_finalize(hipStream_t st, const double *ep_a, int64_t na, const double *ep_b, int64_t nb,
-                                  const double *ep_c, int64_t nc, const double *ke, int64_t nk, double *out,
-                                  int32_t *counter) {
-  k_energy_finalize<<<1, kBlock, 0, st>>>(ep_a, na, ep_b, nb, ep_c, nc, ke, nk, out, counter);
+                                  const double *ep_c, int64_t nc, const double *ke, int64_t nk, const double *ke_b,
+                                  int64_t nkb, double *out, int32_t *counter) {
+  k_energy_finalize<<<1, kBlock, 0, st>>>(ep_a, na, ep_b, nb, ep_c, nc, ke, nk, ke_b, nkb, out, counter);
   return hipGetLastError();
 }
 

@@ -560,3 +560,44 @@ void orc_force_itl(const orc_domain *d, const int64_t *off_even, int32_t n_even,
     }
   if (e_pair) *e_pair = e + c.e + eii;
 }
+
+void orc_md_run_itl(const orc_domain *d, const int64_t *off_even, int32_t n_even, const int64_t *off_odd,
+                    int32_t n_odd, const int32_t *wide3, int32_t n_wide, double a, const orc_tables *t, double rc,
+                    const double box_len[3], double dt, int32_t nsteps, double *x, double *y, double *z, double *vx,
+                    double *vy, double *vz, const int8_t *type_in, double *rho, double *df, double *fx, double *fy,
+                    double *fz, orc_itl *itl, double *energies, int32_t nthreads) {
+  int8_t *type = (int8_t *)type_in;
+  for (int32_t s = 0; s < nsteps; s++) {
+    orc_kick(d, t, 0.5 * dt, vx, vy, vz, type, fx, fy, fz, x, y, z, dt);
+    for (int32_t k = 0; k < itl->n; k++) {
+      const double sc = 0.5 * dt * ORC_FTM2V / t->mass[itl->type[k]];
+      itl->vx[k] += sc * itl->fx[k];
+      itl->vy[k] += sc * itl->fy[k];
+      itl->vz[k] += sc * itl->fz[k];
+      itl->x[k] += dt * itl->vx[k];
+      itl->y[k] += dt * itl->vy[k];
+      itl->z[k] += dt * itl->vz[k];
+    }
+    orc_ghost_periodic_xt(d, box_len, x, y, z, type);
+    double ee = 0.0, ep = 0.0;
+    orc_rho_df_itl(d, off_even, n_even, off_odd, n_odd, wide3, n_wide, a, box_len, t, rc, x, y, z, type, rho, df, itl,
+                   energies ? &ee : NULL, nthreads);
+    orc_ghost_periodic_df(d, df);
+    orc_force_itl(d, off_even, n_even, off_odd, n_odd, wide3, n_wide, a, box_len, t, rc, x, y, z, type, df, fx, fy, fz,
+                  itl, energies ? &ep : NULL, nthreads);
+    orc_kick(d, t, 0.5 * dt, vx, vy, vz, type, fx, fy, fz, NULL, NULL, NULL, dt);
+    double ke_itl = 0.0;
+    for (int32_t k = 0; k < itl->n; k++) {
+      const double m = t->mass[itl->type[k]];
+      const double sc = 0.5 * dt * ORC_FTM2V / m;
+      itl->vx[k] += sc * itl->fx[k];
+      itl->vy[k] += sc * itl->fy[k];
+      itl->vz[k] += sc * itl->fz[k];
+      ke_itl += 0.5 * m * (itl->vx[k] * itl->vx[k] + itl->vy[k] * itl->vy[k] + itl->vz[k] * itl->vz[k]);
+    }
+    if (energies) {
+      energies[2 * s + 0] = ee + ep;
+      energies[2 * s + 1] = orc_kinetic(d, t, vx, vy, vz, type) + ke_itl * ORC_MVV2E;
+    }
+  }
+}

@@ -142,6 +142,15 @@ void orc_force_itl(const orc_domain *d, const int64_t *off_even, int32_t n_even,
                    const int8_t *type, const double *df, double *fx, double *fy, double *fz, orc_itl *itl,
                    double *e_pair, int32_t nthreads);
 
+/* Velocity Verlet with interstitials (kick-drift of lattice atoms and
+ * interstitials, ghost x, rho+df with interstitials, ghost df, force with
+ * interstitials, kick).  Forces of both lists must be current on entry. */
+void orc_md_run_itl(const orc_domain *d, const int64_t *off_even, int32_t n_even, const int64_t *off_odd,
+                    int32_t n_odd, const int32_t *wide3, int32_t n_wide, double a, const orc_tables *t, double rc,
+                    const double box_len[3], double dt, int32_t nsteps, double *x, double *y, double *z, double *vx,
+                    double *vy, double *vz, const int8_t *type, double *rho, double *df, double *fx, double *fy,
+                    double *fz, orc_itl *itl, double *energies, int32_t nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,9 @@ def lib():
         "orc_force_itl": (None, [D, _i64, C.c_int32, _i64, C.c_int32, _i32, C.c_int32, C.c_double, _d, T,
                                  C.c_double, _d, _d, _d, _i8, _d, _d, _d, _d, C.POINTER(OrcItl), _d,
                                  C.c_int32]),
+        "orc_md_run_itl": (None, [D, _i64, C.c_int32, _i64, C.c_int32, _i32, C.c_int32, C.c_double, T, C.c_double,
+                                  _d, C.c_double, C.c_int32, _d, _d, _d, _d, _d, _d, _i8, _d, _d, _d, _d, _d,
+                                  C.POINTER(OrcItl), _d, C.c_int32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -226,3 +229,15 @@ def force_itl(dom, even, odd, wide3, a, box_len, tabs, rc, st, itl, nthreads=0, 
                         _p(st.type, C.c_int8), _p(st.df), _p(st.fx), _p(st.fy), _p(st.fz), C.byref(s),
                         C.byref(e) if want_e else None, nthreads)
     return e.value
+
+
+def md_run_itl(dom, even, odd, wide3, a, tabs, rc, box_len, dt, nsteps, st, itl, nthreads=0, energies=True):
+    L = np.ascontiguousarray(box_len, float)
+    w = np.ascontiguousarray(wide3, np.int32)
+    s = _itl_struct(itl)
+    en = np.zeros((nsteps, 2)) if energies else None
+    lib().orc_md_run_itl(C.byref(domain(dom)), _p(even, C.c_int64), len(even), _p(odd, C.c_int64), len(odd),
+                         _p(w, C.c_int32), len(w), a, C.byref(tabs.s), rc, _p(L), dt, nsteps, _p(st.x), _p(st.y),
+                         _p(st.z), _p(st.vx), _p(st.vy), _p(st.vz), _p(st.type, C.c_int8), _p(st.rho), _p(st.df),
+                         _p(st.fx), _p(st.fy), _p(st.fz), C.byref(s), _p(en) if energies else None, nthreads)
+    return en

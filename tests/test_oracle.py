@@ -314,3 +314,18 @@ def test_oracle_interstitials_match_bruteforce(n_types):
     assert np.abs(Fl - F[:n]).max() < 1e-11 * scale
     assert np.abs(Fi - F[n:]).max() < 1e-11 * scale
     assert abs(ee + ep - E) < 1e-11 * abs(E)
+
+
+def test_oracle_interstitial_md_conserves_energy():
+    c = W.case(cells=(6, 6, 6), n_types=3, sigma=0.04, temperature=600.0, vacancy_frac=0.01,
+               interstitial_frac=0.01, seed=21)
+    st, itl, dom, box = c["st"], c["itl"], c["dom"], c["box"]
+    T = O.Tables(c["tab"], c["masses"])
+    O.rho_df_itl(dom, c["even"], c["odd"], c["wide3"], A, box, T, RC, st, itl)
+    O.ghost_df(dom, st)
+    O.force_itl(dom, c["even"], c["odd"], c["wide3"], A, box, T, RC, st, itl)
+    en = O.md_run_itl(dom, c["even"], c["odd"], c["wide3"], A, T, RC, box, 0.5, 30, st, itl)
+    tot = en.sum(1)
+    n_at = (st.type[st.local_index()] >= 0).sum() + itl.n
+    assert np.ptp(tot) / n_at < 5e-4
+    assert np.ptp(en[:, 1]) > 10 * np.ptp(tot)
