@@ -131,11 +131,19 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only time a device-to-device streaming copy of BYTES (HBM bandwidth / counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs one sub-domain per GPU and has no multi-GPU "
               f"path; --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe:
+        with mmd.Context(0) as ctx:
+            ms = ctx.stream_copy(args.copy_probe, repeats=10)
+        print(json.dumps({"copy_bytes": args.copy_probe, "ms": ms, "GBps_read_plus_write": 2 * args.copy_probe / ms / 1e6}))
+        return
 
     log(f"bench: building {args.cells}^3-cell 3-type BCC workload")
     w = make_workload(args.cells)

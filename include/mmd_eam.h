@@ -210,6 +210,12 @@ int mmd_timing_enable(mmd_ctx *ctx, int32_t on);
 int mmd_timing_get(mmd_ctx *ctx, double *ms_total, int64_t *launches);
 int mmd_timing_reset(mmd_ctx *ctx);
 
+/* Streaming device-to-device copy of `bytes` (multiple of 16) on the context's
+ * stream with a plain 16-byte-per-lane kernel; *ms receives its duration
+ * from HIP events.  Used to measure achievable HBM bandwidth and to calibrate
+ * the hardware traffic counters against a kernel of known traffic. */
+int mmd_stream_copy(mmd_ctx *ctx, size_t bytes, int32_t repeats, double *ms);
+
 /* ------------------------------------------------- host helpers (no GPU) */
 
 /* Neighbour stencil of one parity in 3-D doubled-x offsets (dX,dY,dZ): all

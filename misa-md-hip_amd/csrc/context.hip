@@ -60,7 +60,7 @@ static void dfree(T *&p) {
 }
 
 static void free_state(mmd_ctx *c) {
-  dfree(c->x), dfree(c->y), dfree(c->z), dfree(c->vx), dfree(c->vy), dfree(c->vz);
+  dfree(c->pos), dfree(c->vx), dfree(c->vy), dfree(c->vz);
   dfree(c->fx), dfree(c->fy), dfree(c->fz), dfree(c->rho), dfree(c->df), dfree(c->type);
   dfree(c->stage), dfree(c->e_rho), dfree(c->e_force), dfree(c->e_ke), dfree(c->e_itl);
   dfree(c->aos), c->aos_bytes = 0;
@@ -125,8 +125,8 @@ int run_rho_df(mmd_ctx *c, double cutoff, bool want_e) {
   hipEvent_t ev;
   int rc = timed_begin(c, MMD_K_RHO, &ev);
   if (rc) return rc;
-  HIP_TRY(launch_rho_df(c->stream, c->g, c->d_off, c->nk_even, c->nk_odd, c->x, c->y, c->z, c->type, c->rho, c->df,
-                        c->tf, c->tm, cutoff * cutoff, c->prec, want_e ? c->e_rho : nullptr));
+  HIP_TRY(launch_rho_df(c->stream, c->g, c->d_off, c->nk_even, c->nk_odd, c->pos, c->rho, c->df, c->tf, c->tm,
+                        cutoff * cutoff, c->prec, want_e ? c->e_rho : nullptr));
   rc = timed_end(c, MMD_K_RHO, ev);
   if (rc) return rc;
   if (c->n_itl > 0) return itl_rho_df(c, cutoff, want_e);
@@ -137,8 +137,8 @@ int run_force(mmd_ctx *c, double cutoff, bool want_e) {
   hipEvent_t ev;
   int rc = timed_begin(c, MMD_K_FORCE, &ev);
   if (rc) return rc;
-  HIP_TRY(launch_force(c->stream, c->g, c->d_off, c->nk_even, c->nk_odd, c->x, c->y, c->z, c->type, c->df, c->fx,
-                       c->fy, c->fz, c->tf, c->tm, cutoff * cutoff, c->prec, want_e ? c->e_force : nullptr));
+  HIP_TRY(launch_force(c->stream, c->g, c->d_off, c->nk_even, c->nk_odd, c->pos, c->df, c->fx, c->fy, c->fz, c->tf,
+                       c->tm, cutoff * cutoff, c->prec, want_e ? c->e_force : nullptr));
   rc = timed_end(c, MMD_K_FORCE, ev);
   if (rc) return rc;
   if (c->n_itl > 0) return itl_force(c, cutoff, want_e);
@@ -149,8 +149,8 @@ int run_ghost(mmd_ctx *c, uint32_t fields) {
   hipEvent_t ev;
   int rc = timed_begin(c, MMD_K_GHOST, &ev);
   if (rc) return rc;
-  HIP_TRY(launch_ghost(c->stream, c->g, c->md.box_len[0], c->md.box_len[1], c->md.box_len[2], c->x, c->y, c->z,
-                       c->type, c->df, fields));
+  HIP_TRY(launch_ghost(c->stream, c->g, c->md.box_len[0], c->md.box_len[1], c->md.box_len[2], c->pos, c->type,
+                       c->df, fields));
   return timed_end(c, MMD_K_GHOST, ev);
 }
 
@@ -158,8 +158,8 @@ static int run_kick(mmd_ctx *c, double h, bool drift, bool want_ke) {
   hipEvent_t ev;
   int rc = timed_begin(c, MMD_K_INTEGRATE, &ev);
   if (rc) return rc;
-  HIP_TRY(launch_kick(c->stream, c->g, c->masses, h, c->md.dt, drift, c->x, c->y, c->z, c->vx, c->vy, c->vz, c->fx,
-                      c->fy, c->fz, c->type, want_ke ? c->e_ke : nullptr));
+  HIP_TRY(launch_kick(c->stream, c->g, c->masses, h, c->md.dt, drift, c->pos, c->vx, c->vy, c->vz, c->fx, c->fy,
+                      c->fz, c->type, want_ke ? c->e_ke : nullptr));
   rc = timed_end(c, MMD_K_INTEGRATE, ev);
   if (rc) return rc;
   if (c->n_itl > 0) return itl_kick(c, h, drift, want_ke);
@@ -276,11 +276,13 @@ extern "C" int mmd_set_domain(mmd_ctx *c, const mmd_domain *d) {
     g.gxh = d->ghost[0] / 2, g.gy = d->ghost[1], g.gz = d->ghost[2];
     g.bxh = d->box[0] / 2, g.by = d->box[1], g.bz = d->box[2];
     g.chunks = (g.bxh + kWave - 1) / kWave;
+    g.ty = std::min<int32_t>(g.by, 16);
+    g.n_ytiles = (g.by + g.ty - 1) / g.ty;
     g.n_ext = n_ext;
-    g.n_waves = (int64_t)2 * g.by * g.bz * g.chunks;
+    g.n_waves = (int64_t)2 * g.n_ytiles * g.ty * g.bz * g.chunks;
     c->dom = *d;
     const size_t n = (size_t)n_ext;
-    double **f64[] = {&c->x, &c->y, &c->z, &c->vx, &c->vy, &c->vz, &c->fx, &c->fy, &c->fz, &c->rho, &c->df, &c->stage};
+    double **f64[] = {&c->vx, &c->vy, &c->vz, &c->fx, &c->fy, &c->fz, &c->rho, &c->df};
     for (double **p : f64) {
       if ((rc = dalloc(p, n))) {
         free_state(c);
@@ -288,11 +290,17 @@ extern "C" int mmd_set_domain(mmd_ctx *c, const mmd_domain *d) {
       }
       HIP_TRY(launch_fill_f64(c->stream, *p, (int64_t)n, 0.0));
     }
+    if ((rc = dalloc(&c->stage, 3 * n)) || (rc = dalloc(&c->pos, n))) {
+      free_state(c);
+      return rc;
+    }
+    HIP_TRY(launch_fill_f64(c->stream, reinterpret_cast<double *>(c->pos), (int64_t)(4 * n), 0.0));
     if ((rc = dalloc(&c->type, n))) {
       free_state(c);
       return rc;
     }
     HIP_TRY(launch_fill_i8(c->stream, c->type, (int64_t)n, (int8_t)-1));
+    HIP_TRY(launch_ref2dev_pos(c->stream, c->g, nullptr, nullptr, nullptr, c->type, c->pos, false));
     const size_t nb = (size_t)site_blocks(g);
     if ((rc = dalloc(&c->e_rho, nb)) || (rc = dalloc(&c->e_force, nb)) || (rc = dalloc(&c->e_ke, nb))) {
       free_state(c);
@@ -353,38 +361,105 @@ extern "C" int mmd_set_stencil(mmd_ctx *c, const int64_t *even, int32_t n_even, 
   })
 }
 
-// Mixed-precision r^2-keyed tables resampled from the FP64 r-keyed spline:
+// Host image of the device tables: every array 256-byte aligned in one buffer.
+struct Arena {
+  std::vector<char> buf;
+  size_t put(const void *p, size_t bytes) {
+    const size_t off = (buf.size() + 255) & ~(size_t)255;
+    buf.resize(off + bytes);
+    std::memcpy(buf.data() + off, p, bytes);
+    return off;
+  }
+};
+
+struct SplHost {
+  size_t v34, v56, d01, d2;
+};
+
+// FP64 spline family: spline7 coefficients split by width (see SplF64).
+static SplHost pack_f64(Arena &A, const double *knots, int nfun, int n, double dx) {
+  std::vector<double> c7((size_t)n * 7), v34((size_t)nfun * n * 2), v56(v34.size()), d01(v34.size()),
+      d2((size_t)nfun * n);
+  for (int f = 0; f < nfun; f++) {
+    spline7(knots + (size_t)f * n, n, dx, c7.data());
+    for (int m = 0; m < n; m++) {
+      const double *k = &c7[(size_t)m * 7];
+      const size_t r = (size_t)f * n + m;
+      v34[2 * r] = k[3], v34[2 * r + 1] = k[4];
+      v56[2 * r] = k[5], v56[2 * r + 1] = k[6];
+      d01[2 * r] = k[0], d01[2 * r + 1] = k[1];
+      d2[r] = k[2];
+    }
+  }
+  SplHost h;
+  h.v34 = A.put(v34.data(), v34.size() * 8);
+  h.v56 = A.put(v56.data(), v56.size() * 8);
+  h.d01 = A.put(d01.data(), d01.size() * 8);
+  h.d2 = A.put(d2.data(), d2.size() * 8);
+  return h;
+}
+
+struct MixHost {
+  size_t vb, vc, db, dc;
+};
+
+// Mixed-precision r^2-keyed family resampled from the FP64 r-keyed spline:
 // knot values and exact derivatives of g(sqrt(s)) at uniform s, cubic Hermite
 // in s; FP64 base and linear terms, FP32 quadratic and cubic terms.
-static void build_mixed(const double *c7, int n_r, double r0, double inv_dr, int n_s, double s0, double ds,
-                        double *vrec /*[n_s][4]*/, double *drec /*[n_s][4]*/) {
-  std::vector<double> h(n_s), hp(n_s);
-  for (int i = 0; i < n_s; i++) {
-    const double s = s0 + ds * (double)i;
-    const double r = std::sqrt(s);
-    h[i] = spline7_value(c7, n_r, r0, inv_dr, r);
-    hp[i] = spline7_deriv(c7, n_r, r0, inv_dr, r) / (2.0 * r);
-  }
-  for (int m = 0; m < n_s; m++) {
-    double a1 = 0, a2 = 0, a3 = 0;
-    if (m < n_s - 1) {
-      const double D = h[m + 1] - h[m];
-      a1 = hp[m] * ds;
-      const double a1n = hp[m + 1] * ds;
-      a2 = 3.0 * D - 2.0 * a1 - a1n;
-      a3 = a1 + a1n - 2.0 * D;
+static MixHost pack_mixed(Arena &A, const double *knots, int nfun, int n_r, double r0, double dr, int n_s, double s0,
+                          double ds, std::vector<double> *db_out = nullptr, std::vector<float> *dc_out = nullptr) {
+  std::vector<double> c7((size_t)n_r * 7), h(n_s), hp(n_s), vb((size_t)nfun * n_s * 2), db(vb.size());
+  std::vector<float> vc((size_t)nfun * n_s * 2), dc((size_t)nfun * n_s);
+  const double inv_dr = 1.0 / dr;
+  for (int f = 0; f < nfun; f++) {
+    spline7(knots + (size_t)f * n_r, n_r, dr, c7.data());
+    for (int i = 0; i < n_s; i++) {
+      const double s = s0 + ds * (double)i;
+      const double r = std::sqrt(s);
+      h[i] = spline7_value(c7.data(), n_r, r0, inv_dr, r);
+      hp[i] = spline7_deriv(c7.data(), n_r, r0, inv_dr, r) / (2.0 * r);
     }
-    const float fa[2] = {(float)a2, (float)a3};
-    vrec[4 * m + 0] = h[m];
-    vrec[4 * m + 1] = a1;
-    std::memcpy(&vrec[4 * m + 2], fa, 8);
-    vrec[4 * m + 3] = 0.0;
-    const float fb[2] = {(float)(3.0 * a3 / ds), 0.0f};
-    drec[4 * m + 0] = hp[m];
-    drec[4 * m + 1] = 2.0 * a2 / ds;
-    std::memcpy(&drec[4 * m + 2], fb, 8);
-    drec[4 * m + 3] = 0.0;
+    for (int m = 0; m < n_s; m++) {
+      double a1 = 0, a2 = 0, a3 = 0;
+      if (m < n_s - 1) {
+        const double D = h[m + 1] - h[m];
+        a1 = hp[m] * ds;
+        const double a1n = hp[m + 1] * ds;
+        a2 = 3.0 * D - 2.0 * a1 - a1n;
+        a3 = a1 + a1n - 2.0 * D;
+      }
+      const size_t r = (size_t)f * n_s + m;
+      vb[2 * r] = h[m], vb[2 * r + 1] = a1;
+      vc[2 * r] = (float)a2, vc[2 * r + 1] = (float)a3;
+      db[2 * r] = hp[m], db[2 * r + 1] = 2.0 * a2 / ds;
+      dc[r] = (float)(3.0 * a3 / ds);
+    }
   }
+  if (db_out) *db_out = db;
+  if (dc_out) *dc_out = dc;
+  MixHost o;
+  o.vb = A.put(vb.data(), vb.size() * 8);
+  o.vc = A.put(vc.data(), vc.size() * 4);
+  o.db = A.put(db.data(), db.size() * 8);
+  o.dc = A.put(dc.data(), dc.size() * 4);
+  return o;
+}
+
+// FP64 force records per type pair (a, b): derivative coefficients of rho_b
+// and phi_ab per knot, three double2 (see TabF64::fc).
+static size_t pack_force_f64(Arena &A, const double *rho_r, const double *phi_r, int nt, int n, double dx) {
+  std::vector<double> cr((size_t)n * 7), cp((size_t)n * 7), out((size_t)nt * nt * n * 6);
+  for (int a = 0; a < nt; a++)
+    for (int b = 0; b < nt; b++) {
+      spline7(rho_r + (size_t)b * n, n, dx, cr.data());
+      spline7(phi_r + ((size_t)a * nt + b) * n, n, dx, cp.data());
+      for (int m = 0; m < n; m++) {
+        double *o = &out[(((size_t)a * nt + b) * n + m) * 6];
+        const double *r = &cr[(size_t)m * 7], *q = &cp[(size_t)m * 7];
+        o[0] = r[0], o[1] = r[1], o[2] = r[2], o[3] = q[0], o[4] = q[1], o[5] = q[2];
+      }
+    }
+  return A.put(out.data(), out.size() * 8);
 }
 
 extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
@@ -404,63 +479,65 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
           return fail(MMD_E_INVALID, "mmd_set_tables: phi_%d%d != phi_%d%d (pair tables must be symmetric)", a, b, b,
                       a);
     }
-    const size_t n_fun_r = (size_t)nt + (size_t)nt * nt;
-    const size_t words = (n_fun_r * nr + (size_t)nt * nrho) * 8;
-    std::vector<double> host(words, 0.0), c7((size_t)std::max(nr, nrho) * 7);
-    auto pack = [&](const double *f, int n, double dx, double *dst) {
-      spline7(f, n, dx, c7.data());
-      for (int m = 0; m < n; m++) {
-        const double *k = &c7[(size_t)m * 7];
-        double *o = dst + (size_t)m * 8;
-        o[0] = k[3], o[1] = k[4], o[2] = k[5], o[3] = k[6];
-        o[4] = k[0], o[5] = k[1], o[6] = k[2], o[7] = 0.0;
-      }
-    };
-    double *h_rho = host.data();
-    double *h_emb = h_rho + (size_t)nt * nr * 8;
-    double *h_phi = h_emb + (size_t)nt * nrho * 8;
-    for (int a = 0; a < nt; a++) pack(t->rho_r + (size_t)a * nr, nr, t->dr, h_rho + (size_t)a * nr * 8);
-    for (int a = 0; a < nt; a++) pack(t->emb + (size_t)a * nrho, nrho, t->drho, h_emb + (size_t)a * nrho * 8);
-    for (int p = 0; p < nt * nt; p++) pack(t->phi_r + (size_t)p * nr, nr, t->dr, h_phi + (size_t)p * nr * 8);
-
-    // mixed: r^2-keyed, n_s knots over [r0^2, rmax^2]
-    const int n_s = c->mixed_knots > 0 ? c->mixed_knots : nr;
+    Arena A;
+    const SplHost h_rho = pack_f64(A, t->rho_r, nt, nr, t->dr);
+    const SplHost h_emb = pack_f64(A, t->emb, nt, nrho, t->drho);
+    const SplHost h_phi = pack_f64(A, t->phi_r, nt * nt, nr, t->dr);
+    // mixed: r^2-keyed, n_s knots over [r0^2, rmax^2]; a quarter of the r
+    // knots keeps the resampling error near 1e-8 of the forces
+    const int n_s = c->mixed_knots > 0 ? c->mixed_knots : std::max(256, (nr + 3) / 4);
     const double rmax = t->r0 + t->dr * (double)(nr - 1);
     const double s0 = t->r0 * t->r0, ds = (rmax * rmax - s0) / (double)(n_s - 1);
-    const size_t vwords = (size_t)n_s * 4, dwords = (size_t)n_s * 4;
-    std::vector<double> mixed(n_fun_r * (vwords + dwords), 0.0);
-    double *m_rho_v = mixed.data();
-    double *m_phi_v = m_rho_v + (size_t)nt * vwords;
-    double *m_rho_d = m_phi_v + (size_t)nt * nt * vwords;
-    double *m_phi_d = m_rho_d + (size_t)nt * dwords;
-    const double inv_dr = 1.0 / t->dr;
-    for (int a = 0; a < nt; a++) {
-      spline7(t->rho_r + (size_t)a * nr, nr, t->dr, c7.data());
-      build_mixed(c7.data(), nr, t->r0, inv_dr, n_s, s0, ds, m_rho_v + a * vwords, m_rho_d + a * dwords);
-    }
-    for (int p = 0; p < nt * nt; p++) {
-      spline7(t->phi_r + (size_t)p * nr, nr, t->dr, c7.data());
-      build_mixed(c7.data(), nr, t->r0, inv_dr, n_s, s0, ds, m_phi_v + p * vwords, m_phi_d + p * dwords);
-    }
+    std::vector<double> rdb, pdb;
+    std::vector<float> rdc, pdc;
+    const MixHost m_rho = pack_mixed(A, t->rho_r, nt, nr, t->r0, t->dr, n_s, s0, ds, &rdb, &rdc);
+    const MixHost m_phi = pack_mixed(A, t->phi_r, nt * nt, nr, t->r0, t->dr, n_s, s0, ds, &pdb, &pdc);
+    const size_t h_fc = pack_force_f64(A, t->rho_r, t->phi_r, nt, nr, t->dr);
+    // mixed force records per type pair (a, b): (rho_b dbase, b1), (phi_ab dbase, b1), (rho_b b2, phi_ab b2)
+    std::vector<double> fb((size_t)nt * nt * n_s * 4);
+    std::vector<float> fq((size_t)nt * nt * n_s * 2);
+    for (int a = 0; a < nt; a++)
+      for (int b = 0; b < nt; b++)
+        for (int m = 0; m < n_s; m++) {
+          const size_t r = ((size_t)a * nt + b) * n_s + m, rr = (size_t)b * n_s + m;
+          fb[4 * r + 0] = rdb[2 * rr], fb[4 * r + 1] = rdb[2 * rr + 1];
+          fb[4 * r + 2] = pdb[2 * r], fb[4 * r + 3] = pdb[2 * r + 1];
+          fq[2 * r + 0] = rdc[rr], fq[2 * r + 1] = pdc[r];
+        }
+    const size_t h_fb = A.put(fb.data(), fb.size() * 8), h_fq = A.put(fq.data(), fq.size() * 4);
 
     HIP_TRY(hipStreamSynchronize(c->stream));
     ctx_drop_graphs(c);
     dfree(c->d_tab), dfree(c->d_mixed);
-    if ((rc = dalloc(&c->d_tab, words))) return rc;
-    if ((rc = dalloc(&c->d_mixed, mixed.size()))) return rc;
-    HIP_TRY(hipMemcpy(c->d_tab, host.data(), words * sizeof(double), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->d_mixed, mixed.data(), mixed.size() * sizeof(double), hipMemcpyHostToDevice));
+    char *dev = nullptr;
+    if ((rc = dalloc(&dev, A.buf.size()))) return rc;
+    c->d_tab = reinterpret_cast<double *>(dev);
+    HIP_TRY(hipMemcpy(dev, A.buf.data(), A.buf.size(), hipMemcpyHostToDevice));
+    auto F = [&](const SplHost &h) {
+      SplF64 s;
+      s.v34 = reinterpret_cast<const double2 *>(dev + h.v34);
+      s.v56 = reinterpret_cast<const double2 *>(dev + h.v56);
+      s.d01 = reinterpret_cast<const double2 *>(dev + h.d01);
+      s.d2 = reinterpret_cast<const double *>(dev + h.d2);
+      return s;
+    };
+    auto M = [&](const MixHost &h) {
+      SplMixed s;
+      s.vb = reinterpret_cast<const double2 *>(dev + h.vb);
+      s.vc = reinterpret_cast<const float2 *>(dev + h.vc);
+      s.db = reinterpret_cast<const double2 *>(dev + h.db);
+      s.dc = reinterpret_cast<const float *>(dev + h.dc);
+      return s;
+    };
     TabF64 &tf = c->tf;
-    tf.rho = c->d_tab;
-    tf.emb = c->d_tab + (h_emb - h_rho);
-    tf.phi = c->d_tab + (h_phi - h_rho);
+    tf.rho = F(h_rho), tf.emb = F(h_emb), tf.phi = F(h_phi);
+    tf.fc = reinterpret_cast<const double2 *>(dev + h_fc);
     tf.nt = nt, tf.n_r = nr, tf.n_rho = nrho;
-    tf.r0 = t->r0, tf.inv_dr = inv_dr, tf.rho0 = t->rho0, tf.inv_drho = 1.0 / t->drho;
+    tf.r0 = t->r0, tf.inv_dr = 1.0 / t->dr, tf.rho0 = t->rho0, tf.inv_drho = 1.0 / t->drho;
     TabMixed &tm = c->tm;
-    tm.rho_v = reinterpret_cast<const double4 *>(c->d_mixed);
-    tm.phi_v = reinterpret_cast<const double4 *>(c->d_mixed + (m_phi_v - m_rho_v));
-    tm.rho_d = reinterpret_cast<const double4 *>(c->d_mixed + (m_rho_d - m_rho_v));
-    tm.phi_d = reinterpret_cast<const double4 *>(c->d_mixed + (m_phi_d - m_rho_v));
+    tm.rho = M(m_rho), tm.phi = M(m_phi);
+    tm.fb = reinterpret_cast<const double2 *>(dev + h_fb);
+    tm.fq = reinterpret_cast<const float2 *>(dev + h_fq);
     tm.nt = nt, tm.n_s = n_s, tm.s0 = s0, tm.inv_ds = 1.0 / ds;
     const double r_head = t->r0 + 3.0 * t->dr, r_tail = t->r0 + (double)(nr - 4) * t->dr;
     tm.s_head = r_head * r_head, tm.s_tail = r_tail * r_tail;
@@ -520,8 +597,7 @@ extern "C" int mmd_upload(mmd_ctx *c, uint32_t fields, const mmd_host_soa *h) {
       uint32_t bit;
       const double *src;
       double *dst;
-    } fs[] = {{MMD_F_X, h->x, c->x},     {MMD_F_X, h->y, c->y},     {MMD_F_X, h->z, c->z},
-              {MMD_F_V, h->vx, c->vx},   {MMD_F_V, h->vy, c->vy},   {MMD_F_V, h->vz, c->vz},
+    } fs[] = {{MMD_F_V, h->vx, c->vx},   {MMD_F_V, h->vy, c->vy},   {MMD_F_V, h->vz, c->vz},
               {MMD_F_F, h->fx, c->fx},   {MMD_F_F, h->fy, c->fy},   {MMD_F_F, h->fz, c->fz},
               {MMD_F_RHO, h->rho, c->rho}, {MMD_F_DF, h->df, c->df}};
     for (auto &f : fs) {
@@ -530,7 +606,10 @@ extern "C" int mmd_upload(mmd_ctx *c, uint32_t fields, const mmd_host_soa *h) {
       HIP_TRY(hipMemcpyAsync(c->stage, f.src, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
       HIP_TRY(launch_ref2dev_f64(c->stream, c->g, c->stage, f.dst));
     }
-    if (fields & MMD_F_X) c->itl_stale = true;
+    if (fields & MMD_F_X) {
+      if (!h->x || !h->y || !h->z) return fail(MMD_E_INVALID, "mmd_upload: position pointers missing");
+      c->itl_stale = true;
+    }
     if (fields & MMD_F_TYPE) {
       if (!h->type) return fail(MMD_E_INVALID, "mmd_upload: type pointer missing");
       int32_t mx;
@@ -540,6 +619,15 @@ extern "C" int mmd_upload(mmd_ctx *c, uint32_t fields, const mmd_host_soa *h) {
       int8_t *st = reinterpret_cast<int8_t *>(c->stage);
       HIP_TRY(hipMemcpyAsync(st, h->type, n, hipMemcpyHostToDevice, c->stream));
       HIP_TRY(launch_ref2dev_i8(c->stream, c->g, st, c->type));
+    }
+    if (fields & (MMD_F_X | MMD_F_TYPE)) {
+      double *sx = c->stage, *sy = c->stage + n, *sz = c->stage + 2 * n;
+      if (fields & MMD_F_X) {
+        HIP_TRY(hipMemcpyAsync(sx, h->x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(sy, h->y, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(sz, h->z, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      }
+      HIP_TRY(launch_ref2dev_pos(c->stream, c->g, sx, sy, sz, c->type, c->pos, (fields & MMD_F_X) != 0));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     return MMD_OK;
@@ -558,8 +646,7 @@ extern "C" int mmd_download(mmd_ctx *c, uint32_t fields, const mmd_host_soa *h) 
       uint32_t bit;
       double *dst;
       const double *src;
-    } fs[] = {{MMD_F_X, h->x, c->x},     {MMD_F_X, h->y, c->y},     {MMD_F_X, h->z, c->z},
-              {MMD_F_V, h->vx, c->vx},   {MMD_F_V, h->vy, c->vy},   {MMD_F_V, h->vz, c->vz},
+    } fs[] = {{MMD_F_V, h->vx, c->vx},   {MMD_F_V, h->vy, c->vy},   {MMD_F_V, h->vz, c->vz},
               {MMD_F_F, h->fx, c->fx},   {MMD_F_F, h->fy, c->fy},   {MMD_F_F, h->fz, c->fz},
               {MMD_F_RHO, h->rho, c->rho}, {MMD_F_DF, h->df, c->df}};
     for (auto &f : fs) {
@@ -567,6 +654,15 @@ extern "C" int mmd_download(mmd_ctx *c, uint32_t fields, const mmd_host_soa *h) 
       if (!f.dst) return fail(MMD_E_INVALID, "mmd_download: host pointer missing for field mask 0x%x", f.bit);
       HIP_TRY(launch_dev2ref_f64(c->stream, c->g, f.src, c->stage));
       HIP_TRY(hipMemcpyAsync(f.dst, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    if (fields & MMD_F_X) {
+      if (!h->x || !h->y || !h->z) return fail(MMD_E_INVALID, "mmd_download: position pointers missing");
+      double *sx = c->stage, *sy = c->stage + n, *sz = c->stage + 2 * n;
+      HIP_TRY(launch_dev2ref_pos(c->stream, c->g, c->pos, sx, sy, sz));
+      HIP_TRY(hipMemcpyAsync(h->x, sx, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipMemcpyAsync(h->y, sy, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipMemcpyAsync(h->z, sz, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
     }
     if (fields & MMD_F_TYPE) {
@@ -667,7 +763,8 @@ static int aos_pass(mmd_ctx *c, void *atoms, const mmd_aos_layout *lay, uint32_t
     c->aos_bytes = bytes;
   }
   HIP_TRY(hipMemcpyAsync(c->aos, atoms, bytes, hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(launch_aos_unpack(c->stream, c->g, c->aos, d, in, c->x, c->y, c->z, c->type, c->df, c->rho));
+  HIP_TRY(launch_aos_unpack(c->stream, c->g, c->aos, d, in, c->pos, c->type, c->df, c->rho));
+  if (in & MMD_F_X) c->itl_stale = true;
   if (pass == 0) rc = run_rho_df(c, cutoff, false);
   if (pass == 1) {
     HIP_TRY(launch_df(c->stream, c->g, c->type, c->rho, c->df, c->tf));
@@ -871,4 +968,36 @@ extern "C" int mmd_timing_reset(mmd_ctx *c) {
   if ((rc = timing_collect(c))) return rc;
   for (int k = 0; k < MMD_K_COUNT; k++) c->t_ms[k] = 0.0, c->t_n[k] = 0;
   return MMD_OK;
+}
+
+extern "C" int mmd_stream_copy(mmd_ctx *c, size_t bytes, int32_t repeats, double *ms) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc) return rc;
+    if (bytes == 0 || bytes % 16 || repeats < 1) return fail(MMD_E_INVALID, "mmd_stream_copy: bytes % 16 == 0, repeats >= 1");
+    char *a = nullptr, *b = nullptr;
+    if ((rc = dalloc(&a, bytes))) return rc;
+    if ((rc = dalloc(&b, bytes))) {
+      dfree(a);
+      return rc;
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(launch_fill_f64(c->stream, reinterpret_cast<double *>(a), (int64_t)(bytes / 8), 1.0));
+    HIP_TRY(launch_stream_copy(c->stream, a, b, (int64_t)bytes));  // warm-up
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    for (int32_t r = 0; r < repeats; r++) HIP_TRY(launch_stream_copy(c->stream, a, b, (int64_t)bytes));
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t / repeats;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dfree(a);
+    dfree(b);
+    return MMD_OK;
+  })
 }

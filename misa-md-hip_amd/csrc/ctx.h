@@ -17,13 +17,14 @@ struct mmd_ctx {
   mmd_domain dom{};
   mmd::Geom g{};
 
-  // device-resident SoA state over the ghost-extended lattice (device layout)
-  double *x = nullptr, *y = nullptr, *z = nullptr;
+  // device-resident state over the ghost-extended lattice (device layout):
+  // pos[i] = (x, y, z, type) so a neighbour costs two 16-byte loads
+  double4 *pos = nullptr;
   double *vx = nullptr, *vy = nullptr, *vz = nullptr;
   double *fx = nullptr, *fy = nullptr, *fz = nullptr;
   double *rho = nullptr, *df = nullptr;
   int8_t *type = nullptr;
-  double *stage = nullptr;  // n_ext doubles: reference-layout staging for upload/download
+  double *stage = nullptr;  // 3 * n_ext doubles: reference-layout staging for upload/download
 
   // stencil: [2][kMaxStencil] device-layout offsets
   int32_t *d_off = nullptr;

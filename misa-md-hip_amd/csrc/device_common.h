@@ -30,33 +30,71 @@ struct Geom {
   int32_t gxh, gy, gz;          // ghost width (x in cells)
   int32_t bxh, by, bz;          // local box (x in cells)
   int32_t chunks;               // 64-lane chunks per local row
+  int32_t ty, n_ytiles;         // y rows per tile, tiles covering the local box
   int64_t n_ext;                // ext sites
-  int64_t n_waves;              // waves covering the local box
+  int64_t n_waves;              // waves covering the local box (tiles padded to ty rows)
 };
 
-// Per-knot coefficients of an FP64 r- or rho-keyed spline: [c3 c4 c5 c6 | c0 c1 c2 0]
-// value = ((c3 p + c4) p + c5) p + c6, derivative = (c0 p + c1) p + c2.
+// Loads through a uniform base pointer with a 32-bit element index, so the
+// address is an SGPR base + 32-bit VGPR offset (no 64-bit VALU arithmetic).
+template <typename T>
+__device__ __forceinline__ T ld(const T *__restrict__ base, uint32_t idx) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + idx * (uint32_t)sizeof(T));
+}
+
+// FP64 cubic spline of one function family (all types or type pairs back to
+// back, record = function * n + knot).  The seven coefficients per knot are
+// split by width so a lane's gather touches as few cache lines as possible:
+//   value = ((c3 p + c4) p + c5) p + c6   from v34 = (c3, c4), v56 = (c5, c6)
+//   deriv = (c0 p + c1) p + c2            from d01 = (c0, c1), d2 = c2
+struct SplF64 {
+  const double2 *v34, *v56, *d01;
+  const double *d2;
+};
+
 struct TabF64 {
-  const double *rho;  // [nt][n_r][8]
-  const double *emb;  // [nt][n_rho][8]
-  const double *phi;  // [nt*nt][n_r][8]
+  SplF64 rho;  // [nt][n_r]      rho_t(r) by source type
+  SplF64 emb;  // [nt][n_rho]    F_t(rho)
+  SplF64 phi;  // [nt*nt][n_r]   phi_ab(r)
+  // force records per type pair (a, b) and knot: the derivative coefficients
+  // of rho_b and phi_ab side by side, three 16-byte loads per pair:
+  //   fc[3r] = (rho_b c0, c1), fc[3r+1] = (rho_b c2, phi_ab c0), fc[3r+2] = (phi_ab c1, c2)
+  const double2 *fc;  // [nt*nt][n_r][3]
   int32_t nt, n_r, n_rho;
   double r0, inv_dr, rho0, inv_drho;
 };
+
+__device__ __forceinline__ double spl_value(const SplF64 &s, uint32_t rec, double p) {
+  const double2 a = ld(s.v34, rec), b = ld(s.v56, rec);
+  return ((a.x * p + a.y) * p + b.x) * p + b.y;
+}
+
+__device__ __forceinline__ double spl_deriv(const SplF64 &s, uint32_t rec, double p) {
+  const double2 a = ld(s.d01, rec);
+  return (a.x * p + a.y) * p + ld(s.d2, rec);
+}
 
 // Mixed-precision spline keyed by s = r^2 (no square root or division per
 // pair).  Per knot, with q the fraction of the segment:
 //   value = base + q * (a1 + q * c),   c = a2 + q * a3   (c in FP32)
 //   d/ds  = dbase + q * (b1 + q * b2)                     (b2 stored as FP32)
 // The FP64 base and linear terms carry the leading digits; the FP32 terms are
-// O(ds^2) corrections, so their rounding stays below 1e-13 relative.
-// Value record (32 B): {double base; double a1; float a2, a3; double 0}
-// Deriv record (32 B): {double dbase; double b1; float b2, 0; double 0}
+// O(ds^2) corrections, so their rounding stays far below the interpolation
+// error of the resampled table.
+struct SplMixed {
+  const double2 *vb;  // (base, a1)
+  const float2 *vc;   // (a2, a3)
+  const double2 *db;  // (dbase, b1)
+  const float *dc;    // b2
+};
+
 struct TabMixed {
-  const double4 *rho_v;   // [nt][n_s] value records (rho by source type)
-  const double4 *rho_d;   // [nt][n_s] derivative records
-  const double4 *phi_d;   // [nt*nt][n_s]
-  const double4 *phi_v;   // [nt*nt][n_s]
+  SplMixed rho;  // [nt][n_s]
+  SplMixed phi;  // [nt*nt][n_s]
+  // force records per type pair (a, b) and knot: fb[2r] = (rho_b dbase, b1),
+  // fb[2r+1] = (phi_ab dbase, b1), fq[r] = (rho_b b2, phi_ab b2)
+  const double2 *fb;  // [nt*nt][n_s][2]
+  const float2 *fq;   // [nt*nt][n_s]
   int32_t nt, n_s;
   double s0, inv_ds;
   // The reference spline's first and last three segments use low-order end
@@ -64,6 +102,18 @@ struct TabMixed {
   // pairs with s < s_head or s >= s_tail evaluate the FP64 r-spline instead.
   double s_head, s_tail;
 };
+
+__device__ __forceinline__ double mix_value(const SplMixed &s, uint32_t rec, double q) {
+  const double2 b = ld(s.vb, rec);
+  const float2 c = ld(s.vc, rec);
+  const float cc = fmaf((float)q, c.y, c.x);
+  return fma(q, fma(q, (double)cc, b.y), b.x);
+}
+
+__device__ __forceinline__ double mix_deriv(const SplMixed &s, uint32_t rec, double q) {
+  const double2 b = ld(s.db, rec);
+  return fma(q, fma(q, (double)ld(s.dc, rec), b.y), b.x);
+}
 
 struct Masses {
   double inv_m[kMaxTypes];  // 1 / mass (amu^-1)
@@ -100,8 +150,10 @@ __device__ inline double block_sum(double v, double *lds4) {
   return r;
 }
 
-// One thread per local site: wave w of the launch takes 64 consecutive cells
-// of local row w / chunks (rows ordered z, y, parity).
+// One thread per local site.  Waves walk the box in y-tiles of g.ty rows:
+// logical wave w -> (tile, z, y in tile, parity, 64-cell chunk), so the XCD
+// that owns a contiguous range of logical blocks sweeps z through a thin slab
+// and the +-2-plane neighbour window of that slab stays in its L2.
 
 struct Site {
   int64_t i;    // device index
@@ -116,8 +168,12 @@ __device__ __forceinline__ bool map_site(const Geom &g, Site &st) {
   const int64_t chunk = w % g.chunks;
   const int64_t rl = w / g.chunks;
   const int s = (int)(rl & 1);
-  const int64_t yz = rl >> 1;
-  const int yl = (int)(yz % g.by), zl = (int)(yz / g.by);
+  const int64_t r = rl >> 1;  // (tile, z, y in tile)
+  const int yin = (int)(r % g.ty);
+  const int64_t tz = r / g.ty;
+  const int zl = (int)(tz % g.bz);
+  const int yl = (int)(tz / g.bz) * g.ty + yin;
+  if (yl >= g.by) return false;
   const int xh = (int)chunk * kWave + lane;
   st.active = xh < g.bxh;
   st.s = __builtin_amdgcn_readfirstlane(s);

@@ -49,16 +49,6 @@ namespace {
 constexpr double kFtm2v = 9.64853321e-3;
 constexpr double kMvv2e = 1.0 / kFtm2v;
 
-__device__ __forceinline__ double spl_val(const double *tab, int rec, double p) {
-  const double *c = tab + (size_t)rec * 8;
-  return ((c[0] * p + c[1]) * p + c[2]) * p + c[3];
-}
-
-__device__ __forceinline__ double spl_der(const double *tab, int rec, double p) {
-  const double *c = tab + (size_t)rec * 8 + 4;
-  return (c[0] * p + c[1]) * p + c[2];
-}
-
 __device__ __forceinline__ int seg(double x, double x0, double inv_dx, int n, double &p) {
   double q = (x - x0) * inv_dx;
   q = fmax(q, 0.0);
@@ -82,9 +72,8 @@ __device__ __forceinline__ double minimg(double d, double L) { return d - L * ri
 
 // Is lattice site (sx, sy, sz) (local, x doubled) in W of interstitial m and
 // closer than rc?  Returns the ext device index of the image seen from m.
-__device__ __forceinline__ bool li_pair(const Geom &g, const ItlDev &I, const double *X, const double *Y,
-                                        const double *Z, int m, int sx, int sy, int sz, int64_t &e, double &d2,
-                                        double &dx, double &dy, double &dz) {
+__device__ __forceinline__ bool li_pair(const Geom &g, const ItlDev &I, const double4 *P, int m, int sx, int sy,
+                                        int sz, int64_t &e, double &d2, double &dx, double &dy, double &dz) {
   const int aX = I.ax[m], aY = I.ay[m], aZ = I.az[m];
   const int dX = cdiff(sx - aX, 2 * g.bxh), dY = cdiff(sy - aY, g.by), dZ = cdiff(sz - aZ, g.bz);
   const int sa = aX & 1, s1 = (aX + dX) & 1;
@@ -92,9 +81,10 @@ __device__ __forceinline__ bool li_pair(const Geom &g, const ItlDev &I, const do
   if (dX * dX + hy * hy + hz * hz > I.max_d2q) return false;
   const int xe = aX + dX + 2 * g.gxh, ye = aY + dY + g.gy, ze = aZ + dZ + g.gz;
   e = dev_index(g, xe & 1, xe >> 1, ye, ze);
-  dx = X[e] - I.px[m];
-  dy = Y[e] - I.py[m];
-  dz = Z[e] - I.pz[m];
+  const double4 q = P[e];
+  dx = q.x - I.px[m];
+  dy = q.y - I.py[m];
+  dz = q.z - I.pz[m];
   d2 = dx * dx + dy * dy + dz * dz;
   return d2 < I.rc2;
 }
@@ -189,8 +179,7 @@ __device__ __forceinline__ void src_local(const Geom &g, int xe, int ye, int ze,
 // One wave per interstitial.  MODE 0: rho (+ owner-side lattice rho/df update),
 // MODE 1: force (+ owner-side lattice force update).
 template <int MODE, bool WANT_E>
-__global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const double *__restrict__ X,
-                                                       const double *__restrict__ Y, const double *__restrict__ Z,
+__global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const double4 *__restrict__ P,
                                                        const int8_t *__restrict__ T, double *__restrict__ RHO,
                                                        double *__restrict__ DF, double *__restrict__ FX,
                                                        double *__restrict__ FY, double *__restrict__ FZ, TabF64 tf,
@@ -212,18 +201,19 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
       const int64_t e = dev_index(g, xe & 1, xe >> 1, ye, ze);
       const int ti = T[e];
       if (ti < 0) continue;
-      const double dx = X[e] - I.px[k], dy = Y[e] - I.py[k], dz = Z[e] - I.pz[k];
+      const double4 q = P[e];
+      const double dx = q.x - I.px[k], dy = q.y - I.py[k], dz = q.z - I.pz[k];
       const double d2 = dx * dx + dy * dy + dz * dz;
       if (d2 >= I.rc2) continue;
       const double r = sqrt(d2);
       double p;
       const int m = seg(r, tf.r0, tf.inv_dr, tf.n_r, p);
       if (MODE == 0) {
-        acc += spl_val(tf.rho, ti * tf.n_r + m, p);
+        acc += spl_value(tf.rho, ti * tf.n_r + m, p);
       } else {
         const double dfi = DF[e];
-        const double psip = dfi * spl_der(tf.rho, tk * tf.n_r + m, p) + dfk * spl_der(tf.rho, ti * tf.n_r + m, p) +
-                            spl_der(tf.phi, (ti * tf.nt + tk) * tf.n_r + m, p);
+        const double psip = dfi * spl_deriv(tf.rho, tk * tf.n_r + m, p) + dfk * spl_deriv(tf.rho, ti * tf.n_r + m, p) +
+                            spl_deriv(tf.phi, (ti * tf.nt + tk) * tf.n_r + m, p);
         const double fp = -psip / r;  // force on the lattice atom along del = x_i - x_k
         fxk -= dx * fp, fyk -= dy * fp, fzk -= dz * fp;
       }
@@ -236,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
         if (mm > k) break;
         int64_t e2;
         double d2b, ex, ey, ez;
-        if (li_pair(g, I, X, Y, Z, mm, sx, sy, sz, e2, d2b, ex, ey, ez)) owner = false;
+        if (li_pair(g, I, P, mm, sx, sy, sz, e2, d2b, ex, ey, ez)) owner = false;
       }
       if (!owner) continue;
       const int64_t src = dev_index(g, sx & 1, (sx >> 1) + g.gxh, sy + g.gy, sz + g.gz);
@@ -255,20 +245,20 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
         }
         int64_t e2;
         double d2b, ex, ey, ez;
-        if (!li_pair(g, I, X, Y, Z, mm, sx, sy, sz, e2, d2b, ex, ey, ez)) continue;
+        if (!li_pair(g, I, P, mm, sx, sy, sz, e2, d2b, ex, ey, ez)) continue;
         const int tm = I.type[mm];
         const double rb = sqrt(d2b);
         double pb;
         const int mb = seg(rb, tf.r0, tf.inv_dr, tf.n_r, pb);
         if (MODE == 0) {
-          rho_i += spl_val(tf.rho, tm * tf.n_r + mb, pb);
+          rho_i += spl_value(tf.rho, tm * tf.n_r + mb, pb);
         } else {
-          const double psip = dfi * spl_der(tf.rho, tm * tf.n_r + mb, pb) +
-                              I.df[mm] * spl_der(tf.rho, ti * tf.n_r + mb, pb) +
-                              spl_der(tf.phi, (ti * tf.nt + tm) * tf.n_r + mb, pb);
+          const double psip = dfi * spl_deriv(tf.rho, tm * tf.n_r + mb, pb) +
+                              I.df[mm] * spl_deriv(tf.rho, ti * tf.n_r + mb, pb) +
+                              spl_deriv(tf.phi, (ti * tf.nt + tm) * tf.n_r + mb, pb);
           const double fp = -psip / rb;
           gx += ex * fp, gy += ey * fp, gz += ez * fp;
-          if (WANT_E) epair += spl_val(tf.phi, (ti * tf.nt + tm) * tf.n_r + mb, pb);
+          if (WANT_E) epair += spl_value(tf.phi, (ti * tf.nt + tm) * tf.n_r + mb, pb);
         }
       }
       if (MODE == 0) {
@@ -277,10 +267,10 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
         if (WANT_E) {
           double po;
           const int mo = seg(RHO[src], tf.rho0, tf.inv_drho, tf.n_rho, po);
-          e_loc += spl_val(tf.emb, ti * tf.n_rho + me, pe) - spl_val(tf.emb, ti * tf.n_rho + mo, po);
+          e_loc += spl_value(tf.emb, ti * tf.n_rho + me, pe) - spl_value(tf.emb, ti * tf.n_rho + mo, po);
         }
         RHO[src] = rho_i;
-        DF[src] = spl_der(tf.emb, ti * tf.n_rho + me, pe);
+        DF[src] = spl_deriv(tf.emb, ti * tf.n_rho + me, pe);
       } else {
         FX[src] += gx, FY[src] += gy, FZ[src] += gz;
         if (WANT_E) e_loc += epair;
@@ -298,13 +288,13 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
       double p;
       const int m = seg(r, tf.r0, tf.inv_dr, tf.n_r, p);
       if (MODE == 0) {
-        acc += spl_val(tf.rho, tm * tf.n_r + m, p);
+        acc += spl_value(tf.rho, tm * tf.n_r + m, p);
       } else {
-        const double psip = dfk * spl_der(tf.rho, tm * tf.n_r + m, p) + I.df[mm] * spl_der(tf.rho, tk * tf.n_r + m, p) +
-                            spl_der(tf.phi, (tk * tf.nt + tm) * tf.n_r + m, p);
+        const double psip = dfk * spl_deriv(tf.rho, tm * tf.n_r + m, p) + I.df[mm] * spl_deriv(tf.rho, tk * tf.n_r + m, p) +
+                            spl_deriv(tf.phi, (tk * tf.nt + tm) * tf.n_r + m, p);
         const double fp = -psip / r;
         fxk += dx * fp, fyk += dy * fp, fzk += dz * fp;
-        if (WANT_E) e_loc += 0.5 * spl_val(tf.phi, (tk * tf.nt + tm) * tf.n_r + m, p);
+        if (WANT_E) e_loc += 0.5 * spl_value(tf.phi, (tk * tf.nt + tm) * tf.n_r + m, p);
       }
     }
     if (MODE == 0) {
@@ -313,8 +303,8 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
         double pe;
         const int me = seg(acc, tf.rho0, tf.inv_drho, tf.n_rho, pe);
         I.rho[k] = acc;
-        I.df[k] = spl_der(tf.emb, tk * tf.n_rho + me, pe);
-        if (WANT_E) e_loc += spl_val(tf.emb, tk * tf.n_rho + me, pe);
+        I.df[k] = spl_deriv(tf.emb, tk * tf.n_rho + me, pe);
+        if (WANT_E) e_loc += spl_value(tf.emb, tk * tf.n_rho + me, pe);
       }
     } else {
       fxk = wave_sum(fxk), fyk = wave_sum(fyk), fzk = wave_sum(fzk);
@@ -332,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_itl_df(ItlDev I, TabF64 tf) {
   if (k >= I.n) return;
   double p;
   const int m = seg(I.rho[k], tf.rho0, tf.inv_drho, tf.n_rho, p);
-  I.df[k] = spl_der(tf.emb, I.type[k] * tf.n_rho + m, p);
+  I.df[k] = spl_deriv(tf.emb, I.type[k] * tf.n_rho + m, p);
 }
 
 template <bool DRIFT, bool WANT_KE>
@@ -467,10 +457,10 @@ int itl_rho_df(mmd_ctx *c, double cutoff, bool want_e) {
   ItlDev I = make_dev(c);
   const dim3 grid((c->n_itl + kWavesPerBlock - 1) / kWavesPerBlock);
   if (want_e)
-    k_itl_pass<0, true><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->x, c->y, c->z, c->type, c->rho, c->df, c->fx,
+    k_itl_pass<0, true><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->type, c->rho, c->df, c->fx,
                                                           c->fy, c->fz, c->tf, c->e_itl);
   else
-    k_itl_pass<0, false><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->x, c->y, c->z, c->type, c->rho, c->df, c->fx,
+    k_itl_pass<0, false><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->type, c->rho, c->df, c->fx,
                                                            c->fy, c->fz, c->tf, c->e_itl);
   ITL_TRY(hipGetLastError());
   return timed_end(c, MMD_K_ITL, ev);
@@ -497,10 +487,10 @@ int itl_force(mmd_ctx *c, double cutoff, bool want_e) {
   // the embedding-energy partials of the rho pass live in the first half of e_itl
   double *ep = c->e_itl + c->itl_part_n / 2;
   if (want_e)
-    k_itl_pass<1, true><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->x, c->y, c->z, c->type, c->rho, c->df, c->fx,
+    k_itl_pass<1, true><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->type, c->rho, c->df, c->fx,
                                                           c->fy, c->fz, c->tf, ep);
   else
-    k_itl_pass<1, false><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->x, c->y, c->z, c->type, c->rho, c->df, c->fx,
+    k_itl_pass<1, false><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->type, c->rho, c->df, c->fx,
                                                            c->fy, c->fz, c->tf, ep);
   ITL_TRY(hipGetLastError());
   return timed_end(c, MMD_K_ITL, ev);

@@ -11,16 +11,20 @@ namespace mmd {
 int64_t site_blocks(const Geom &g);
 
 // EAM passes
-hipError_t launch_rho_df(hipStream_t st, const Geom &g, const int32_t *off, int32_t nk_even, int32_t nk_odd, const double *X,
-                         const double *Y, const double *Z, const int8_t *T, double *RHO, double *DF,
-                         const TabF64 &tf, const TabMixed &tm, double rc2, int prec, double *e_part);
+hipError_t launch_rho_df(hipStream_t st, const Geom &g, const int32_t *off, int32_t nk_even, int32_t nk_odd,
+                         const double4 *P, double *RHO, double *DF, const TabF64 &tf, const TabMixed &tm, double rc2,
+                         int prec, double *e_part);
 hipError_t launch_df(hipStream_t st, const Geom &g, const int8_t *T, const double *RHO, double *DF,
                      const TabF64 &tf);
-hipError_t launch_force(hipStream_t st, const Geom &g, const int32_t *off, int32_t nk_even, int32_t nk_odd, const double *X,
-                        const double *Y, const double *Z, const int8_t *T, const double *DF, double *FX, double *FY,
-                        double *FZ, const TabF64 &tf, const TabMixed &tm, double rc2, int prec, double *e_part);
+hipError_t launch_force(hipStream_t st, const Geom &g, const int32_t *off, int32_t nk_even, int32_t nk_odd,
+                        const double4 *P, const double *DF, double *FX, double *FY, double *FZ, const TabF64 &tf,
+                        const TabMixed &tm, double rc2, int prec, double *e_part);
 
-// layout conversion between the reference layout (host view) and the device layout
+// layout conversion between the reference layout (host view) and the device layout;
+// positions: three staged reference-layout arrays <-> (x, y, z, type) records
+hipError_t launch_ref2dev_pos(hipStream_t st, const Geom &g, const double *rx, const double *ry, const double *rz,
+                              const int8_t *T, double4 *P, bool set_xyz);
+hipError_t launch_dev2ref_pos(hipStream_t st, const Geom &g, const double4 *P, double *rx, double *ry, double *rz);
 hipError_t launch_ref2dev_f64(hipStream_t st, const Geom &g, const double *ref, double *dev);
 hipError_t launch_dev2ref_f64(hipStream_t st, const Geom &g, const double *dev, double *ref);
 hipError_t launch_ref2dev_i8(hipStream_t st, const Geom &g, const int8_t *ref, int8_t *dev);
@@ -31,26 +35,27 @@ struct AosDesc {
   int64_t stride, off_type, off_x, off_v, off_f, off_rho, off_df;
   int32_t type_bytes;
 };
-hipError_t launch_aos_unpack(hipStream_t st, const Geom &g, const char *aos, AosDesc d, uint32_t fields, double *X,
-                             double *Y, double *Z, int8_t *T, double *DF, double *RHO);
+hipError_t launch_aos_unpack(hipStream_t st, const Geom &g, const char *aos, AosDesc d, uint32_t fields, double4 *P,
+                             int8_t *T, double *DF, double *RHO);
 hipError_t launch_aos_pack(hipStream_t st, const Geom &g, char *aos, AosDesc d, uint32_t fields, const int8_t *T,
                            const double *RHO, const double *DF, const double *FX, const double *FY,
                            const double *FZ);
 
 // periodic single-domain ghost refresh
-hipError_t launch_ghost(hipStream_t st, const Geom &g, double Lx, double Ly, double Lz, double *X, double *Y,
-                        double *Z, int8_t *T, double *DF, uint32_t fields);
+hipError_t launch_ghost(hipStream_t st, const Geom &g, double Lx, double Ly, double Lz, double4 *P, int8_t *T,
+                        double *DF, uint32_t fields);
 
 // velocity Verlet: v += h a; optionally x += dt v.  ke_part (optional) gets the
 // kinetic energy partials of the updated velocities (half-kick variant).
-hipError_t launch_kick(hipStream_t st, const Geom &g, const Masses &M, double h, double dt, bool drift, double *X,
-                       double *Y, double *Z, double *VX, double *VY, double *VZ, const double *FX, const double *FY,
-                       const double *FZ, const int8_t *T, double *ke_part);
+hipError_t launch_kick(hipStream_t st, const Geom &g, const Masses &M, double h, double dt, bool drift, double4 *P,
+                       double *VX, double *VY, double *VZ, const double *FX, const double *FY, const double *FZ,
+                       const int8_t *T, double *ke_part);
 
 // energies: out[slot] = {sum(ep_a) + sum(ep_b), sum(ke)}, slot = (*counter)++
 hipError_t launch_energy_finalize(hipStream_t st, const double *ep_a, int64_t na, const double *ep_b, int64_t nb,
                                   const double *ep_c, int64_t nc, const double *ke, int64_t nk, const double *ke_b,
                                   int64_t nkb, double *out, int32_t *counter);
+hipError_t launch_stream_copy(hipStream_t st, const void *src, void *dst, int64_t bytes);
 hipError_t launch_fill_i32(hipStream_t st, int32_t *p, int64_t n, int32_t v);
 hipError_t launch_fill_f64(hipStream_t st, double *p, int64_t n, double v);
 hipError_t launch_fill_i8(hipStream_t st, int8_t *p, int64_t n, int8_t v);

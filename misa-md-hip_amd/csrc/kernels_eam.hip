@@ -17,13 +17,6 @@ namespace mmd {
 #endif
 constexpr int kUnroll = MMD_PAIR_UNROLL;  // stencil entries whose loads are in flight together
 
-// Loads through a uniform base pointer with a 32-bit element index, so the
-// address is an SGPR base + 32-bit VGPR offset (no 64-bit VALU arithmetic).
-template <typename T>
-__device__ __forceinline__ T ld(const T *__restrict__ base, uint32_t idx) {
-  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + idx * (uint32_t)sizeof(T));
-}
-
 // 1/sqrt(d2) for d2 > 0: hardware reciprocal-square-root seed refined by two
 // Newton steps to full FP64 accuracy (a few ulp), instead of the IEEE sqrt and
 // divide sequences.
@@ -53,17 +46,6 @@ __device__ __forceinline__ Seg seg_f64(double x, double x0, double inv_dx, int n
   return s;
 }
 
-// record = knot index (+ function offset); 8 doubles per record
-__device__ __forceinline__ double val8(const double *__restrict__ tab, uint32_t rec, double p) {
-  const double4 c = ld(reinterpret_cast<const double4 *>(tab), 2 * rec);
-  return ((c.x * p + c.y) * p + c.z) * p + c.w;
-}
-
-__device__ __forceinline__ double der8(const double *__restrict__ tab, uint32_t rec, double p) {
-  const double4 c = ld(reinterpret_cast<const double4 *>(tab), 2 * rec + 1);
-  return (c.x * p + c.y) * p + c.z;
-}
-
 struct SegS {
   int m;
   double q;
@@ -79,19 +61,6 @@ __device__ __forceinline__ SegS seg_s(double s2, const TabMixed &t) {
   return r;
 }
 
-__device__ __forceinline__ double valS(const double4 *__restrict__ tab, uint32_t rec, double q) {
-  const double4 c = ld(tab, rec);
-  const float2 a23 = __builtin_bit_cast(float2, c.z);
-  const float cc = fmaf((float)q, a23.y, a23.x);  // FP32 quadratic/cubic correction
-  return fma(q, fma(q, (double)cc, c.y), c.x);
-}
-
-__device__ __forceinline__ double derS(const double4 *__restrict__ tab, uint32_t rec, double q) {
-  const double4 c = ld(tab, rec);
-  const float2 b2 = __builtin_bit_cast(float2, c.z);
-  return fma(q, fma(q, (double)b2.x, c.y), c.x);
-}
-
 // ------------------------------------------------------------- pair terms
 
 // rho_{tj}(r) for r^2 = d2
@@ -99,40 +68,46 @@ template <int PREC>
 __device__ __forceinline__ double rho_pair(const TabF64 &tf, const TabMixed &tm, int tj, double d2) {
   if (PREC == 1 && d2 < tm.s_tail && d2 >= tm.s_head) {
     const SegS sg = seg_s(d2, tm);
-    return valS(tm.rho_v, (uint32_t)(tj * tm.n_s + sg.m), sg.q);
+    return mix_value(tm.rho, (uint32_t)(tj * tm.n_s + sg.m), sg.q);
   }
   const double r = d2 * rsqrt64(d2);
   const Seg sg = seg_f64(r, tf.r0, tf.inv_dr, tf.n_r);
-  return val8(tf.rho, (uint32_t)(tj * tf.n_r + sg.m), sg.p);
+  return spl_value(tf.rho, (uint32_t)(tj * tf.n_r + sg.m), sg.p);
 }
 
 // fpair = -(df_i rho'_{tj} + df_j rho'_{ti} + phi'_{ti tj}) / r (the scalar that
 // multiplies del = x_i - x_j); phi receives phi_{ti tj}(r) when WANT_E.
+// rho'_{tj} and phi'_{ti tj} come from one combined per-type-pair record;
+// rho'_{ti} is gathered separately, by unlike-type pairs only.
 template <int PREC, bool WANT_E>
 __device__ __forceinline__ double fpair(const TabF64 &tf, const TabMixed &tm, int ti, int tj, double d2, double dfi,
                                         double dfj, double &phi) {
+  const int tp = ti * tf.nt + tj;
   if (PREC == 1 && d2 < tm.s_tail && d2 >= tm.s_head) {
     // r^2-keyed: d/ds tables, -(dpsi/dr)/r = -2 dpsi/ds
     const SegS sg = seg_s(d2, tm);
-    const uint32_t rj = (uint32_t)(tj * tm.n_s + sg.m);
-    const uint32_t rp = (uint32_t)((ti * tm.nt + tj) * tm.n_s + sg.m);
-    const double rhojp = derS(tm.rho_d, rj, sg.q);
+    const uint32_t rp = (uint32_t)(tp * tm.n_s + sg.m);
+    const double2 b0 = ld(tm.fb, 2 * rp), b1 = ld(tm.fb, 2 * rp + 1);
+    const float2 q2 = ld(tm.fq, rp);
+    const double q = sg.q;
+    const double rhojp = fma(q, fma(q, (double)q2.x, b0.y), b0.x);
+    const double phip = fma(q, fma(q, (double)q2.y, b1.y), b1.x);
     double rhoip = rhojp;
-    if (ti != tj) rhoip = derS(tm.rho_d, (uint32_t)(ti * tm.n_s + sg.m), sg.q);
-    const double phip = derS(tm.phi_d, rp, sg.q);
-    if (WANT_E) phi = valS(tm.phi_v, rp, sg.q);
+    if (ti != tj) rhoip = mix_deriv(tm.rho, (uint32_t)(ti * tm.n_s + sg.m), q);
+    if (WANT_E) phi = mix_value(tm.phi, rp, q);
     return -2.0 * (dfi * rhojp + dfj * rhoip + phip);
   }
   const double rinv = rsqrt64(d2);
   const double r = d2 * rinv;
   const Seg sg = seg_f64(r, tf.r0, tf.inv_dr, tf.n_r);
-  const uint32_t rj = (uint32_t)(tj * tf.n_r + sg.m);
-  const uint32_t rp = (uint32_t)((ti * tf.nt + tj) * tf.n_r + sg.m);
-  const double rhojp = der8(tf.rho, rj, sg.p);
+  const uint32_t rp = (uint32_t)(tp * tf.n_r + sg.m);
+  const double2 c0 = ld(tf.fc, 3 * rp), c1 = ld(tf.fc, 3 * rp + 1), c2 = ld(tf.fc, 3 * rp + 2);
+  const double p = sg.p;
+  const double rhojp = (c0.x * p + c0.y) * p + c1.x;
+  const double phip = (c1.y * p + c2.x) * p + c2.y;
   double rhoip = rhojp;
-  if (ti != tj) rhoip = der8(tf.rho, (uint32_t)(ti * tf.n_r + sg.m), sg.p);
-  const double phip = der8(tf.phi, rp, sg.p);
-  if (WANT_E) phi = val8(tf.phi, rp, sg.p);
+  if (ti != tj) rhoip = spl_deriv(tf.rho, (uint32_t)(ti * tf.n_r + sg.m), p);
+  if (WANT_E) phi = spl_value(tf.phi, rp, p);
   return -(dfi * rhojp + dfj * rhoip + phip) * rinv;
 }
 
@@ -142,25 +117,24 @@ __device__ __forceinline__ double fpair(const TabF64 &tf, const TabMixed &tm, in
 // loads of a trip are issued before any is used, pairs are evaluated without
 // per-pair branches (out-of-cutoff and vacant neighbours are masked), and a
 // trip whose pairs are all outside the cutoff for the whole wave is skipped.
-// The stencil rows are padded with zero offsets past nk.
+// The stencil rows are padded with zero offsets past nk.  A neighbour is one
+// 32-byte (x, y, z, type) record: two 16-byte loads.
 
 template <int PREC, bool WANT_E>
 __global__ __launch_bounds__(kBlock) void k_rho_df(Geom g, const int32_t *__restrict__ off, int32_t nk_even,
-                                                     int32_t nk_odd, const double *__restrict__ X,
-                                                     const double *__restrict__ Y, const double *__restrict__ Z,
-                                                     const int8_t *__restrict__ T, double *__restrict__ RHO,
-                                                     double *__restrict__ DF, TabF64 tf, TabMixed tm, double rc2,
-                                                     double *__restrict__ e_part) {
+                                                     int32_t nk_odd, const double4 *__restrict__ P,
+                                                     double *__restrict__ RHO, double *__restrict__ DF, TabF64 tf,
+                                                     TabMixed tm, double rc2, double *__restrict__ e_part) {
   __shared__ double lds[kWavesPerBlock];
   double e_loc = 0.0;
   Site st;
   if (map_site(g, st) && st.active) {
     const uint32_t i = (uint32_t)st.i;
-    const int t0 = ld(T, i);
+    const double4 pi = ld(P, i);
+    const int t0 = (int)pi.w;
     if (t0 >= 0) {
       const int32_t *o = off + st.s * kMaxStencil;
       const int nk = st.s ? nk_odd : nk_even;
-      const double xi = ld(X, i), yi = ld(Y, i), zi = ld(Z, i);
       const double d2_safe = 0.25 * rc2;
       double acc = 0.0;
       for (int k = 0; k < nk; k += kUnroll) {
@@ -168,9 +142,9 @@ __global__ __launch_bounds__(kBlock) void k_rho_df(Geom g, const int32_t *__rest
         double d2[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
-          const uint32_t j = i + (uint32_t)o[k + u];
-          tj[u] = ld(T, j);
-          const double dx = xi - ld(X, j), dy = yi - ld(Y, j), dz = zi - ld(Z, j);
+          const double4 pj = ld(P, i + (uint32_t)o[k + u]);
+          tj[u] = (int)pj.w;
+          const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
           d2[u] = dx * dx + dy * dy + dz * dz;
         }
         bool any = false;
@@ -191,8 +165,8 @@ __global__ __launch_bounds__(kBlock) void k_rho_df(Geom g, const int32_t *__rest
       RHO[i] = acc;
       const Seg se = seg_f64(acc, tf.rho0, tf.inv_drho, tf.n_rho);
       const uint32_t re = (uint32_t)(t0 * tf.n_rho + se.m);
-      DF[i] = der8(tf.emb, re, se.p);
-      if (WANT_E) e_loc = val8(tf.emb, re, se.p);
+      DF[i] = spl_deriv(tf.emb, re, se.p);
+      if (WANT_E) e_loc = spl_value(tf.emb, re, se.p);
     }
   }
   if (WANT_E) {
@@ -210,27 +184,26 @@ __global__ __launch_bounds__(kBlock) void k_df(Geom g, const int8_t *__restrict_
   const int t0 = ld(T, i);
   if (t0 < 0) return;
   const Seg se = seg_f64(ld(RHO, i), tf.rho0, tf.inv_drho, tf.n_rho);
-  DF[i] = der8(tf.emb, (uint32_t)(t0 * tf.n_rho + se.m), se.p);
+  DF[i] = spl_deriv(tf.emb, (uint32_t)(t0 * tf.n_rho + se.m), se.p);
 }
 
 template <int PREC, bool WANT_E>
 __global__ __launch_bounds__(kBlock) void k_force(Geom g, const int32_t *__restrict__ off, int32_t nk_even,
-                                                    int32_t nk_odd, const double *__restrict__ X,
-                                                    const double *__restrict__ Y, const double *__restrict__ Z,
-                                                    const int8_t *__restrict__ T, const double *__restrict__ DF,
-                                                    double *__restrict__ FX, double *__restrict__ FY,
-                                                    double *__restrict__ FZ, TabF64 tf, TabMixed tm, double rc2,
-                                                    double *__restrict__ e_part) {
+                                                    int32_t nk_odd, const double4 *__restrict__ P,
+                                                    const double *__restrict__ DF, double *__restrict__ FX,
+                                                    double *__restrict__ FY, double *__restrict__ FZ, TabF64 tf,
+                                                    TabMixed tm, double rc2, double *__restrict__ e_part) {
   __shared__ double lds[kWavesPerBlock];
   double e_loc = 0.0;
   Site st;
   if (map_site(g, st) && st.active) {
     const uint32_t i = (uint32_t)st.i;
-    const int t0 = ld(T, i);
+    const double4 pi = ld(P, i);
+    const int t0 = (int)pi.w;
     if (t0 >= 0) {
       const int32_t *o = off + st.s * kMaxStencil;
       const int nk = st.s ? nk_odd : nk_even;
-      const double xi = ld(X, i), yi = ld(Y, i), zi = ld(Z, i), dfi = ld(DF, i);
+      const double dfi = ld(DF, i);
       const double d2_safe = 0.25 * rc2;
       double ax = 0.0, ay = 0.0, az = 0.0, ep = 0.0;
       for (int k = 0; k < nk; k += kUnroll) {
@@ -239,10 +212,11 @@ __global__ __launch_bounds__(kBlock) void k_force(Geom g, const int32_t *__restr
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
           const uint32_t j = i + (uint32_t)o[k + u];
-          tj[u] = ld(T, j);
-          dx[u] = xi - ld(X, j);
-          dy[u] = yi - ld(Y, j);
-          dz[u] = zi - ld(Z, j);
+          const double4 pj = ld(P, j);
+          tj[u] = (int)pj.w;
+          dx[u] = pi.x - pj.x;
+          dy[u] = pi.y - pj.y;
+          dz[u] = pi.z - pj.z;
           dfj[u] = ld(DF, j);
           d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
         }
@@ -285,20 +259,20 @@ int64_t site_blocks(const Geom &g) {
   return (nb + kXcds - 1) / kXcds * kXcds;
 }
 
-hipError_t launch_rho_df(hipStream_t st, const Geom &g, const int32_t *off, int32_t nk_even, int32_t nk_odd, const double *X,
-                         const double *Y, const double *Z, const int8_t *T, double *RHO, double *DF,
-                         const TabF64 &tf, const TabMixed &tm, double rc2, int prec, double *e_part) {
+hipError_t launch_rho_df(hipStream_t st, const Geom &g, const int32_t *off, int32_t nk_even, int32_t nk_odd,
+                         const double4 *P, double *RHO, double *DF, const TabF64 &tf, const TabMixed &tm, double rc2,
+                         int prec, double *e_part) {
   const dim3 grid((unsigned)site_blocks(g)), block(kBlock);
   if (prec == 0) {
     if (e_part)
-      k_rho_df<0, true><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, X, Y, Z, T, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<0, true><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
     else
-      k_rho_df<0, false><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, X, Y, Z, T, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<0, false><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
   } else {
     if (e_part)
-      k_rho_df<1, true><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, X, Y, Z, T, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<1, true><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
     else
-      k_rho_df<1, false><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, X, Y, Z, T, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<1, false><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
   }
   return hipGetLastError();
 }
@@ -309,20 +283,20 @@ hipError_t launch_df(hipStream_t st, const Geom &g, const int8_t *T, const doubl
   return hipGetLastError();
 }
 
-hipError_t launch_force(hipStream_t st, const Geom &g, const int32_t *off, int32_t nk_even, int32_t nk_odd, const double *X,
-                        const double *Y, const double *Z, const int8_t *T, const double *DF, double *FX, double *FY,
-                        double *FZ, const TabF64 &tf, const TabMixed &tm, double rc2, int prec, double *e_part) {
+hipError_t launch_force(hipStream_t st, const Geom &g, const int32_t *off, int32_t nk_even, int32_t nk_odd,
+                        const double4 *P, const double *DF, double *FX, double *FY, double *FZ, const TabF64 &tf,
+                        const TabMixed &tm, double rc2, int prec, double *e_part) {
   const dim3 grid((unsigned)site_blocks(g)), block(kBlock);
   if (prec == 0) {
     if (e_part)
-      k_force<0, true><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, X, Y, Z, T, DF, FX, FY, FZ, tf, tm, rc2, e_part);
+      k_force<0, true><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
     else
-      k_force<0, false><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, X, Y, Z, T, DF, FX, FY, FZ, tf, tm, rc2, e_part);
+      k_force<0, false><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
   } else {
     if (e_part)
-      k_force<1, true><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, X, Y, Z, T, DF, FX, FY, FZ, tf, tm, rc2, e_part);
+      k_force<1, true><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
     else
-      k_force<1, false><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, X, Y, Z, T, DF, FX, FY, FZ, tf, tm, rc2, e_part);
+      k_force<1, false><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
   }
   return hipGetLastError();
 }

@@ -46,14 +46,38 @@ __global__ __launch_bounds__(kBlock) void k_dev2ref(Geom g, const T *__restrict_
   ref[ref_of(g, ext_of(g, idx))] = dev[idx];
 }
 
+__global__ __launch_bounds__(kBlock) void k_ref2dev_pos(Geom g, const double *__restrict__ rx,
+                                                          const double *__restrict__ ry, const double *__restrict__ rz,
+                                                          const int8_t *__restrict__ T, double4 *__restrict__ P,
+                                                          bool set_xyz) {
+  const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= (uint32_t)g.n_ext) return;
+  double4 p = P[idx];
+  if (set_xyz) {
+    const int64_t r = ref_of(g, ext_of(g, idx));
+    p.x = rx[r], p.y = ry[r], p.z = rz[r];
+  }
+  p.w = (double)T[idx];
+  P[idx] = p;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dev2ref_pos(Geom g, const double4 *__restrict__ P, double *__restrict__ rx,
+                                                          double *__restrict__ ry, double *__restrict__ rz) {
+  const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= (uint32_t)g.n_ext) return;
+  const int64_t r = ref_of(g, ext_of(g, idx));
+  const double4 p = P[idx];
+  rx[r] = p.x, ry[r] = p.y, rz[r] = p.z;
+}
+
 __global__ __launch_bounds__(kBlock) void k_aos_unpack(Geom g, const char *__restrict__ aos, AosDesc d,
-                                                         uint32_t fields, double *__restrict__ X,
-                                                         double *__restrict__ Y, double *__restrict__ Z,
+                                                         uint32_t fields, double4 *__restrict__ P,
                                                          int8_t *__restrict__ T, double *__restrict__ DF,
                                                          double *__restrict__ RHO) {
   const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
   if (idx >= (uint32_t)g.n_ext) return;
   const char *rec = aos + ref_of(g, ext_of(g, idx)) * d.stride;
+  double4 p = P[idx];
   if (fields & 0x20u) {
     int32_t t;
     if (d.type_bytes == 1)
@@ -62,14 +86,15 @@ __global__ __launch_bounds__(kBlock) void k_aos_unpack(Geom g, const char *__res
       t = *reinterpret_cast<const int16_t *>(rec + d.off_type);
     else
       t = *reinterpret_cast<const int32_t *>(rec + d.off_type);
-    T[idx] = (int8_t)(t < 0 ? -1 : t);
+    const int8_t t8 = (int8_t)(t < 0 ? -1 : t);
+    T[idx] = t8;
+    p.w = (double)t8;
   }
   if (fields & 0x01u) {
     const double *x = reinterpret_cast<const double *>(rec + d.off_x);
-    X[idx] = x[0];
-    Y[idx] = x[1];
-    Z[idx] = x[2];
+    p.x = x[0], p.y = x[1], p.z = x[2];
   }
+  if (fields & 0x21u) P[idx] = p;
   if (fields & 0x10u) DF[idx] = *reinterpret_cast<const double *>(rec + d.off_df);
   if (fields & 0x08u) RHO[idx] = *reinterpret_cast<const double *>(rec + d.off_rho);
 }
@@ -116,10 +141,8 @@ __device__ __forceinline__ int32_t wrap_img(int32_t v, int32_t n, int32_t &img) 
   return v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_ghost(Geom g, double Lx, double Ly, double Lz, double *__restrict__ X,
-                                                    double *__restrict__ Y, double *__restrict__ Z,
-                                                    int8_t *__restrict__ T, double *__restrict__ DF,
-                                                    uint32_t fields) {
+__global__ __launch_bounds__(kBlock) void k_ghost(Geom g, double Lx, double Ly, double Lz, double4 *__restrict__ P,
+                                                    int8_t *__restrict__ T, double *__restrict__ DF, uint32_t fields) {
   const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
   if (idx >= (uint32_t)g.n_ext) return;
   const ExtPos p = ext_of(g, idx);
@@ -130,16 +153,17 @@ __global__ __launch_bounds__(kBlock) void k_ghost(Geom g, double Lx, double Ly, 
   const int64_t src = dev_index(g, p.s, sx + g.gxh, sy + g.gy, sz + g.gz);
   if (fields & 0x01u) {
     T[idx] = T[src];
-    X[idx] = X[src] + (double)ix * Lx;
-    Y[idx] = Y[src] + (double)iy * Ly;
-    Z[idx] = Z[src] + (double)iz * Lz;
+    double4 q = P[src];
+    q.x += (double)ix * Lx;
+    q.y += (double)iy * Ly;
+    q.z += (double)iz * Lz;
+    P[idx] = q;
   }
   if (fields & 0x10u) DF[idx] = DF[src];
 }
 
 template <bool DRIFT, bool WANT_KE>
-__global__ __launch_bounds__(kBlock) void k_kick(Geom g, Masses M, double h, double dt, double *__restrict__ X,
-                                                   double *__restrict__ Y, double *__restrict__ Z,
+__global__ __launch_bounds__(kBlock) void k_kick(Geom g, Masses M, double h, double dt, double4 *__restrict__ P,
                                                    double *__restrict__ VX, double *__restrict__ VY,
                                                    double *__restrict__ VZ, const double *__restrict__ FX,
                                                    const double *__restrict__ FY, const double *__restrict__ FZ,
@@ -157,9 +181,11 @@ __global__ __launch_bounds__(kBlock) void k_kick(Geom g, Masses M, double h, dou
       VY[i] = vy;
       VZ[i] = vz;
       if (DRIFT) {
-        X[i] += dt * vx;
-        Y[i] += dt * vy;
-        Z[i] += dt * vz;
+        double4 q = P[i];
+        q.x += dt * vx;
+        q.y += dt * vy;
+        q.z += dt * vz;
+        P[i] = q;
       }
       if (WANT_KE) ke = 0.5 * M.m[t] * (vx * vx + vy * vy + vz * vz) * kMvv2e;
     }
@@ -205,10 +231,24 @@ __global__ __launch_bounds__(kBlock) void k_fill(T *__restrict__ p, int64_t n, T
   if (i < n) p[i] = v;
 }
 
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                          int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) dst[i] = src[i];
+}
+
 inline dim3 grid_ext(const Geom &g) { return dim3((unsigned)((g.n_ext + kBlock - 1) / kBlock)); }
 
 }  // namespace
 
+hipError_t launch_ref2dev_pos(hipStream_t st, const Geom &g, const double *rx, const double *ry, const double *rz,
+                              const int8_t *T, double4 *P, bool set_xyz) {
+  k_ref2dev_pos<<<grid_ext(g), kBlock, 0, st>>>(g, rx, ry, rz, T, P, set_xyz);
+  return hipGetLastError();
+}
+hipError_t launch_dev2ref_pos(hipStream_t st, const Geom &g, const double4 *P, double *rx, double *ry, double *rz) {
+  k_dev2ref_pos<<<grid_ext(g), kBlock, 0, st>>>(g, P, rx, ry, rz);
+  return hipGetLastError();
+}
 hipError_t launch_ref2dev_f64(hipStream_t st, const Geom &g, const double *ref, double *dev) {
   k_ref2dev<double><<<grid_ext(g), kBlock, 0, st>>>(g, ref, dev);
   return hipGetLastError();
@@ -226,9 +266,9 @@ hipError_t launch_dev2ref_i8(hipStream_t st, const Geom &g, const int8_t *dev, i
   return hipGetLastError();
 }
 
-hipError_t launch_aos_unpack(hipStream_t st, const Geom &g, const char *aos, AosDesc d, uint32_t fields, double *X,
-                             double *Y, double *Z, int8_t *T, double *DF, double *RHO) {
-  k_aos_unpack<<<grid_ext(g), kBlock, 0, st>>>(g, aos, d, fields, X, Y, Z, T, DF, RHO);
+hipError_t launch_aos_unpack(hipStream_t st, const Geom &g, const char *aos, AosDesc d, uint32_t fields, double4 *P,
+                             int8_t *T, double *DF, double *RHO) {
+  k_aos_unpack<<<grid_ext(g), kBlock, 0, st>>>(g, aos, d, fields, P, T, DF, RHO);
   return hipGetLastError();
 }
 
@@ -239,26 +279,26 @@ hipError_t launch_aos_pack(hipStream_t st, const Geom &g, char *aos, AosDesc d, 
   return hipGetLastError();
 }
 
-hipError_t launch_ghost(hipStream_t st, const Geom &g, double Lx, double Ly, double Lz, double *X, double *Y,
-                        double *Z, int8_t *T, double *DF, uint32_t fields) {
-  k_ghost<<<grid_ext(g), kBlock, 0, st>>>(g, Lx, Ly, Lz, X, Y, Z, T, DF, fields);
+hipError_t launch_ghost(hipStream_t st, const Geom &g, double Lx, double Ly, double Lz, double4 *P, int8_t *T,
+                        double *DF, uint32_t fields) {
+  k_ghost<<<grid_ext(g), kBlock, 0, st>>>(g, Lx, Ly, Lz, P, T, DF, fields);
   return hipGetLastError();
 }
 
-hipError_t launch_kick(hipStream_t st, const Geom &g, const Masses &M, double h, double dt, bool drift, double *X,
-                       double *Y, double *Z, double *VX, double *VY, double *VZ, const double *FX, const double *FY,
-                       const double *FZ, const int8_t *T, double *ke_part) {
+hipError_t launch_kick(hipStream_t st, const Geom &g, const Masses &M, double h, double dt, bool drift, double4 *P,
+                       double *VX, double *VY, double *VZ, const double *FX, const double *FY, const double *FZ,
+                       const int8_t *T, double *ke_part) {
   const dim3 grid((unsigned)site_blocks(g));
   if (drift) {
     if (ke_part)
-      k_kick<true, true><<<grid, kBlock, 0, st>>>(g, M, h, dt, X, Y, Z, VX, VY, VZ, FX, FY, FZ, T, ke_part);
+      k_kick<true, true><<<grid, kBlock, 0, st>>>(g, M, h, dt, P, VX, VY, VZ, FX, FY, FZ, T, ke_part);
     else
-      k_kick<true, false><<<grid, kBlock, 0, st>>>(g, M, h, dt, X, Y, Z, VX, VY, VZ, FX, FY, FZ, T, ke_part);
+      k_kick<true, false><<<grid, kBlock, 0, st>>>(g, M, h, dt, P, VX, VY, VZ, FX, FY, FZ, T, ke_part);
   } else {
     if (ke_part)
-      k_kick<false, true><<<grid, kBlock, 0, st>>>(g, M, h, dt, X, Y, Z, VX, VY, VZ, FX, FY, FZ, T, ke_part);
+      k_kick<false, true><<<grid, kBlock, 0, st>>>(g, M, h, dt, P, VX, VY, VZ, FX, FY, FZ, T, ke_part);
     else
-      k_kick<false, false><<<grid, kBlock, 0, st>>>(g, M, h, dt, X, Y, Z, VX, VY, VZ, FX, FY, FZ, T, ke_part);
+      k_kick<false, false><<<grid, kBlock, 0, st>>>(g, M, h, dt, P, VX, VY, VZ, FX, FY, FZ, T, ke_part);
   }
   return hipGetLastError();
 }
@@ -267,6 +307,12 @@ hipError_t launch_energy_finalize(hipStream_t st, const double *ep_a, int64_t na
                                   const double *ep_c, int64_t nc, const double *ke, int64_t nk, const double *ke_b,
                                   int64_t nkb, double *out, int32_t *counter) {
   k_energy_finalize<<<1, kBlock, 0, st>>>(ep_a, na, ep_b, nb, ep_c, nc, ke, nk, ke_b, nkb, out, counter);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_copy(hipStream_t st, const void *src, void *dst, int64_t bytes) {
+  const int64_t n = bytes / 16;
+  k_stream_copy<<<dim3(8192), kBlock, 0, st>>>(static_cast<const double2 *>(src), static_cast<double2 *>(dst), n);
   return hipGetLastError();
 }
 

@@ -125,6 +125,7 @@ def lib():
         "mmd_timing_enable": (C.c_int, [P, C.c_int32]),
         "mmd_timing_get": (C.c_int, [P, _d, _i64]),
         "mmd_timing_reset": (C.c_int, [P]),
+        "mmd_stream_copy": (C.c_int, [P, C.c_size_t, C.c_int32, _d]),
         "mmd_stencil_build": (C.c_int32, [C.c_double, C.c_double, C.c_int32, _i32, C.c_int32]),
         "mmd_stencil_linear": (C.c_int, [C.POINTER(Domain), _i32, C.c_int32, _i64]),
         "mmd_synth_tables": (C.c_int, [C.POINTER(SynthParams), _d, _d, _d, _d, _d, _d, _d]),
@@ -399,6 +400,11 @@ class Context:
         n = np.zeros(K_COUNT, np.int64)
         _check(lib().mmd_timing_get(self.h, ptr(ms), ptr(n, C.c_int64)))
         return ms, n
+
+    def stream_copy(self, nbytes, repeats=10):
+        ms = C.c_double()
+        _check(lib().mmd_stream_copy(self.h, nbytes, repeats, C.byref(ms)))
+        return ms.value
 
     def timing_reset(self):
         _check(lib().mmd_timing_reset(self.h))
