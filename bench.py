@@ -1,16 +1,22 @@
 #!/usr/bin/env python3
 """MISA-MD EAM step throughput on one MI355X.
 
-Workload (default): 128x128x128 BCC cells (4,194,304 atoms), a = 2.855 A,
-3 atom types drawn 90/5/5 from the seed with 9 synthetic pair tables (A, B, C
-jittered +-10%, phi symmetric), 5000-point splines on r in [0.5, 5.6] A,
-Gaussian displacement 0.05 A, Maxwell velocities at 800 K, velocity Verlet
-with dt = 1 fs, fully device-resident (ghost refresh, rho + df, ghost df,
-force, integration all on the GPU).  One step = one full velocity-Verlet step
-over every atom; the metric is atom-steps/s of the FP64 path.  The same run
-times the mixed-precision path on the same box and reports its force/energy
-error against FP64, and times the CPU oracle (a restatement of the
-reference's EAM algorithm) on the host cores.
+Headline workload (the FP64 metric): 128x128x128 BCC cells (4,194,304 atoms),
+a = 2.855 A, 3 atom types drawn 90/5/5 from the seed with 9 synthetic pair
+tables (A, B, C jittered +-10%, phi symmetric), 5000-point splines on
+r in [0.5, 5.6] A, Gaussian displacement 0.05 A, Maxwell velocities at 800 K,
+velocity Verlet with dt = 1 fs.  Everything is device resident: ghost
+refresh, rho + df, ghost df, force and integration run on the GPU, energies
+are recorded every 10 steps.  One step = one full velocity-Verlet step over
+every atom; `value` is atom-steps/s of the FP64 path.
+
+The same run also measures
+  * mixed precision against FP64 on the defect box (same 128^3 3-type alloy
+    plus 0.5% vacancies and 0.2% interstitials at tetrahedral sites, 1200 K):
+    atom-steps/s of both, force error of mixed vs FP64 on identical states
+    (start and end of the run) and the per-step energy difference per atom;
+  * the CPU oracle (FP64 C restatement of the reference's EAM algorithm,
+    OpenMP on the host cores) on the headline box.
 
 Inputs are resident in HBM before the timed region starts.  Per-kernel
 durations come from HIP events recorded on the context's stream around every
@@ -34,26 +40,35 @@ import mmd  # noqa: E402
 A, RC, SKIN, DT = 2.855, 5.6, 0.3, 1.0
 PEAK_HBM_GBPS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
+ENERGY_EVERY = 10
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_workload(cells, seed=2024):
-    spec = mmd.lattice_spec((cells, cells, cells), ghost_cells=3, a=A, n_types=3, sigma=0.05, temperature=800.0,
-                            seed=seed)
-    dom, box, st, _ = mmd.lattice_init(spec)
+def make_workload(cells, seed=2024, temperature=800.0, vacancy_frac=0.0, interstitial_frac=0.0):
+    spec = mmd.lattice_spec((cells, cells, cells), ghost_cells=3, a=A, n_types=3, sigma=0.05,
+                            temperature=temperature, vacancy_frac=vacancy_frac,
+                            interstitial_frac=interstitial_frac, seed=seed)
+    cap = int(interstitial_frac * 2 * cells ** 3 * 1.5) + 16
+    dom, box, st, itl = mmd.lattice_init(spec, itl_cap=cap)
     tab = mmd.synth_tables(n_types=3, rc=RC, jitter=0.1, seed=seed)
     st3 = mmd.stencil_build(A, RC + SKIN, 0)
-    return dict(dom=dom, box=box, st=st, tab=tab, st3=st3, even=mmd.stencil_linear(dom, st3),
-                odd=mmd.stencil_linear(dom, -st3))
+    return dict(dom=dom, box=box, st=st, itl=itl, tab=tab, st3=st3, even=mmd.stencil_linear(dom, st3),
+                odd=mmd.stencil_linear(dom, -st3), wide3=mmd.wide_stencil(A, RC, SKIN))
+
+
+def n_atoms_of(w):
+    li = w["st"].local_index()
+    return int((w["st"].type[li] >= 0).sum()) + w["itl"].n
 
 
 def pairs_in_cutoff(w, sample=20000, seed=0):
-    """Mean number of in-cutoff neighbours per atom (N_c) over a sample."""
+    """Mean number of in-cutoff lattice neighbours per atom (N_c) over a sample."""
     st, dom = w["st"], w["dom"]
     li = st.local_index()
+    li = li[st.type[li] >= 0]
     rng = np.random.default_rng(seed)
     idx = rng.choice(li, size=min(sample, len(li)), replace=False)
     X = (idx % dom.ext[0]) - dom.ghost[0]
@@ -67,17 +82,36 @@ def pairs_in_cutoff(w, sample=20000, seed=0):
     return tot / len(idx)
 
 
-def run_gpu(w, prec, steps, warmup, energy_every):
+def open_ctx(w, prec, energy_every=ENERGY_EVERY, state=None, itl=None):
     ctx = mmd.Context(0)
     ctx.set_domain(w["dom"])
     ctx.set_stencil(w["even"], w["odd"])
     ctx.set_tables(w["tab"], mmd.DEFAULT_MASSES)
     ctx.set_precision(prec)
-    ctx.upload(w["st"], mmd.F_X | mmd.F_V | mmd.F_TYPE)
+    ctx.upload(state or w["st"], mmd.F_X | mmd.F_V | mmd.F_TYPE)
     ctx.md_setup(w["box"], A, DT, RC, energy_every=energy_every)
+    it = itl if itl is not None else w["itl"]
+    if it.n:
+        ctx.itl_set(it, w["wide3"], cap=max(it.n, 1))
     ctx.md_prime()
-    f0 = w["st"].copy()
-    ctx.download(f0, mmd.F_F)
+    return ctx
+
+
+def forces(ctx, w):
+    f = w["st"].copy()
+    ctx.download(f, mmd.F_F)
+    li = f.local_index()
+    li = li[w["st"].type[li] >= 0]
+    F = np.stack([f.fx[li], f.fy[li], f.fz[li]], 1)
+    if w["itl"].n:
+        gi = ctx.itl_get(w["itl"].n)
+        F = np.concatenate([F, np.stack([gi.fx, gi.fy, gi.fz], 1)])
+    return F
+
+
+def run_gpu(w, prec, steps, warmup, keep_state=False):
+    ctx = open_ctx(w, prec)
+    f0 = forces(ctx, w)
     e0 = ctx.energy()
     if warmup:
         ctx.md_run(warmup)
@@ -91,8 +125,21 @@ def run_gpu(w, prec, steps, warmup, energy_every):
     t1 = time.perf_counter()
     ms, n = ctx.timing_get()
     ctx.timing(False)
+    out = dict(seconds=t1 - t0, kernel_ms=ms, launches=n, energies=en, f0=f0, e0=e0)
+    if keep_state:
+        st = w["st"].copy()
+        ctx.download(st, mmd.F_X | mmd.F_V | mmd.F_TYPE)
+        out["state"] = st
+        out["itl"] = ctx.itl_get(w["itl"].n) if w["itl"].n else w["itl"]
+        out["f_end"] = forces(ctx, w)
     ctx.close()
-    return dict(seconds=t1 - t0, kernel_ms=ms, launches=n, energies=en, f0=f0, e0=e0)
+    return out
+
+
+def force_err(F, Fr):
+    d = np.linalg.norm(F - Fr, axis=1)
+    rms = np.sqrt((np.linalg.norm(Fr, axis=1) ** 2).mean())
+    return float(np.linalg.norm(F - Fr) / np.linalg.norm(Fr)), float(d.max() / rms)
 
 
 def cpu_baseline(w, steps):
@@ -103,23 +150,52 @@ def cpu_baseline(w, steps):
     O.rho_df(w["dom"], w["even"], w["odd"], T, RC, st, nthreads=nthreads)
     O.ghost_df(w["dom"], st)
     O.force(w["dom"], w["even"], w["odd"], T, RC, st, nthreads=nthreads)
-    n_atoms = int((st.type[st.local_index()] >= 0).sum())
+    n_atoms = n_atoms_of(w)
     t0 = time.perf_counter()
     O.md_run(w["dom"], w["even"], w["odd"], T, RC, w["box"], DT, steps, st, nthreads=nthreads, energies=False)
     dt = time.perf_counter() - t0
-    return dict(value=n_atoms * steps / dt, unit="atom-steps/s", cores=nthreads, kind="port",
-                sample=f"CPU oracle (FP64 restatement of the reference EAM step, OpenMP) on the same "
-                       f"{n_atoms}-atom box for {steps} velocity-Verlet steps, {dt:.1f} s")
+    return dict(value=round(n_atoms * steps / dt, 1), unit="atom-steps/s", cores=nthreads, kind="port",
+                sample=f"CPU oracle (FP64 C restatement of the reference EAM step, OpenMP, {nthreads} threads) on "
+                       f"the same {n_atoms}-atom box for {steps} velocity-Verlet steps ({dt:.2f} s wall)")
 
 
 def load_traffic():
-    """HBM bytes per force launch measured by rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by the copy-kernel calibration), if profiled."""
+    """HBM bytes per force launch from rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration (profiles/)."""
     p = os.path.join(ROOT, "profiles", "force_fp64_128cube_traffic.json")
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f).get("bytes_per_launch")
     return None
+
+
+def mixed_vs_fp64(w, steps, warmup):
+    n_atoms = n_atoms_of(w)
+    r64 = run_gpu(w, mmd.PREC_FP64, steps, warmup, keep_state=True)
+    rmx = run_gpu(w, mmd.PREC_MIXED, steps, warmup, keep_state=True)
+    v64 = n_atoms * steps / r64["seconds"]
+    vmx = n_atoms * steps / rmx["seconds"]
+    nw0, pa0 = force_err(rmx["f0"], r64["f0"])
+    # FP64 forces on the mixed run's final state
+    ctx = open_ctx(w, mmd.PREC_FP64, state=rmx["state"], itl=rmx["itl"])
+    f_ref_end = forces(ctx, w)
+    ctx.close()
+    nw1, pa1 = force_err(rmx["f_end"], f_ref_end)
+    e64, emx = r64["energies"].sum(1), rmx["energies"].sum(1)
+    rec = np.arange(ENERGY_EVERY - 1, steps, ENERGY_EVERY)
+    d64 = e64[rec] - (r64["e0"][0] + r64["e0"][1])
+    dmx = emx[rec] - (rmx["e0"][0] + rmx["e0"][1])
+    per_step = np.abs(dmx - d64) / n_atoms / (rec + 1 + warmup)
+    mk, mn = rmx["kernel_ms"], rmx["launches"]
+    return {"workload": f"BCC {w['dom'].box[1]}^3 cells, 3-type alloy, 0.5% vacancies, "
+                        f"{w['itl'].n} interstitials (0.2%), 1200 K, dt 1 fs",
+            "atoms": n_atoms, "steps": steps, "fp64_value": round(v64, 1), "value": round(vmx, 1),
+            "speedup_vs_fp64": round(vmx / v64, 3),
+            "force_rel_err_normwise": max(nw0, nw1), "force_err_max_over_rms": max(pa0, pa1),
+            "energy_diff_per_atom_per_step_max": float(per_step.max()) if len(per_step) else None,
+            "force_kernel_ms": round(mk[mmd.K_FORCE] / max(mn[mmd.K_FORCE], 1), 4),
+            "rho_df_kernel_ms": round(mk[mmd.K_RHO] / max(mn[mmd.K_RHO], 1), 4),
+            "interstitial_ms_per_step": round(mk[mmd.K_ITL] / steps, 4)}
 
 
 def main():
@@ -142,33 +218,33 @@ def main():
     if args.copy_probe:
         with mmd.Context(0) as ctx:
             ms = ctx.stream_copy(args.copy_probe, repeats=10)
-        print(json.dumps({"copy_bytes": args.copy_probe, "ms": ms, "GBps_read_plus_write": 2 * args.copy_probe / ms / 1e6}))
+        print(json.dumps({"copy_bytes": args.copy_probe, "ms": ms,
+                          "GBps_read_plus_write": 2 * args.copy_probe / ms / 1e6}))
         return
 
     log(f"bench: building {args.cells}^3-cell 3-type BCC workload")
     w = make_workload(args.cells)
-    li = w["st"].local_index()
-    n_atoms = int((w["st"].type[li] >= 0).sum())
+    n_atoms = n_atoms_of(w)
     S = len(w["st3"])
     Nc = pairs_in_cutoff(w)
 
     log(f"bench: FP64 {args.steps} steps (+{args.warmup} warmup), {n_atoms} atoms")
-    r64 = run_gpu(w, mmd.PREC_FP64, args.steps, args.warmup, energy_every=10)
+    r64 = run_gpu(w, mmd.PREC_FP64, args.steps, args.warmup)
     value = n_atoms * args.steps / r64["seconds"]
     ms, n = r64["kernel_ms"], r64["launches"]
     t_force = ms[mmd.K_FORCE] / max(n[mmd.K_FORCE], 1) * 1e-3
     t_rho = ms[mmd.K_RHO] / max(n[mmd.K_RHO], 1) * 1e-3
-    flops_force = (8 * S + 40 * Nc) * n_atoms
+    flops_atom = 8 * S + 40 * Nc
+    flops_force = flops_atom * n_atoms
     flops_rho = (8 * S + 10 * Nc + 10) * n_atoms
     bytes_force = 57 * n_atoms
-    traffic = load_traffic()
     achieved = flops_force / t_force / 1e12
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": load_traffic(),
                 "kernel": "k_force<FP64>", "kernel_ms": round(t_force * 1e3, 4),
+                "flops_per_atom": round(flops_atom, 1), "alg_bytes_per_atom": 57,
                 "hbm_achieved_GBps": round(bytes_force / t_force / 1e9, 1),
                 "hbm_frac": round(bytes_force / t_force / 1e9 / PEAK_HBM_GBPS, 4),
-                "flops_per_atom": round(8 * S + 40 * Nc, 1), "alg_bytes_per_atom": 57,
                 "rho_df_kernel_ms": round(t_rho * 1e3, 4),
                 "rho_df_tflops": round(flops_rho / t_rho / 1e12, 3)}
     out = {"metric": "atom-steps/s (FP64 EAM velocity-Verlet step)", "value": round(value, 1),
@@ -177,32 +253,15 @@ def main():
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": f"BCC {args.cells}^3 cells, 3-type alloy 90/5/5, 800 K, dt 1 fs, FP64",
                       "atoms": n_atoms, "stencil_entries": S, "pairs_in_cutoff": round(Nc, 2),
-                      "energy_every": 10},
+                      "energy_every": ENERGY_EVERY},
            "roofline": roofline,
            "kernel_ms_per_step": {k: round(ms[i] / max(args.steps, 1), 4) for i, k in
                                   enumerate(("rho_df", "force", "integrate", "ghost", "interstitial"))}}
 
     if not args.no_mixed:
-        log("bench: mixed precision on the same box")
-        rmx = run_gpu(w, mmd.PREC_MIXED, args.steps, args.warmup, energy_every=10)
-        vmx = n_atoms * args.steps / rmx["seconds"]
-        F64 = np.stack([r64["f0"].fx[li], r64["f0"].fy[li], r64["f0"].fz[li]], 1)
-        FMX = np.stack([rmx["f0"].fx[li], rmx["f0"].fy[li], rmx["f0"].fz[li]], 1)
-        d = np.linalg.norm(FMX - F64, axis=1)
-        rms = np.sqrt((np.linalg.norm(F64, axis=1) ** 2).mean())
-        e64 = r64["energies"][9::10].sum(1)
-        emx = rmx["energies"][9::10].sum(1)
-        drift = np.abs((emx - e64) - (rmx["e0"][0] + rmx["e0"][1] - r64["e0"][0] - r64["e0"][1]))
-        steps_at = np.arange(10, args.steps + 1, 10)
-        mk = rmx["kernel_ms"]
-        mn = rmx["launches"]
-        out["mixed"] = {"value": round(vmx, 1), "speedup_vs_fp64": round(vmx / value, 3),
-                        "force_rel_err_normwise": float(np.linalg.norm(FMX - F64) / np.linalg.norm(F64)),
-                        "force_err_max_over_rms": float(d.max() / rms),
-                        "energy_diff_per_atom_per_step": float((drift / steps_at).max() / n_atoms)
-                        if len(steps_at) else None,
-                        "force_kernel_ms": round(mk[mmd.K_FORCE] / max(mn[mmd.K_FORCE], 1), 4),
-                        "rho_df_kernel_ms": round(mk[mmd.K_RHO] / max(mn[mmd.K_RHO], 1), 4)}
+        log("bench: mixed vs FP64 on the defect box (0.5% vacancies, 0.2% interstitials, 1200 K)")
+        wd = make_workload(args.cells, seed=2025, temperature=1200.0, vacancy_frac=0.005, interstitial_frac=0.002)
+        out["mixed"] = mixed_vs_fp64(wd, args.steps, args.warmup)
 
     if not args.no_cpu:
         log(f"bench: CPU oracle baseline, {args.cpu_steps} steps")

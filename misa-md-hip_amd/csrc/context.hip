@@ -401,13 +401,23 @@ static SplHost pack_f64(Arena &A, const double *knots, int nfun, int n, double d
 
 struct MixHost {
   size_t vb, vc, db, dc;
+  double s_lo, s_hi;  // range where the table meets the accuracy bound
 };
+
+// Accuracy bounds of the r^2-keyed tables, relative error against the FP64
+// r-spline they resample, per pair term: values 1e-9; the pair-force term
+// phi' 2e-8 (the force on an atom cancels ~50x over its ~60 neighbours, so
+// the per-atom force error stays well inside 1e-6); rho' enters the force
+// multiplied by F'(rho), so its bound is 2e-8 / max|F'|.
+constexpr double kMixedTolValue = 1e-9;
+constexpr double kMixedTolForce = 2e-8;
 
 // Mixed-precision r^2-keyed family resampled from the FP64 r-keyed spline:
 // knot values and exact derivatives of g(sqrt(s)) at uniform s, cubic Hermite
 // in s; FP64 base and linear terms, FP32 quadratic and cubic terms.
 static MixHost pack_mixed(Arena &A, const double *knots, int nfun, int n_r, double r0, double dr, int n_s, double s0,
-                          double ds, std::vector<double> *db_out = nullptr, std::vector<float> *dc_out = nullptr) {
+                          double ds, double tol_deriv, std::vector<double> *db_out = nullptr,
+                          std::vector<float> *dc_out = nullptr) {
   std::vector<double> c7((size_t)n_r * 7), h(n_s), hp(n_s), vb((size_t)nfun * n_s * 2), db(vb.size());
   std::vector<float> vc((size_t)nfun * n_s * 2), dc((size_t)nfun * n_s);
   const double inv_dr = 1.0 / dr;
@@ -435,9 +445,43 @@ static MixHost pack_mixed(Arena &A, const double *knots, int nfun, int n_r, doub
       dc[r] = (float)(3.0 * a3 / ds);
     }
   }
+  // Dense check of every segment against the FP64 spline (the same FP32
+  // rounding as the device); the table is used only on the contiguous range of
+  // segments around the middle that meet the bounds, the rest falls back to
+  // the FP64 r-spline.  Near r -> 0 the uniform s grid is coarse in r and the
+  // end segments of the r-spline carry low-order kinks.
+  std::vector<char> ok((size_t)n_s, 1);
+  for (int f = 0; f < nfun; f++) {
+    spline7(knots + (size_t)f * n_r, n_r, dr, c7.data());
+    double hmax = 0.0, gmax = 0.0;
+    for (int i = 0; i < n_s; i++) {
+      const double r = std::sqrt(s0 + ds * (double)i);
+      hmax = std::max(hmax, std::fabs(spline7_value(c7.data(), n_r, r0, inv_dr, r)));
+      gmax = std::max(gmax, std::fabs(spline7_deriv(c7.data(), n_r, r0, inv_dr, r) / (2.0 * r)));
+    }
+    for (int m = 0; m < n_s - 1; m++) {
+      const size_t rec = (size_t)f * n_s + m;
+      for (int k = 1; k < 8; k++) {
+        const double q = (double)k / 8.0, sv = s0 + ds * ((double)m + q), r = std::sqrt(sv);
+        const double h_ex = spline7_value(c7.data(), n_r, r0, inv_dr, r);
+        const double g_ex = spline7_deriv(c7.data(), n_r, r0, inv_dr, r) / (2.0 * r);
+        const float c = std::fma((float)q, vc[2 * rec + 1], vc[2 * rec]);
+        const double h = std::fma(q, std::fma(q, (double)c, vb[2 * rec + 1]), vb[2 * rec]);
+        const double g = std::fma(q, std::fma(q, (double)dc[rec], db[2 * rec + 1]), db[2 * rec]);
+        if (std::fabs(h - h_ex) > kMixedTolValue * (std::fabs(h_ex) + 1e-3 * hmax) ||
+            std::fabs(g - g_ex) > tol_deriv * (std::fabs(g_ex) + 1e-3 * gmax))
+          ok[(size_t)m] = 0;
+      }
+    }
+  }
+  int lo = n_s / 2, hi = n_s / 2;
+  while (lo > 0 && ok[(size_t)lo - 1]) lo--;
+  while (hi < n_s - 1 && ok[(size_t)hi]) hi++;
   if (db_out) *db_out = db;
   if (dc_out) *dc_out = dc;
   MixHost o;
+  o.s_lo = s0 + ds * (double)lo;
+  o.s_hi = s0 + ds * (double)hi;
   o.vb = A.put(vb.data(), vb.size() * 8);
   o.vc = A.put(vc.data(), vc.size() * 4);
   o.db = A.put(db.data(), db.size() * 8);
@@ -490,8 +534,17 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
     const double s0 = t->r0 * t->r0, ds = (rmax * rmax - s0) / (double)(n_s - 1);
     std::vector<double> rdb, pdb;
     std::vector<float> rdc, pdc;
-    const MixHost m_rho = pack_mixed(A, t->rho_r, nt, nr, t->r0, t->dr, n_s, s0, ds, &rdb, &rdc);
-    const MixHost m_phi = pack_mixed(A, t->phi_r, nt * nt, nr, t->r0, t->dr, n_s, s0, ds, &pdb, &pdc);
+    // largest |F'| over the central 80% of the embedding tables
+    double fpmax = 0.02;
+    {
+      std::vector<double> ce((size_t)nrho * 7);
+      for (int a = 0; a < nt; a++) {
+        spline7(t->emb + (size_t)a * nrho, nrho, t->drho, ce.data());
+        for (int m = nrho / 10; m < nrho - nrho / 10; m++) fpmax = std::max(fpmax, std::fabs(ce[(size_t)m * 7 + 2]));
+      }
+    }
+    const MixHost m_rho = pack_mixed(A, t->rho_r, nt, nr, t->r0, t->dr, n_s, s0, ds, kMixedTolForce / fpmax, &rdb, &rdc);
+    const MixHost m_phi = pack_mixed(A, t->phi_r, nt * nt, nr, t->r0, t->dr, n_s, s0, ds, kMixedTolForce, &pdb, &pdc);
     const size_t h_fc = pack_force_f64(A, t->rho_r, t->phi_r, nt, nr, t->dr);
     // mixed force records per type pair (a, b): (rho_b dbase, b1), (phi_ab dbase, b1), (rho_b b2, phi_ab b2)
     std::vector<double> fb((size_t)nt * nt * n_s * 4);
@@ -539,8 +592,8 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
     tm.fb = reinterpret_cast<const double2 *>(dev + h_fb);
     tm.fq = reinterpret_cast<const float2 *>(dev + h_fq);
     tm.nt = nt, tm.n_s = n_s, tm.s0 = s0, tm.inv_ds = 1.0 / ds;
-    const double r_head = t->r0 + 3.0 * t->dr, r_tail = t->r0 + (double)(nr - 4) * t->dr;
-    tm.s_head = r_head * r_head, tm.s_tail = r_tail * r_tail;
+    tm.s_head = std::max(m_rho.s_lo, m_phi.s_lo);
+    tm.s_tail = std::min(m_rho.s_hi, m_phi.s_hi);
     for (int a = 0; a < kMaxTypes; a++) {
       c->masses.m[a] = a < nt ? t->mass[a] : 1.0;
       c->masses.inv_m[a] = 1.0 / c->masses.m[a];

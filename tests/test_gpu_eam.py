@@ -298,3 +298,40 @@ def test_error_paths():
         ctx.set_tables(t3, c["masses"])
     assert e.value.code == mmd.E_INVALID
     ctx.close()
+
+
+def test_mixed_precision_close_contacts():
+    """Colliding lattice atoms (pairs at 1.0-1.6 A, as in hot defect runs) stay
+    within the mixed-precision target: the r^2 tables hand such pairs to the
+    FP64 spline."""
+    c = W.case(cells=(8, 8, 8), n_types=3, sigma=0.05, temperature=1200.0, vacancy_frac=0.01, seed=31)
+    st = c["st"]
+    li = st.local_index()
+    occ = li[st.type[li] >= 0]
+    rng = np.random.default_rng(5)
+    for k, i in enumerate(rng.choice(occ, 12, replace=False)):
+        j = i + c["even"][0] if ((i % c["dom"].ext[0]) % 2 == 0) else i + c["odd"][0]
+        if st.type[j] < 0:
+            continue
+        d = np.array([st.x[j] - st.x[i], st.y[j] - st.y[i], st.z[j] - st.z[i]])
+        target = 1.0 + 0.05 * k
+        shift = d * (1.0 - target / np.linalg.norm(d))
+        st.x[i] += shift[0]
+        st.y[i] += shift[1]
+        st.z[i] += shift[2]
+    res = {}
+    for prec in (mmd.PREC_FP64, mmd.PREC_MIXED):
+        with _ctx(c, prec=prec) as ctx:
+            ctx.md_setup(c["box"], A, 1.0, RC)
+            ctx.ghost_refresh(mmd.F_X)
+            ctx.rho_df(RC)
+            ctx.ghost_refresh(mmd.F_DF)
+            ctx.force(RC)
+            o = st.copy()
+            ctx.download(o, mmd.F_F)
+            res[prec] = o
+    a, b = res[mmd.PREC_FP64], res[mmd.PREC_MIXED]
+    Fa = np.stack([a.fx[occ], a.fy[occ], a.fz[occ]], 1)
+    Fb = np.stack([b.fx[occ], b.fy[occ], b.fz[occ]], 1)
+    normwise, per_atom = _force_err(Fb, Fa)
+    assert normwise <= 1e-6 and per_atom <= 1e-6, (normwise, per_atom)
