@@ -195,6 +195,91 @@ int mmd_itl_set(mmd_ctx *ctx, int32_t n, const double *x, const double *y, const
 int mmd_itl_get(mmd_ctx *ctx, int32_t *n, double *x, double *y, double *z, double *vx, double *vy, double *vz,
                 double *fx, double *fy, double *fz, double *rho, double *df, int8_t *type);
 
+/* ----------------------------------------------------------- serving loop */
+
+/* Continuous batching of independent MD replicas on one GPU (no counterpart
+ * in the reference plugin, which runs one sub-domain per call).  Replicas
+ * arrive over time; each is admitted when a slot and device memory (an
+ * arena carved from free HBM at creation) are available, its lattice stays
+ * resident, and all active replicas advance together through persistent
+ * batched kernels replayed from a captured HIP graph.  A replica's results
+ * are identical whether it runs alone or in a mix. */
+typedef struct mmd_server mmd_server;
+
+typedef struct {
+  int32_t device;
+  int32_t precision;     /* MMD_PREC_FP64 or MMD_PREC_MIXED                       */
+  double lattice_const;  /* A                                                     */
+  double cutoff, skin;   /* A; stencil radius = cutoff + skin                      */
+  double dt;             /* fs                                                    */
+  int32_t ghost_cells;
+  double type_cum[8];    /* cumulative type fractions of the random alloy         */
+  double sigma;          /* initial Gaussian displacement, A                      */
+  int32_t max_replicas;  /* concurrent replicas                                   */
+  double hbm_fraction;   /* fraction of free HBM taken for the replica arena       */
+  size_t hbm_limit;      /* 0 = none; else upper bound on the arena in bytes       */
+  int32_t chunk_steps;   /* steps per scheduling round                            */
+  int32_t workers;       /* host threads preparing initial conditions             */
+  int32_t use_graph;     /* replay the batched step from a captured HIP graph     */
+} mmd_server_opts;
+
+typedef struct {
+  uint64_t id;
+  int32_t cells;           /* cubic box edge in BCC cells                          */
+  double temperature;      /* K                                                    */
+  int32_t steps;
+  uint64_t seed;
+  double arrival_s;        /* arrival time after the start of mmd_server_run       */
+  const mmd_host_soa *keep; /* optional: receives the final ext state (reference layout) */
+} mmd_replica_spec;
+
+typedef struct {
+  uint64_t id;
+  int32_t status;          /* MMD_OK, or why the replica failed or was rejected    */
+  int32_t n_atoms;
+  double t_arrival;        /* seconds after the start of the run                   */
+  double t_admit;          /* slot and memory granted, preparation starts          */
+  double t_ready;          /* state resident, forces primed, joins the batch       */
+  double t_done;
+  double epot0, ekin0;     /* energies at admission (eV)                           */
+  double epot, ekin;       /* energies after the last step (eV)                    */
+  int32_t slot;            /* batch slot it occupied (-1: never admitted)          */
+  uint64_t arena_offset, arena_bytes; /* device memory it occupied                 */
+} mmd_replica_result;
+
+typedef struct {
+  double wall_s;
+  double atom_steps;
+  double atom_steps_per_s;
+  double latency_p50_s, latency_p99_s, latency_mean_s;
+  double wait_mean_s;      /* arrival -> admission                                  */
+  int32_t completed, failed, max_active;
+  int64_t batched_steps;
+  double arena_bytes;
+} mmd_server_stats;
+
+int mmd_server_create(const mmd_server_opts *opts, const mmd_eam_tables *tab, mmd_server **out);
+int mmd_server_destroy(mmd_server *srv);
+/* Runs a whole arrival trace in real time; returns when every replica has
+ * completed or failed.  results[i] belongs to specs[i]. */
+int mmd_server_run(mmd_server *srv, const mmd_replica_spec *specs, int32_t n, mmd_replica_result *results,
+                   mmd_server_stats *stats);
+/* Device bytes a replica of `cells` occupies (the admission size). */
+int mmd_server_replica_bytes(int32_t cells, int32_t ghost_cells, size_t *bytes);
+
+/* The same scheduling loop driven by a simulated executor: a virtual clock,
+ * a fake device-memory budget and a linear cost model, no GPU.  Used to test
+ * the admission and batching policy. */
+typedef struct {
+  size_t arena_bytes;
+  int32_t max_replicas, chunk_steps, ghost_cells, workers;
+  double step_overhead_s; /* per batched step                                   */
+  double step_atom_s;     /* per atom-step                                      */
+  double load_atom_s;     /* per atom of initial-condition preparation          */
+} mmd_sim_params;
+int mmd_server_simulate(const mmd_sim_params *p, const mmd_replica_spec *specs, int32_t n,
+                        mmd_replica_result *results, mmd_server_stats *stats);
+
 /* ---------------------------------------------------------------- timing */
 
 #define MMD_K_RHO 0

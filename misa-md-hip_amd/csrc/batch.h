@@ -1,0 +1,56 @@
+// Batched (multi-replica) device descriptors and launchers used by the
+// serving loop.  One launch covers every active replica: a persistent grid
+// walks a device work list of (replica, logical block) items, so a captured
+// HIP graph stays valid while the mix of replicas changes (only the work list
+// in device memory is rewritten between steps).  Each item runs exactly the
+// per-block code of the single-domain kernels, and energies are reduced per
+// replica in a fixed order, so a replica's results do not depend on which
+// other replicas share the launch.
+#ifndef MMD_BATCH_H
+#define MMD_BATCH_H
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+
+namespace mmd {
+
+struct BatchDom {
+  Geom g;
+  double4 *pos;
+  double *vx, *vy, *vz, *fx, *fy, *fz, *rho, *df;
+  int8_t *type;
+  const int32_t *off;  // [2][kMaxStencil] device-layout stencil
+  int32_t nk_even, nk_odd;
+  double *e_rho, *e_force, *e_ke;  // per logical site block
+  double *energy;                  // {potential, kinetic} after the last step, then at admission
+  double L[3];
+  int64_t n_blocks;  // logical site blocks, as the single-domain launch numbers them (site_blocks)
+};
+
+struct BatchLists {
+  const BatchDom *dom;
+  const int2 *site_items;  // (replica, logical site block)
+  const int *n_site_items;
+  const int2 *ext_items;   // (replica, 256-site ext block)
+  const int *n_ext_items;
+  const int *reps;         // replica indices for per-replica reductions
+  const int *n_reps;
+};
+
+// grid of the persistent batched kernels (a multiple of the 8 XCDs)
+constexpr int kBatchGrid = 2048;
+
+hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, const Masses &M, double h, double dt, bool drift,
+                             bool want_ke);
+hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, uint32_t fields);
+hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, const TabF64 &tf, const TabMixed &tm, double rc2,
+                               int prec, bool want_e);
+hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, const TabF64 &tf, const TabMixed &tm, double rc2,
+                              int prec, bool want_e);
+// per-replica {potential, kinetic} -> energy[2 * which], which = 0 (step) or 1 (admission)
+hipError_t launch_batch_energy(hipStream_t st, const BatchLists &L, int which);
+
+}  // namespace mmd
+
+#endif

@@ -34,6 +34,27 @@ using namespace mmd;
 
 namespace mmd {
 
+Geom make_geom(const mmd_domain &d) {
+  Geom g{};
+  g.ext_x = d.ext[0], g.ext_y = d.ext[1], g.ext_z = d.ext[2];
+  g.exh = d.ext[0] / 2;
+  g.gxh = d.ghost[0] / 2, g.gy = d.ghost[1], g.gz = d.ghost[2];
+  g.bxh = d.box[0] / 2, g.by = d.box[1], g.bz = d.box[2];
+  g.chunks = (g.bxh + kWave - 1) / kWave;
+  g.ty = std::min<int32_t>(g.by, 16);
+  g.n_ytiles = (g.by + g.ty - 1) / g.ty;
+  g.n_ext = (int64_t)d.ext[0] * d.ext[1] * d.ext[2];
+  g.n_waves = (int64_t)2 * g.n_ytiles * g.ty * g.bz * g.chunks;
+  return g;
+}
+
+static int64_t floor_div2(int64_t v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); }
+
+int32_t stencil_dev_offset(int p, int64_t dX, int64_t dY, int64_t dZ, const mmd_domain &d) {
+  const int64_t s1 = (p + dX) & 1;
+  return (int32_t)(((dZ * d.ext[1] + dY) * 2 + (s1 - p)) * (d.ext[0] / 2) + floor_div2(p + dX));
+}
+
 int ctx_bind(mmd_ctx *c) {
   if (!c) return fail(MMD_E_INVALID, "null context");
   HIP_TRY(hipSetDevice(c->device));
@@ -270,16 +291,8 @@ extern "C" int mmd_set_domain(mmd_ctx *c, const mmd_domain *d) {
     c->has_domain = false;
     c->has_stencil = false;
     c->n_itl = 0;
-    Geom &g = c->g;
-    g.ext_x = d->ext[0], g.ext_y = d->ext[1], g.ext_z = d->ext[2];
-    g.exh = d->ext[0] / 2;
-    g.gxh = d->ghost[0] / 2, g.gy = d->ghost[1], g.gz = d->ghost[2];
-    g.bxh = d->box[0] / 2, g.by = d->box[1], g.bz = d->box[2];
-    g.chunks = (g.bxh + kWave - 1) / kWave;
-    g.ty = std::min<int32_t>(g.by, 16);
-    g.n_ytiles = (g.by + g.ty - 1) / g.ty;
-    g.n_ext = n_ext;
-    g.n_waves = (int64_t)2 * g.n_ytiles * g.ty * g.bz * g.chunks;
+    c->g = make_geom(*d);
+    const Geom &g = c->g;
     c->dom = *d;
     const size_t n = (size_t)n_ext;
     double **f64[] = {&c->vx, &c->vy, &c->vz, &c->fx, &c->fy, &c->fz, &c->rho, &c->df};
@@ -318,8 +331,6 @@ static int64_t centred_mod(int64_t v, int64_t n) {
   return r;
 }
 
-static int64_t floor_div2(int64_t v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); }
-
 extern "C" int mmd_set_stencil(mmd_ctx *c, const int64_t *even, int32_t n_even, const int64_t *odd, int32_t n_odd) {
   GUARD({
     clear_error();
@@ -345,9 +356,7 @@ extern "C" int mmd_set_stencil(mmd_ctx *c, const int64_t *even, int32_t n_even, 
             (!dX && !dY && !dZ))
           return fail(MMD_E_RANGE, "mmd_set_stencil: parity %d entry %d (offset %lld) leaves the ghost shell", p, k,
                       (long long)o);
-        const int64_t s1 = (p + dX) & 1;
-        const int64_t dev = ((dZ * d.ext[1] + dY) * 2 + (s1 - p)) * (d.ext[0] / 2) + floor_div2(p + dX);
-        host[(size_t)p * kMaxStencil + k] = (int32_t)dev;
+        host[(size_t)p * kMaxStencil + k] = stencil_dev_offset(p, dX, dY, dZ, d);
       }
     }
     HIP_TRY(hipStreamSynchronize(c->stream));

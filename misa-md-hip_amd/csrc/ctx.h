@@ -83,6 +83,11 @@ struct mmd_ctx {
 
 namespace mmd {
 
+// device geometry of a sub-domain (shared with the serving loop)
+Geom make_geom(const mmd_domain &d);
+// device-layout offset of stencil entry (dX doubled, dY, dZ) for centre parity p
+int32_t stencil_dev_offset(int p, int64_t dX, int64_t dY, int64_t dZ, const mmd_domain &d);
+
 int ctx_bind(mmd_ctx *c);
 void ctx_drop_graphs(mmd_ctx *c);
 int timed_begin(mmd_ctx *c, int cls, hipEvent_t *ev);

@@ -161,9 +161,12 @@ struct Site {
   bool active;  // inside the local box
 };
 
-__device__ __forceinline__ bool map_site(const Geom &g, Site &st) {
+// lb = logical block (the single-domain kernels use xcd_block(); the batched
+// serving kernels take it from their work list, so a replica's sites map to
+// blocks identically in both)
+__device__ __forceinline__ bool map_site_lb(const Geom &g, int64_t lb, Site &st) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t w = xcd_block() * kWavesPerBlock + (threadIdx.x >> 6);
+  const int64_t w = lb * kWavesPerBlock + (threadIdx.x >> 6);
   if (w >= g.n_waves) return false;
   const int64_t chunk = w % g.chunks;
   const int64_t rl = w / g.chunks;
@@ -180,6 +183,8 @@ __device__ __forceinline__ bool map_site(const Geom &g, Site &st) {
   st.i = dev_index(g, st.s, xh + g.gxh, yl + g.gy, zl + g.gz);
   return true;
 }
+
+__device__ __forceinline__ bool map_site(const Geom &g, Site &st) { return map_site_lb(g, xcd_block(), st); }
 
 }  // namespace mmd
 

@@ -1,0 +1,158 @@
+// Persistent batched kernels of the serving loop (see batch.h).  The work
+// list is split into one contiguous range per XCD, so the blocks of an XCD
+// walk neighbouring blocks of the same replica and share its halo in L2.
+#include "batch.h"
+#include "eam_body.h"
+#include "kernels.h"
+#include "md_body.h"
+
+namespace mmd {
+
+namespace {
+
+struct Range {
+  int begin, end, stride;
+};
+
+__device__ __forceinline__ Range xcd_range(int n) {
+  const int xcd = blockIdx.x % kXcds, per_block = gridDim.x / kXcds;
+  const int per = (n + kXcds - 1) / kXcds;
+  Range r;
+  r.begin = xcd * per + (int)(blockIdx.x / kXcds);
+  r.end = min(n, (xcd + 1) * per);
+  r.stride = per_block;
+  return r;
+}
+
+template <bool DRIFT, bool WANT_KE>
+__global__ __launch_bounds__(kBlock) void kb_kick(BatchLists L, Masses M, double h, double dt) {
+  __shared__ double lds[kWavesPerBlock];
+  const Range r = xcd_range(*L.n_site_items);
+  for (int it = r.begin; it < r.end; it += r.stride) {
+    const int2 w = L.site_items[it];
+    const BatchDom &d = L.dom[w.x];
+    kick_block<DRIFT, WANT_KE>(d.g, w.y, M, h, dt, d.pos, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.type, d.e_ke, lds);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void kb_ghost(BatchLists L, uint32_t fields) {
+  const Range r = xcd_range(*L.n_ext_items);
+  for (int it = r.begin; it < r.end; it += r.stride) {
+    const int2 w = L.ext_items[it];
+    const BatchDom &d = L.dom[w.x];
+    ghost_site(d.g, (uint32_t)w.y * kBlock + threadIdx.x, d.L[0], d.L[1], d.L[2], d.pos, d.type, d.df, fields);
+  }
+}
+
+template <int PREC, bool WANT_E>
+__global__ __launch_bounds__(kBlock) void kb_rho_df(BatchLists L, TabF64 tf, TabMixed tm, double rc2) {
+  __shared__ double lds[kWavesPerBlock];
+  const Range r = xcd_range(*L.n_site_items);
+  for (int it = r.begin; it < r.end; it += r.stride) {
+    const int2 w = L.site_items[it];
+    const BatchDom &d = L.dom[w.x];
+    rho_df_block<PREC, WANT_E>(d.g, w.y, d.off, d.nk_even, d.nk_odd, d.pos, d.rho, d.df, tf, tm, rc2, d.e_rho, lds);
+  }
+}
+
+template <int PREC, bool WANT_E>
+__global__ __launch_bounds__(kBlock) void kb_force(BatchLists L, TabF64 tf, TabMixed tm, double rc2) {
+  __shared__ double lds[kWavesPerBlock];
+  const Range r = xcd_range(*L.n_site_items);
+  for (int it = r.begin; it < r.end; it += r.stride) {
+    const int2 w = L.site_items[it];
+    const BatchDom &d = L.dom[w.x];
+    force_block<PREC, WANT_E>(d.g, w.y, d.off, d.nk_even, d.nk_odd, d.pos, d.df, d.fx, d.fy, d.fz, tf, tm, rc2,
+                              d.e_force, lds);
+  }
+}
+
+__device__ double ordered_sum_b(const double *__restrict__ p, int64_t n) {
+  double s = 0.0;
+  for (int64_t k = threadIdx.x; k < n; k += kBlock) s += p[k];
+  return s;
+}
+
+// one block per replica: the same fixed-order reduction as the single-domain
+// energy finalize; writes {potential, kinetic} to energy[2 * which]
+__global__ __launch_bounds__(kBlock) void kb_energy(BatchLists L, int which) {
+  __shared__ double lds[kWavesPerBlock];
+  const int n = *L.n_reps;
+  for (int q = blockIdx.x; q < n; q += gridDim.x) {
+    const BatchDom &d = L.dom[L.reps[q]];
+    double pe = ordered_sum_b(d.e_rho, d.n_blocks);
+    pe += ordered_sum_b(d.e_force, d.n_blocks);
+    const double kin = ordered_sum_b(d.e_ke, d.n_blocks);
+    const double PE = block_sum(pe, lds);
+    const double KE = block_sum(kin, lds);
+    if (threadIdx.x == 0) {
+      d.energy[2 * which] = PE;
+      d.energy[2 * which + 1] = KE;
+    }
+  }
+}
+
+}  // namespace
+
+hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, const Masses &M, double h, double dt, bool drift,
+                             bool want_ke) {
+  const dim3 grid(kBatchGrid), block(kBlock);
+  if (drift) {
+    if (want_ke)
+      kb_kick<true, true><<<grid, block, 0, st>>>(L, M, h, dt);
+    else
+      kb_kick<true, false><<<grid, block, 0, st>>>(L, M, h, dt);
+  } else {
+    if (want_ke)
+      kb_kick<false, true><<<grid, block, 0, st>>>(L, M, h, dt);
+    else
+      kb_kick<false, false><<<grid, block, 0, st>>>(L, M, h, dt);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, uint32_t fields) {
+  kb_ghost<<<dim3(kBatchGrid), dim3(kBlock), 0, st>>>(L, fields);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, const TabF64 &tf, const TabMixed &tm, double rc2,
+                               int prec, bool want_e) {
+  const dim3 grid(kBatchGrid), block(kBlock);
+  if (prec == 0) {
+    if (want_e)
+      kb_rho_df<0, true><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+    else
+      kb_rho_df<0, false><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+  } else {
+    if (want_e)
+      kb_rho_df<1, true><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+    else
+      kb_rho_df<1, false><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, const TabF64 &tf, const TabMixed &tm, double rc2,
+                              int prec, bool want_e) {
+  const dim3 grid(kBatchGrid), block(kBlock);
+  if (prec == 0) {
+    if (want_e)
+      kb_force<0, true><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+    else
+      kb_force<0, false><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+  } else {
+    if (want_e)
+      kb_force<1, true><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+    else
+      kb_force<1, false><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_energy(hipStream_t st, const BatchLists &L, int which) {
+  kb_energy<<<dim3(256), dim3(kBlock), 0, st>>>(L, which);
+  return hipGetLastError();
+}
+
+}  // namespace mmd

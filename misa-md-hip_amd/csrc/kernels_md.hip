@@ -1,36 +1,12 @@
 // gfx950 kernels around the EAM passes: layout conversion, host AoS record
 // pack/unpack, periodic ghost-shell refresh, velocity Verlet and the
 // deterministic energy reduction.  All are HBM-bound streaming kernels.
-#include "device_common.h"
 #include "kernels.h"
+#include "md_body.h"
 
 namespace mmd {
 
 namespace {
-
-constexpr double kFtm2v = 9.64853321e-3;  // (eV/A/amu) -> A/fs^2
-constexpr double kMvv2e = 1.0 / kFtm2v;   // amu A^2/fs^2 -> eV
-
-struct ExtPos {
-  int32_t s, xh, ye, ze;
-};
-
-// device index -> ext coordinates (32-bit arithmetic: n_ext < 2^31 is checked at set-up)
-__device__ __forceinline__ ExtPos ext_of(const Geom &g, uint32_t idx) {
-  ExtPos p;
-  const uint32_t row = idx / (uint32_t)g.exh;
-  p.xh = (int32_t)(idx - row * (uint32_t)g.exh);
-  p.s = (int32_t)(row & 1u);
-  const uint32_t yz = row >> 1;
-  const uint32_t ze = yz / (uint32_t)g.ext_y;
-  p.ye = (int32_t)(yz - ze * (uint32_t)g.ext_y);
-  p.ze = (int32_t)ze;
-  return p;
-}
-
-__device__ __forceinline__ int64_t ref_of(const Geom &g, const ExtPos &p) {
-  return ((int64_t)p.ze * g.ext_y + p.ye) * g.ext_x + 2 * p.xh + p.s;
-}
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_ref2dev(Geom g, const T *__restrict__ ref, T *__restrict__ dev) {
@@ -128,38 +104,9 @@ __global__ __launch_bounds__(kBlock) void k_aos_pack(Geom g, char *__restrict__ 
   }
 }
 
-__device__ __forceinline__ int32_t wrap_img(int32_t v, int32_t n, int32_t &img) {
-  if (v < 0) {
-    img = -1;
-    return v + n;
-  }
-  if (v >= n) {
-    img = 1;
-    return v - n;
-  }
-  img = 0;
-  return v;
-}
-
 __global__ __launch_bounds__(kBlock) void k_ghost(Geom g, double Lx, double Ly, double Lz, double4 *__restrict__ P,
                                                     int8_t *__restrict__ T, double *__restrict__ DF, uint32_t fields) {
-  const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= (uint32_t)g.n_ext) return;
-  const ExtPos p = ext_of(g, idx);
-  const int32_t xl = p.xh - g.gxh, yl = p.ye - g.gy, zl = p.ze - g.gz;
-  if (xl >= 0 && xl < g.bxh && yl >= 0 && yl < g.by && zl >= 0 && zl < g.bz) return;
-  int32_t ix, iy, iz;
-  const int32_t sx = wrap_img(xl, g.bxh, ix), sy = wrap_img(yl, g.by, iy), sz = wrap_img(zl, g.bz, iz);
-  const int64_t src = dev_index(g, p.s, sx + g.gxh, sy + g.gy, sz + g.gz);
-  if (fields & 0x01u) {
-    T[idx] = T[src];
-    double4 q = P[src];
-    q.x += (double)ix * Lx;
-    q.y += (double)iy * Ly;
-    q.z += (double)iz * Lz;
-    P[idx] = q;
-  }
-  if (fields & 0x10u) DF[idx] = DF[src];
+  ghost_site(g, blockIdx.x * kBlock + threadIdx.x, Lx, Ly, Lz, P, T, DF, fields);
 }
 
 template <bool DRIFT, bool WANT_KE>
@@ -169,31 +116,7 @@ __global__ __launch_bounds__(kBlock) void k_kick(Geom g, Masses M, double h, dou
                                                    const double *__restrict__ FY, const double *__restrict__ FZ,
                                                    const int8_t *__restrict__ T, double *__restrict__ ke_part) {
   __shared__ double lds[kWavesPerBlock];
-  double ke = 0.0;
-  Site st;
-  if (map_site(g, st) && st.active) {
-    const int64_t i = st.i;
-    const int t = T[i];
-    if (t >= 0) {
-      const double sc = h * kFtm2v * M.inv_m[t];
-      const double vx = VX[i] + sc * FX[i], vy = VY[i] + sc * FY[i], vz = VZ[i] + sc * FZ[i];
-      VX[i] = vx;
-      VY[i] = vy;
-      VZ[i] = vz;
-      if (DRIFT) {
-        double4 q = P[i];
-        q.x += dt * vx;
-        q.y += dt * vy;
-        q.z += dt * vz;
-        P[i] = q;
-      }
-      if (WANT_KE) ke = 0.5 * M.m[t] * (vx * vx + vy * vy + vz * vz) * kMvv2e;
-    }
-  }
-  if (WANT_KE) {
-    const double b = block_sum(ke, lds);
-    if (threadIdx.x == 0) ke_part[xcd_block()] = b;
-  }
+  kick_block<DRIFT, WANT_KE>(g, xcd_block(), M, h, dt, P, VX, VY, VZ, FX, FY, FZ, T, ke_part, lds);
 }
 
 __device__ double ordered_sum(const double *__restrict__ p, int64_t n) {

@@ -1,0 +1,107 @@
+// Device code of the streaming passes shared by the single-domain kernels
+// (kernels_md.hip) and the batched serving kernels (kernels_batch.hip):
+// ext-site coordinates, periodic ghost refresh and velocity Verlet.
+#ifndef MMD_MD_BODY_H
+#define MMD_MD_BODY_H
+
+#include "device_common.h"
+
+namespace mmd {
+
+constexpr double kFtm2v = 9.64853321e-3;  // (eV/A/amu) -> A/fs^2
+constexpr double kMvv2e = 1.0 / kFtm2v;   // amu A^2/fs^2 -> eV
+
+struct ExtPos {
+  int32_t s, xh, ye, ze;
+};
+
+// device index -> ext coordinates (32-bit arithmetic: n_ext < 2^31 is checked at set-up)
+__device__ __forceinline__ ExtPos ext_of(const Geom &g, uint32_t idx) {
+  ExtPos p;
+  const uint32_t row = idx / (uint32_t)g.exh;
+  p.xh = (int32_t)(idx - row * (uint32_t)g.exh);
+  p.s = (int32_t)(row & 1u);
+  const uint32_t yz = row >> 1;
+  const uint32_t ze = yz / (uint32_t)g.ext_y;
+  p.ye = (int32_t)(yz - ze * (uint32_t)g.ext_y);
+  p.ze = (int32_t)ze;
+  return p;
+}
+
+__device__ __forceinline__ int64_t ref_of(const Geom &g, const ExtPos &p) {
+  return ((int64_t)p.ze * g.ext_y + p.ye) * g.ext_x + 2 * p.xh + p.s;
+}
+
+__device__ __forceinline__ int32_t wrap_img(int32_t v, int32_t n, int32_t &img) {
+  if (v < 0) {
+    img = -1;
+    return v + n;
+  }
+  if (v >= n) {
+    img = 1;
+    return v - n;
+  }
+  img = 0;
+  return v;
+}
+
+// periodic single-domain ghost refresh of ext site idx (no-op for local sites)
+__device__ __forceinline__ void ghost_site(const Geom &g, uint32_t idx, double Lx, double Ly, double Lz,
+                                           double4 *__restrict__ P, int8_t *__restrict__ T, double *__restrict__ DF,
+                                           uint32_t fields) {
+  if (idx >= (uint32_t)g.n_ext) return;
+  const ExtPos p = ext_of(g, idx);
+  const int32_t xl = p.xh - g.gxh, yl = p.ye - g.gy, zl = p.ze - g.gz;
+  if (xl >= 0 && xl < g.bxh && yl >= 0 && yl < g.by && zl >= 0 && zl < g.bz) return;
+  int32_t ix, iy, iz;
+  const int32_t sx = wrap_img(xl, g.bxh, ix), sy = wrap_img(yl, g.by, iy), sz = wrap_img(zl, g.bz, iz);
+  const int64_t src = dev_index(g, p.s, sx + g.gxh, sy + g.gy, sz + g.gz);
+  if (fields & 0x01u) {
+    T[idx] = T[src];
+    double4 q = P[src];
+    q.x += (double)ix * Lx;
+    q.y += (double)iy * Ly;
+    q.z += (double)iz * Lz;
+    P[idx] = q;
+  }
+  if (fields & 0x10u) DF[idx] = DF[src];
+}
+
+// velocity Verlet on the sites of logical block lb: v += h a (+ x += dt v)
+template <bool DRIFT, bool WANT_KE>
+__device__ __forceinline__ void kick_block(const Geom &g, int64_t lb, const Masses &M, double h, double dt,
+                                           double4 *__restrict__ P, double *__restrict__ VX, double *__restrict__ VY,
+                                           double *__restrict__ VZ, const double *__restrict__ FX,
+                                           const double *__restrict__ FY, const double *__restrict__ FZ,
+                                           const int8_t *__restrict__ T, double *__restrict__ ke_part,
+                                           double *lds) {
+  double ke = 0.0;
+  Site st;
+  if (map_site_lb(g, lb, st) && st.active) {
+    const int64_t i = st.i;
+    const int t = T[i];
+    if (t >= 0) {
+      const double sc = h * kFtm2v * M.inv_m[t];
+      const double vx = VX[i] + sc * FX[i], vy = VY[i] + sc * FY[i], vz = VZ[i] + sc * FZ[i];
+      VX[i] = vx;
+      VY[i] = vy;
+      VZ[i] = vz;
+      if (DRIFT) {
+        double4 q = P[i];
+        q.x += dt * vx;
+        q.y += dt * vy;
+        q.z += dt * vz;
+        P[i] = q;
+      }
+      if (WANT_KE) ke = 0.5 * M.m[t] * (vx * vx + vy * vy + vz * vz) * kMvv2e;
+    }
+  }
+  if (WANT_KE) {
+    const double b = block_sum(ke, lds);
+    if (threadIdx.x == 0) ke_part[lb] = b;
+  }
+}
+
+}  // namespace mmd
+
+#endif

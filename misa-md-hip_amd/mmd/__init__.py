@@ -78,6 +78,39 @@ class LatticeSpec(C.Structure):
                 ("seed", C.c_uint64)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("precision", C.c_int32), ("lattice_const", C.c_double),
+                ("cutoff", C.c_double), ("skin", C.c_double), ("dt", C.c_double), ("ghost_cells", C.c_int32),
+                ("type_cum", C.c_double * 8), ("sigma", C.c_double), ("max_replicas", C.c_int32),
+                ("hbm_fraction", C.c_double), ("hbm_limit", C.c_size_t), ("chunk_steps", C.c_int32),
+                ("workers", C.c_int32), ("use_graph", C.c_int32)]
+
+
+class ReplicaSpec(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("cells", C.c_int32), ("temperature", C.c_double), ("steps", C.c_int32),
+                ("seed", C.c_uint64), ("arrival_s", C.c_double), ("keep", C.POINTER(HostSoa))]
+
+
+class ReplicaResult(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int32), ("n_atoms", C.c_int32), ("t_arrival", C.c_double),
+                ("t_admit", C.c_double), ("t_ready", C.c_double), ("t_done", C.c_double),
+                ("epot0", C.c_double), ("ekin0", C.c_double), ("epot", C.c_double), ("ekin", C.c_double),
+                ("slot", C.c_int32), ("arena_offset", C.c_uint64), ("arena_bytes", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("atom_steps", C.c_double), ("atom_steps_per_s", C.c_double),
+                ("latency_p50_s", C.c_double), ("latency_p99_s", C.c_double), ("latency_mean_s", C.c_double),
+                ("wait_mean_s", C.c_double), ("completed", C.c_int32), ("failed", C.c_int32),
+                ("max_active", C.c_int32), ("batched_steps", C.c_int64), ("arena_bytes", C.c_double)]
+
+
+class SimParams(C.Structure):
+    _fields_ = [("arena_bytes", C.c_size_t), ("max_replicas", C.c_int32), ("chunk_steps", C.c_int32),
+                ("ghost_cells", C.c_int32), ("workers", C.c_int32), ("step_overhead_s", C.c_double),
+                ("step_atom_s", C.c_double), ("load_atom_s", C.c_double)]
+
+
 # reference record _cuAtomElement (src/kernel_types.h:40-48)
 REF_AOS_DTYPE = np.dtype({"names": ["id", "type", "x", "v", "f", "rho", "df"],
                           "formats": [np.int64, np.int32, (np.float64, 3), (np.float64, 3), (np.float64, 3),
@@ -132,6 +165,13 @@ def lib():
         "mmd_lattice_domain": (C.c_int, [C.POINTER(LatticeSpec), C.POINTER(Domain), _d]),
         "mmd_lattice_init": (C.c_int, [C.POINTER(LatticeSpec), C.POINTER(HostSoa), C.c_int32, _i32,
                                        _d, _d, _d, _d, _d, _d, _i8]),
+        "mmd_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(EamTables), C.POINTER(P)]),
+        "mmd_server_destroy": (C.c_int, [P]),
+        "mmd_server_run": (C.c_int, [P, C.POINTER(ReplicaSpec), C.c_int32, C.POINTER(ReplicaResult),
+                                     C.POINTER(ServerStats)]),
+        "mmd_server_replica_bytes": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_size_t)]),
+        "mmd_server_simulate": (C.c_int, [C.POINTER(SimParams), C.POINTER(ReplicaSpec), C.c_int32,
+                                          C.POINTER(ReplicaResult), C.POINTER(ServerStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -419,3 +459,92 @@ def wide_stencil(a, rc, skin, reach=1.6):
     site within rc + skin + reach of it."""
     w = stencil_build(a, rc + skin + reach, 0)
     return np.vstack([np.zeros((1, 3), np.int32), w])
+
+
+# -------------------------------------------------------------------- serving
+
+def _struct_dict(o):
+    return {k: getattr(o, k) for k, _ in o._fields_}
+
+
+def replica_bytes(cells, ghost_cells=3):
+    b = C.c_size_t(0)
+    _check(lib().mmd_server_replica_bytes(int(cells), int(ghost_cells), C.byref(b)))
+    return b.value
+
+
+def _spec_array(reps):
+    """reps: sequence of dicts with cells, temperature, steps, seed, arrival_s and optional id / keep
+    (a HostState that receives the final ext state in the reference layout)."""
+    arr = (ReplicaSpec * max(len(reps), 1))()
+    keep = []
+    for i, r in enumerate(reps):
+        a = arr[i]
+        a.id = int(r.get("id", i))
+        a.cells, a.temperature, a.steps = int(r["cells"]), float(r["temperature"]), int(r["steps"])
+        a.seed, a.arrival_s = int(r["seed"]), float(r.get("arrival_s", 0.0))
+        if r.get("keep") is not None:
+            h = r["keep"].soa()
+            keep.append(h)
+            a.keep = C.pointer(h)
+    return arr, keep
+
+
+def simulate(reps, arena_bytes, max_replicas=8, chunk_steps=10, ghost_cells=3, workers=4, step_overhead_s=1e-4,
+             step_atom_s=1e-9, load_atom_s=1e-8):
+    """Runs the serving scheduler against a simulated executor (virtual clock, fake memory budget)."""
+    p = SimParams(int(arena_bytes), max_replicas, chunk_steps, ghost_cells, workers, step_overhead_s, step_atom_s,
+                  load_atom_s)
+    arr, _ = _spec_array(reps)
+    res = (ReplicaResult * max(len(reps), 1))()
+    st = ServerStats()
+    _check(lib().mmd_server_simulate(C.byref(p), arr, len(reps), res, C.byref(st)))
+    return [_struct_dict(res[i]) for i in range(len(reps))], _struct_dict(st)
+
+
+class Server:
+    """Continuous-batching MD server (mmd_server): replicas of a random alloy, one GPU."""
+
+    def __init__(self, tab, masses=DEFAULT_MASSES, device=0, precision=PREC_MIXED, a=2.855, cutoff=5.6, skin=0.3,
+                 dt=1.0, ghost_cells=3, type_frac=(0.9, 0.05, 0.05), sigma=0.05, max_replicas=64,
+                 hbm_fraction=0.5, hbm_limit=0, chunk_steps=10, workers=8, use_graph=True):
+        o = ServerOpts()
+        o.device, o.precision, o.lattice_const, o.cutoff, o.skin, o.dt = device, precision, a, cutoff, skin, dt
+        o.ghost_cells = ghost_cells
+        nt = tab["n_types"]
+        fr = np.asarray(type_frac[:nt], float) if nt > 1 else np.ones(1)
+        cum = np.cumsum(fr / fr.sum())
+        for t in range(8):
+            o.type_cum[t] = cum[t] if t < nt else 1.0
+        o.sigma, o.max_replicas, o.hbm_fraction, o.hbm_limit = sigma, max_replicas, hbm_fraction, int(hbm_limit)
+        o.chunk_steps, o.workers, o.use_graph = chunk_steps, workers, int(bool(use_graph))
+        self._tab = tables_struct(tab, masses)
+        h = C.c_void_p()
+        _check(lib().mmd_server_create(C.byref(o), C.byref(self._tab), C.byref(h)))
+        self.h = h
+        self.opts = o
+
+    def run(self, reps):
+        arr, keep = _spec_array(reps)
+        res = (ReplicaResult * max(len(reps), 1))()
+        st = ServerStats()
+        _check(lib().mmd_server_run(self.h, arr, len(reps), res, C.byref(st)))
+        del keep
+        return [_struct_dict(res[i]) for i in range(len(reps))], _struct_dict(st)
+
+    def close(self):
+        if self.h:
+            lib().mmd_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
