@@ -15,6 +15,11 @@ The same run also measures
     plus 0.5% vacancies and 0.2% interstitials at tetrahedral sites, 1200 K):
     atom-steps/s of both, force error of mixed vs FP64 on identical states
     (start and end of the run) and the per-step energy difference per atom;
+  * the serving loop: 2,000 independent replicas (box edge log-uniform in
+    10..64 BCC cells, 3-type alloy, 300-1500 K, 200 steps each, mixed
+    precision) arriving as a Poisson stream at 75% of the capacity measured
+    by a closed-loop burst just before; throughput over the whole trace and
+    p50/p99 per-replica latency (arrival to results);
   * the CPU oracle (FP64 C restatement of the reference's EAM algorithm,
     OpenMP on the host cores) on the headline box.
 
@@ -198,15 +203,95 @@ def mixed_vs_fp64(w, steps, warmup):
             "interstitial_ms_per_step": round(mk[mmd.K_ITL] / steps, 4)}
 
 
+def dropin_hooks(w, reps=3):
+    """The reference hook sequence on host records: per step the rho pass (df fused) and the force pass, each
+    copying the whole ghost-extended array of 104-byte _cuAtomElement records (pinned, as hip_create_atoms_mem
+    allocates them) to the device and the local layers back, as the reference plugin does."""
+    import ctypes as C
+    n = w["dom"].n_ext()
+    nbytes = n * mmd.REF_AOS_DTYPE.itemsize
+    p = C.c_void_p()
+    mmd._check(mmd.lib().mmd_host_alloc(C.byref(p), nbytes))
+    try:
+        recs = np.frombuffer((C.c_char * nbytes).from_address(p.value), dtype=mmd.REF_AOS_DTYPE)
+        st = w["st"]
+        recs["id"] = np.arange(n)
+        recs["type"] = st.type
+        recs["x"] = np.stack([st.x, st.y, st.z], 1)
+        recs["v"] = np.stack([st.vx, st.vy, st.vz], 1)
+        with mmd.Context(0) as ctx:
+            ctx.set_domain(w["dom"])
+            ctx.set_stencil(w["even"], w["odd"])
+            ctx.set_tables(w["tab"], mmd.DEFAULT_MASSES)
+            ctx.rho_calc_aos(recs, RC)
+            ctx.force_calc_aos(recs, RC)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                ctx.rho_calc_aos(recs, RC)
+                ctx.force_calc_aos(recs, RC)
+            dt = (time.perf_counter() - t0) / reps
+        del recs
+    finally:
+        mmd.lib().mmd_host_free(p)
+    atoms = n_atoms_of(w)
+    return {"what": "rho (fused df) + force hooks on pinned host _cuAtomElement records, PCIe copies included",
+            "atom_passes_per_s": round(atoms / dt, 1), "ms_per_step": round(dt * 1e3, 3),
+            "host_bytes_per_step": int(2 * (nbytes + w["dom"].box[2] * w["dom"].ext[0] * w["dom"].ext[1] * 104))}
+
+
+def replica_trace(n, seed, rate, t0=0.0, steps=200):
+    """Replicas with box edge log-uniform in [10, 64] cells, 300-1500 K, Poisson arrivals at `rate`/s."""
+    rng = np.random.default_rng(seed)
+    cells = np.rint(np.exp(rng.uniform(np.log(10.0), np.log(64.0), n))).astype(int)
+    temps = rng.uniform(300.0, 1500.0, n)
+    arr = t0 + np.cumsum(rng.exponential(1.0 / rate, n)) if rate > 0 else np.full(n, t0)
+    return [dict(id=i, cells=int(cells[i]), temperature=float(temps[i]), steps=steps, seed=seed * 100003 + i,
+                 arrival_s=float(arr[i])) for i in range(n)]
+
+
+def serving(n_replicas, load, seed=77, burst=256, steps=200):
+    tab = mmd.synth_tables(n_types=3, rc=RC, jitter=0.1, seed=seed)
+    with mmd.Server(tab, precision=mmd.PREC_MIXED, a=A, cutoff=RC, skin=SKIN, dt=DT, sigma=0.05, max_replicas=64,
+                    hbm_fraction=0.5, chunk_steps=10, workers=16) as srv:
+        # capacity: closed-loop burst of the same replica mix, all present at t = 0
+        res, st = srv.run(replica_trace(burst, seed + 1, 0.0, steps=steps))  # queue never empties until the tail
+        t_start = min(r["t_ready"] for r in res)
+        t_end = max(r["t_done"] for r in res)
+        capacity = st["atom_steps"] / (t_end - t_start)
+        reps = replica_trace(n_replicas, seed, 1.0, steps=steps)
+        mean_work = np.mean([2 * r["cells"] ** 3 * r["steps"] for r in reps])
+        rate = load * capacity / mean_work
+        reps = replica_trace(n_replicas, seed, rate, steps=steps)
+        log(f"bench: serving {n_replicas} replicas, capacity {capacity:.3e} atom-steps/s, "
+            f"arrivals {rate:.2f}/s (load {load})")
+        res, st = srv.run(reps)
+    done = [r for r in res if r["status"] == 0]
+    de = [abs(r["epot"] + r["ekin"] - r["epot0"] - r["ekin0"]) / r["n_atoms"] / steps for r in done]
+    return {"workload": f"{n_replicas} replicas, box edge log-uniform 10-64 BCC cells (2k-524k atoms), 3-type alloy, "
+                        f"300-1500 K, {steps} steps each, Poisson arrivals at {load:.0%} of measured capacity, "
+                        "mixed precision, batched persistent kernels under a captured HIP graph",
+            "value": round(st["atom_steps_per_s"], 1), "unit": "atom-steps/s",
+            "capacity_atom_steps_per_s": round(capacity, 1), "arrival_rate_per_s": round(rate, 3),
+            "offered_load": load, "latency_p50_s": round(st["latency_p50_s"], 4),
+            "latency_p99_s": round(st["latency_p99_s"], 4), "latency_mean_s": round(st["latency_mean_s"], 4),
+            "wait_mean_s": round(st["wait_mean_s"], 4), "wall_s": round(st["wall_s"], 3),
+            "atom_steps": st["atom_steps"], "completed": st["completed"], "failed": st["failed"],
+            "max_active": st["max_active"], "batched_steps": st["batched_steps"],
+            "total_energy_change_per_atom_per_step_median": float(np.median(de)) if de else None}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cells", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-replicas", type=int, default=2000)
+    ap.add_argument("--serve-load", type=float, default=0.75)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only time a device-to-device streaming copy of BYTES (HBM bandwidth / counter calibration)")
     args = ap.parse_args()
@@ -239,7 +324,9 @@ def main():
     flops_rho = (8 * S + 10 * Nc + 10) * n_atoms
     bytes_force = 57 * n_atoms
     achieved = flops_force / t_force / 1e12
-    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "roof_note": "compute roof: FP64 vector ALU (no matrix instructions in this kernel); MI355X's FP64 "
+                             "vector and matrix peaks are both 78.6 TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": load_traffic(),
                 "kernel": "k_force<FP64>", "kernel_ms": round(t_force * 1e3, 4),
                 "flops_per_atom": round(flops_atom, 1), "alg_bytes_per_atom": 57,
@@ -258,10 +345,17 @@ def main():
            "kernel_ms_per_step": {k: round(ms[i] / max(args.steps, 1), 4) for i, k in
                                   enumerate(("rho_df", "force", "integrate", "ghost", "interstitial"))}}
 
+    log("bench: drop-in hook path on host records (PCIe included)")
+    out["dropin_pcie"] = dropin_hooks(w)
+
     if not args.no_mixed:
         log("bench: mixed vs FP64 on the defect box (0.5% vacancies, 0.2% interstitials, 1200 K)")
         wd = make_workload(args.cells, seed=2025, temperature=1200.0, vacancy_frac=0.005, interstitial_frac=0.002)
         out["mixed"] = mixed_vs_fp64(wd, args.steps, args.warmup)
+
+    if not args.no_serve:
+        out["serving"] = serving(args.serve_replicas, args.serve_load)
+        out["serving"]["capacity_vs_fp64_path"] = round(out["serving"]["capacity_atom_steps_per_s"] / value, 3)
 
     if not args.no_cpu:
         log(f"bench: CPU oracle baseline, {args.cpu_steps} steps")
