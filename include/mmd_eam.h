@@ -102,6 +102,8 @@ typedef struct {
 /* hip_env_init (src/hip_arch_acc_imp.cpp:28-40): binds the context to
  * `device` (the reference always used device 0) and creates its stream. */
 int mmd_ctx_create(int32_t device, mmd_ctx **out);
+/* Number of visible GPUs (a rank picks its device from it). */
+int mmd_device_count(int32_t *n);
 /* hip_env_clean (src/hip_arch_acc_imp.cpp:42-45): frees everything the
  * context owns (the reference leaked its buffers). */
 int mmd_ctx_destroy(mmd_ctx *ctx);
@@ -325,6 +327,14 @@ typedef struct {
 } mmd_synth_params;
 int mmd_synth_tables(const mmd_synth_params *p, double *rho_r, double *emb, double *phi_r, double *r0, double *dr,
                      double *rho0, double *drho);
+
+/* EAM tables from a DYNAMO setfl file (LAMMPS eam/alloy format), the on-disk
+ * source of the tables hip_pot_init copies (src/hip_arch_acc_imp.cpp:146-150).
+ * mmd_setfl_info gives the sizes; mmd_setfl_read fills caller arrays shaped
+ * as mmd_eam_tables expects (phi = (r phi) / r on the r grid, phi symmetric). */
+int mmd_setfl_info(const char *path, int32_t *n_types, int32_t *n_r, int32_t *n_rho, double *cutoff);
+int mmd_setfl_read(const char *path, double *rho_r, double *emb, double *phi_r, double *mass, double *r0, double *dr,
+                   double *rho0, double *drho);
 
 /* Initial conditions for a periodic single-domain BCC box of cells[3] unit
  * cells: types drawn from `type_frac` (cumulative, n_types entries), Gaussian

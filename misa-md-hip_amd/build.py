@@ -16,7 +16,7 @@ OUT = os.path.join(HERE, "mmd", "libmmd_eam.so")
 BUILD = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["kernels_eam.hip", "kernels_md.hip", "kernels_batch.hip", "context.hip", "itl.hip", "serve.hip"]
-CXX_SOURCES = ["host_helpers.cpp"]
+CXX_SOURCES = ["host_helpers.cpp", "setfl.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIPFLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",

@@ -250,6 +250,15 @@ extern "C" int mmd_ctx_create(int32_t device, mmd_ctx **out) {
   })
 }
 
+extern "C" int mmd_device_count(int32_t *n) {
+  clear_error();
+  if (!n) return fail(MMD_E_INVALID, "mmd_device_count: null out");
+  int k = 0;
+  HIP_TRY(hipGetDeviceCount(&k));
+  *n = k;
+  return MMD_OK;
+}
+
 extern "C" int mmd_ctx_destroy(mmd_ctx *c) {
   GUARD({
     if (!c) return MMD_OK;

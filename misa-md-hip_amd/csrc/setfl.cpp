@@ -1,0 +1,142 @@
+// Reader of EAM potential files in the DYNAMO "setfl" format (LAMMPS
+// pair_style eam/alloy), the on-disk form of the tables MISA-MD's eam object
+// holds and hip_pot_init copies to the device (src/hip_arch_acc_imp.cpp:146-150).
+//
+// Format: three comment lines; "N el_1 .. el_N"; "Nrho drho Nr dr cutoff";
+// per element a line "Z mass a lattice" followed by Nrho values of F(rho) on
+// rho = k * drho and Nr values of rho(r) on r = k * dr; then, for i = 0..N-1
+// and j = 0..i, Nr values of r * phi_ij(r).  Values may wrap across lines.
+//
+// The library's tables hold phi itself: phi_ij(r_k) = (r phi)_k / r_k for
+// k >= 1 and a linear extrapolation from k = 1, 2 at r = 0 (a point no pair
+// reaches).  LAMMPS instead interpolates r * phi and divides by r per pair; the
+// two interpolants agree to the tables' O(dr^4) interpolation error.
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "host_helpers.h"
+#include "mmd_eam.h"
+
+namespace {
+
+struct Reader {
+  FILE *f = nullptr;
+  std::string tok;
+  int line = 1;
+  ~Reader() {
+    if (f) std::fclose(f);
+  }
+  // next line as text (for the comment lines and the element list)
+  bool getline(std::string &out) {
+    out.clear();
+    int c;
+    while ((c = std::fgetc(f)) != EOF && c != '\n') out.push_back((char)c);
+    if (c == '\n') line++;
+    return c != EOF || !out.empty();
+  }
+  bool next(std::string &t) {
+    t.clear();
+    int c;
+    while ((c = std::fgetc(f)) != EOF && (c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == ','))
+      if (c == '\n') line++;
+    if (c == EOF) return false;
+    do t.push_back((char)c);
+    while ((c = std::fgetc(f)) != EOF && !(c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == ','));
+    if (c == '\n') line++;
+    return true;
+  }
+  bool number(double &v) {
+    if (!next(tok)) return false;
+    char *end = nullptr;
+    errno = 0;
+    v = std::strtod(tok.c_str(), &end);
+    return end && *end == '\0' && errno == 0 && std::isfinite(v);
+  }
+  bool integer(long &v) {
+    double d;
+    if (!number(d) || d != std::floor(d)) return false;
+    v = (long)d;
+    return true;
+  }
+};
+
+struct Header {
+  int32_t n_types = 0, n_r = 0, n_rho = 0;
+  double dr = 0.0, drho = 0.0, cutoff = 0.0;
+};
+
+int read_header(Reader &R, const char *path, Header &h) {
+  R.f = std::fopen(path, "r");
+  if (!R.f) return mmd::fail(MMD_E_INVALID, "setfl: cannot open %s", path);
+  std::string s;
+  for (int k = 0; k < 3; k++)
+    if (!R.getline(s)) return mmd::fail(MMD_E_INVALID, "setfl %s: missing comment line %d", path, k + 1);
+  long n;
+  if (!R.integer(n) || n < 1 || n > 8) return mmd::fail(MMD_E_INVALID, "setfl %s: element count must be 1..8", path);
+  h.n_types = (int32_t)n;
+  if (!R.getline(s)) return mmd::fail(MMD_E_INVALID, "setfl %s: element names missing", path);
+  long nrho, nr;
+  if (!R.integer(nrho) || !R.number(h.drho) || !R.integer(nr) || !R.number(h.dr) || !R.number(h.cutoff) ||
+      nrho < 5 || nr < 5 || !(h.drho > 0.0) || !(h.dr > 0.0) || !(h.cutoff > 0.0) || nrho > (1 << 24) ||
+      nr > (1 << 24))
+    return mmd::fail(MMD_E_INVALID, "setfl %s: bad grid line (Nrho drho Nr dr cutoff)", path);
+  h.n_rho = (int32_t)nrho, h.n_r = (int32_t)nr;
+  return MMD_OK;
+}
+
+}  // namespace
+
+extern "C" int mmd_setfl_info(const char *path, int32_t *n_types, int32_t *n_r, int32_t *n_rho, double *cutoff) {
+  mmd::clear_error();
+  if (!path) return mmd::fail(MMD_E_INVALID, "mmd_setfl_info: null path");
+  Reader R;
+  Header h;
+  int rc = read_header(R, path, h);
+  if (rc) return rc;
+  if (n_types) *n_types = h.n_types;
+  if (n_r) *n_r = h.n_r;
+  if (n_rho) *n_rho = h.n_rho;
+  if (cutoff) *cutoff = h.cutoff;
+  return MMD_OK;
+}
+
+extern "C" int mmd_setfl_read(const char *path, double *rho_r, double *emb, double *phi_r, double *mass,
+                              double *r0, double *dr, double *rho0, double *drho) {
+  mmd::clear_error();
+  if (!path || !rho_r || !emb || !phi_r || !mass) return mmd::fail(MMD_E_INVALID, "mmd_setfl_read: null argument");
+  Reader R;
+  Header h;
+  int rc = read_header(R, path, h);
+  if (rc) return rc;
+  const int nt = h.n_types, nr = h.n_r, nrho = h.n_rho;
+  auto bad = [&](const char *what) {
+    return mmd::fail(MMD_E_INVALID, "setfl %s: %s (near line %d)", path, what, R.line);
+  };
+  for (int t = 0; t < nt; t++) {
+    long z;
+    double a;
+    if (!R.integer(z) || !R.number(mass[t]) || !R.number(a) || !R.next(R.tok) || !(mass[t] > 0.0))
+      return bad("element line (Z mass a lattice)");
+    for (int k = 0; k < nrho; k++)
+      if (!R.number(emb[(size_t)t * nrho + k])) return bad("F(rho) value");
+    for (int k = 0; k < nr; k++)
+      if (!R.number(rho_r[(size_t)t * nr + k])) return bad("rho(r) value");
+  }
+  std::vector<double> z2r((size_t)nr);
+  for (int i = 0; i < nt; i++)
+    for (int j = 0; j <= i; j++) {
+      for (int k = 0; k < nr; k++)
+        if (!R.number(z2r[(size_t)k])) return bad("r*phi(r) value");
+      double *p = phi_r + ((size_t)i * nt + j) * nr;
+      for (int k = 1; k < nr; k++) p[k] = z2r[(size_t)k] / (k * h.dr);
+      p[0] = 2.0 * p[1] - p[2];
+      if (i != j) std::memcpy(phi_r + ((size_t)j * nt + i) * nr, p, sizeof(double) * (size_t)nr);
+    }
+  *r0 = 0.0, *dr = h.dr, *rho0 = 0.0, *drho = h.drho;
+  return MMD_OK;
+}

@@ -133,6 +133,7 @@ def lib():
         "mmd_last_error": (C.c_char_p, []),
         "mmd_ctx_create": (C.c_int, [C.c_int32, C.POINTER(P)]),
         "mmd_ctx_destroy": (C.c_int, [P]),
+        "mmd_device_count": (C.c_int, [_i32]),
         "mmd_set_domain": (C.c_int, [P, C.POINTER(Domain)]),
         "mmd_set_stencil": (C.c_int, [P, _i64, C.c_int32, _i64, C.c_int32]),
         "mmd_set_tables": (C.c_int, [P, C.POINTER(EamTables)]),
@@ -165,6 +166,8 @@ def lib():
         "mmd_lattice_domain": (C.c_int, [C.POINTER(LatticeSpec), C.POINTER(Domain), _d]),
         "mmd_lattice_init": (C.c_int, [C.POINTER(LatticeSpec), C.POINTER(HostSoa), C.c_int32, _i32,
                                        _d, _d, _d, _d, _d, _d, _i8]),
+        "mmd_setfl_info": (C.c_int, [C.c_char_p, _i32, _i32, _i32, _d]),
+        "mmd_setfl_read": (C.c_int, [C.c_char_p, _d, _d, _d, _d, _d, _d, _d, _d]),
         "mmd_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(EamTables), C.POINTER(P)]),
         "mmd_server_destroy": (C.c_int, [P]),
         "mmd_server_run": (C.c_int, [P, C.POINTER(ReplicaSpec), C.c_int32, C.POINTER(ReplicaResult),
@@ -228,6 +231,21 @@ def synth_tables(n_types=1, rc=5.6, A=1.2, B=0.3, Cc=0.01, jitter=0.0, seed=1, n
 
 
 DEFAULT_MASSES = (55.845, 58.693, 63.546)
+
+
+def setfl_read(path):
+    """EAM tables (and masses) from a DYNAMO setfl file, in the form synth_tables returns."""
+    nt, nr, nrho, rc = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+    p = str(path).encode()
+    _check(lib().mmd_setfl_info(p, C.byref(nt), C.byref(nr), C.byref(nrho), C.byref(rc)))
+    nt, nr, nrho = nt.value, nr.value, nrho.value
+    rho_r, emb, phi = np.zeros((nt, nr)), np.zeros((nt, nrho)), np.zeros((nt, nt, nr))
+    mass = np.zeros(nt)
+    vals = [C.c_double() for _ in range(4)]
+    _check(lib().mmd_setfl_read(p, ptr(rho_r), ptr(emb), ptr(phi), ptr(mass), *[C.byref(v) for v in vals]))
+    tab = dict(n_types=nt, n_r=nr, r0=vals[0].value, dr=vals[1].value, n_rho=nrho, rho0=vals[2].value,
+               drho=vals[3].value, rho_r=rho_r, emb=emb, phi_r=phi, rc=rc.value)
+    return tab, tuple(mass)
 
 
 def lattice_spec(cells, ghost_cells=3, a=2.855, n_types=1, type_frac=(0.9, 0.05, 0.05), masses=DEFAULT_MASSES,
