@@ -20,19 +20,23 @@ struct BatchDom {
   double4 *pos;
   double *vx, *vy, *vz, *fx, *fy, *fz, *rho, *df;
   int8_t *type;
-  const int32_t *off;  // [2][kMaxStencil] device-layout stencil
+  const int32_t *off;  // [2][kMaxStencil] stencil LDS offsets (shared by all replicas)
   int32_t nk_even, nk_odd;
-  double *e_rho, *e_force, *e_ke;  // per logical site block
-  double *energy;                  // {potential, kinetic} after the last step, then at admission
+  double *e_rho, *e_force;  // per pair-pass tile block
+  double *e_ke;             // per row-mapped site block
+  double *energy;           // {potential, kinetic} after the last step, then at admission
   double L[3];
-  int64_t n_blocks;  // logical site blocks, as the single-domain launch numbers them (site_blocks)
+  int64_t n_blocks;  // row-mapped site blocks, as the single-domain launch numbers them (site_blocks)
+  int64_t n_tiles;   // pair-pass tile blocks (tile_blocks)
 };
 
 struct BatchLists {
   const BatchDom *dom;
-  const int2 *site_items;  // (replica, logical site block)
+  const int2 *site_items;  // (replica, row-mapped site block): velocity Verlet
   const int *n_site_items;
-  const int2 *ext_items;   // (replica, 256-site ext block)
+  const int2 *tile_items;  // (replica, tile block): rho + df and force
+  const int *n_tile_items;
+  const int2 *ext_items;   // (replica, 256-site ext block): ghost refresh
   const int *n_ext_items;
   const int *reps;         // replica indices for per-replica reductions
   const int *n_reps;
@@ -44,10 +48,11 @@ constexpr int kBatchGrid = 2048;
 hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, const Masses &M, double h, double dt, bool drift,
                              bool want_ke);
 hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, uint32_t fields);
-hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, const TabF64 &tf, const TabMixed &tm, double rc2,
-                               int prec, bool want_e);
-hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, const TabF64 &tf, const TabMixed &tm, double rc2,
-                              int prec, bool want_e);
+hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, const Halo &h, const TabF64 &tf,
+                               const TabMixed &tm, double rc2, int prec, bool want_e);
+hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, const Halo &h, const TabF64 &tf,
+                              const TabMixed &tm, double rc2, int prec, bool want_e);
+hipError_t batch_kernels_allow_lds(size_t bytes);
 // per-replica {potential, kinetic} -> energy[2 * which], which = 0 (step) or 1 (admission)
 hipError_t launch_batch_energy(hipStream_t st, const BatchLists &L, int which);
 

@@ -34,6 +34,8 @@ using namespace mmd;
 
 namespace mmd {
 
+static int64_t floor_div2(int64_t v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); }
+
 Geom make_geom(const mmd_domain &d) {
   Geom g{};
   g.ext_x = d.ext[0], g.ext_y = d.ext[1], g.ext_z = d.ext[2];
@@ -45,10 +47,42 @@ Geom make_geom(const mmd_domain &d) {
   g.n_ytiles = (g.by + g.ty - 1) / g.ty;
   g.n_ext = (int64_t)d.ext[0] * d.ext[1] * d.ext[2];
   g.n_waves = (int64_t)2 * g.n_ytiles * g.ty * g.bz * g.chunks;
+  g.ntx = (g.bxh + kTileX - 1) / kTileX;
+  g.nty = (g.by + kTileY - 1) / kTileY;
+  g.ntz = (g.bz + kTileZ - 1) / kTileZ;
+  g.n_tiles = (int64_t)g.ntx * g.nty * g.ntz;
   return g;
 }
 
-static int64_t floor_div2(int64_t v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); }
+Halo make_halo(const int32_t *st3[2], const int32_t nk[2]) {
+  Halo h{};
+  for (int p = 0; p < 2; p++)
+    for (int32_t k = 0; k < nk[p]; k++) {
+      const int32_t *e = st3[p] + 3 * (size_t)k;
+      h.rx = std::max<int32_t>(h.rx, (int32_t)std::llabs(floor_div2(p + e[0])));
+      h.ry = std::max<int32_t>(h.ry, std::abs(e[1]));
+      h.rz = std::max<int32_t>(h.rz, std::abs(e[2]));
+    }
+  h.lx = kTileX + 2 * h.rx, h.ly = kTileY + 2 * h.ry, h.lz = kTileZ + 2 * h.rz;
+  h.n = h.lx * h.ly * h.lz * 2;
+  return h;
+}
+
+int32_t stencil_lds_offset(int p, int64_t dX, int64_t dY, int64_t dZ, const Halo &h) {
+  const int64_t s1 = (p + dX) & 1;
+  return (int32_t)(((dZ * h.ly + dY) * 2 + (s1 - p)) * h.lx + floor_div2(p + dX));
+}
+
+int check_tile_lds(int device, const Halo &h) {
+  const size_t need = tile_lds_bytes(h, true);
+  int lim = 0;
+  HIP_TRY(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  if (need > (size_t)lim)
+    return fail(MMD_E_RANGE, "stencil reach (%d, %d, %d) needs %zu bytes of LDS per tile, the device allows %d",
+                h.rx, h.ry, h.rz, need, lim);
+  if (need > 65536) HIP_TRY(pair_kernels_allow_lds(need));
+  return MMD_OK;
+}
 
 int32_t stencil_dev_offset(int p, int64_t dX, int64_t dY, int64_t dZ, const mmd_domain &d) {
   const int64_t s1 = (p + dX) & 1;
@@ -146,7 +180,7 @@ int run_rho_df(mmd_ctx *c, double cutoff, bool want_e) {
   hipEvent_t ev;
   int rc = timed_begin(c, MMD_K_RHO, &ev);
   if (rc) return rc;
-  HIP_TRY(launch_rho_df(c->stream, c->g, c->d_off, c->nk_even, c->nk_odd, c->pos, c->rho, c->df, c->tf, c->tm,
+  HIP_TRY(launch_rho_df(c->stream, c->g, c->halo, c->d_off, c->nk_even, c->nk_odd, c->pos, c->rho, c->df, c->tf, c->tm,
                         cutoff * cutoff, c->prec, want_e ? c->e_rho : nullptr));
   rc = timed_end(c, MMD_K_RHO, ev);
   if (rc) return rc;
@@ -158,7 +192,7 @@ int run_force(mmd_ctx *c, double cutoff, bool want_e) {
   hipEvent_t ev;
   int rc = timed_begin(c, MMD_K_FORCE, &ev);
   if (rc) return rc;
-  HIP_TRY(launch_force(c->stream, c->g, c->d_off, c->nk_even, c->nk_odd, c->pos, c->df, c->fx, c->fy, c->fz, c->tf,
+  HIP_TRY(launch_force(c->stream, c->g, c->halo, c->d_off, c->nk_even, c->nk_odd, c->pos, c->df, c->fx, c->fy, c->fz, c->tf,
                        c->tm, cutoff * cutoff, c->prec, want_e ? c->e_force : nullptr));
   rc = timed_end(c, MMD_K_FORCE, ev);
   if (rc) return rc;
@@ -188,9 +222,9 @@ static int run_kick(mmd_ctx *c, double h, bool drift, bool want_ke) {
 }
 
 static int run_finalize(mmd_ctx *c, double *out, int32_t *counter) {
-  const int64_t nb = site_blocks(c->g);
+  const int64_t nt = tile_blocks(c->g), nb = site_blocks(c->g);
   const bool itl = c->n_itl > 0;
-  HIP_TRY(launch_energy_finalize(c->stream, c->e_rho, nb, c->e_force, nb, itl ? c->e_itl : nullptr,
+  HIP_TRY(launch_energy_finalize(c->stream, c->e_rho, nt, c->e_force, nt, itl ? c->e_itl : nullptr,
                                  itl ? c->itl_part_n : 0, c->e_ke, nb, itl ? c->e_itl + c->itl_part_n : nullptr,
                                  itl ? c->itl_kick_n : 0, out, counter));
   return MMD_OK;
@@ -323,8 +357,8 @@ extern "C" int mmd_set_domain(mmd_ctx *c, const mmd_domain *d) {
     }
     HIP_TRY(launch_fill_i8(c->stream, c->type, (int64_t)n, (int8_t)-1));
     HIP_TRY(launch_ref2dev_pos(c->stream, c->g, nullptr, nullptr, nullptr, c->type, c->pos, false));
-    const size_t nb = (size_t)site_blocks(g);
-    if ((rc = dalloc(&c->e_rho, nb)) || (rc = dalloc(&c->e_force, nb)) || (rc = dalloc(&c->e_ke, nb))) {
+    const size_t nb = (size_t)site_blocks(g), nt = (size_t)tile_blocks(g);
+    if ((rc = dalloc(&c->e_rho, nt)) || (rc = dalloc(&c->e_force, nt)) || (rc = dalloc(&c->e_ke, nb))) {
       free_state(c);
       return rc;
     }
@@ -351,10 +385,12 @@ extern "C" int mmd_set_stencil(mmd_ctx *c, const int64_t *even, int32_t n_even, 
     if (n_even < 0 || n_odd < 0 || n_even > cap || n_odd > cap || (n_even && !even) || (n_odd && !odd))
       return fail(MMD_E_RANGE, "mmd_set_stencil: sizes must be in [0, %d]", cap);
     const mmd_domain &d = c->dom;
-    std::vector<int32_t> host(2 * kMaxStencil, 0);
+    // decompose the reference linear offsets into (dX doubled, dY, dZ)
+    std::vector<int32_t> e3[2];
     for (int p = 0; p < 2; p++) {
       const int64_t *src = p ? odd : even;
       const int32_t n = p ? n_odd : n_even;
+      e3[p].resize(3 * (size_t)n);
       for (int32_t k = 0; k < n; k++) {
         const int64_t o = src[k];
         const int64_t dX = centred_mod(o, d.ext[0]);
@@ -365,9 +401,20 @@ extern "C" int mmd_set_stencil(mmd_ctx *c, const int64_t *even, int32_t n_even, 
             (!dX && !dY && !dZ))
           return fail(MMD_E_RANGE, "mmd_set_stencil: parity %d entry %d (offset %lld) leaves the ghost shell", p, k,
                       (long long)o);
-        host[(size_t)p * kMaxStencil + k] = stencil_dev_offset(p, dX, dY, dZ, d);
+        e3[p][3 * (size_t)k] = (int32_t)dX, e3[p][3 * (size_t)k + 1] = (int32_t)dY, e3[p][3 * (size_t)k + 2] = (int32_t)dZ;
       }
     }
+    // the pair passes address neighbours inside the LDS tile
+    const int32_t *st3[2] = {e3[0].data(), e3[1].data()};
+    const int32_t nk[2] = {n_even, n_odd};
+    const Halo h = make_halo(st3, nk);
+    if ((rc = check_tile_lds(c->device, h))) return rc;
+    std::vector<int32_t> host(2 * kMaxStencil, 0);
+    for (int p = 0; p < 2; p++)
+      for (int32_t k = 0; k < nk[p]; k++)
+        host[(size_t)p * kMaxStencil + k] =
+            stencil_lds_offset(p, st3[p][3 * (size_t)k], st3[p][3 * (size_t)k + 1], st3[p][3 * (size_t)k + 2], h);
+    c->halo = h;
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (!c->d_off && (rc = dalloc(&c->d_off, 2 * kMaxStencil))) return rc;
     HIP_TRY(hipMemcpy(c->d_off, host.data(), host.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -530,7 +577,7 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
     int rc = ctx_bind(c);
     if (rc) return rc;
     if (!t || t->n_types < 1 || t->n_types > kMaxTypes || t->n_r < 5 || t->n_rho < 5 || !(t->dr > 0.0) ||
-        !(t->drho > 0.0) || !(t->r0 > 0.0) || !t->rho_r || !t->emb || !t->phi_r || !t->mass)
+        !(t->drho > 0.0) || !(t->r0 >= 0.0) || !t->rho_r || !t->emb || !t->phi_r || !t->mass)
       return fail(MMD_E_INVALID, "mmd_set_tables: bad table descriptor");
     const int nt = t->n_types, nr = t->n_r, nrho = t->n_rho;
     for (int a = 0; a < nt; a++) {

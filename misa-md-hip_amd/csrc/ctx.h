@@ -26,9 +26,10 @@ struct mmd_ctx {
   int8_t *type = nullptr;
   double *stage = nullptr;  // 3 * n_ext doubles: reference-layout staging for upload/download
 
-  // stencil: [2][kMaxStencil] device-layout offsets
+  // stencil: [2][kMaxStencil] LDS offsets within the pair-pass tile, and its halo
   int32_t *d_off = nullptr;
   int32_t nk_even = 0, nk_odd = 0;
+  mmd::Halo halo{};
 
   // tables
   double *d_tab = nullptr, *d_mixed = nullptr;
@@ -87,6 +88,12 @@ namespace mmd {
 Geom make_geom(const mmd_domain &d);
 // device-layout offset of stencil entry (dX doubled, dY, dZ) for centre parity p
 int32_t stencil_dev_offset(int p, int64_t dX, int64_t dY, int64_t dZ, const mmd_domain &d);
+// stencil reach and LDS tile of stencils st3[p] (nk[p] entries (dX doubled, dY, dZ) per parity)
+Halo make_halo(const int32_t *st3[2], const int32_t nk[2]);
+// LDS offset of stencil entry (dX doubled, dY, dZ) for centre parity p within a tile of halo h
+int32_t stencil_lds_offset(int p, int64_t dX, int64_t dY, int64_t dZ, const Halo &h);
+// fails with MMD_E_RANGE when the tile of h does not fit the device's LDS
+int check_tile_lds(int device, const Halo &h);
 
 int ctx_bind(mmd_ctx *c);
 void ctx_drop_graphs(mmd_ctx *c);

@@ -24,6 +24,10 @@ constexpr int kMaxStencil = 512;  // entries per parity
 constexpr int kMaxTypes = 8;
 constexpr int kXcds = 8;          // MI355X: 8 accelerator complex dies, blocks dispatched round-robin
 
+// Pair-pass tiles: a block of 256 threads computes kTileX x kTileY x kTileZ
+// cells (both sub-lattices) after staging the tile plus its stencil halo in LDS.
+constexpr int kTileX = 8, kTileY = 4, kTileZ = 4;
+
 struct Geom {
   int32_t ext_x, ext_y, ext_z;  // ext_x in doubled units
   int32_t exh;                  // ext_x / 2 (cells per row)
@@ -33,7 +37,22 @@ struct Geom {
   int32_t ty, n_ytiles;         // y rows per tile, tiles covering the local box
   int64_t n_ext;                // ext sites
   int64_t n_waves;              // waves covering the local box (tiles padded to ty rows)
+  int32_t ntx, nty, ntz;        // pair-pass tiles along x, y, z
+  int64_t n_tiles;
 };
+
+// Stencil reach (cells along x, rows along y and z) and the LDS tile it
+// implies: lx x ly x lz cells of both parities, n records.
+struct Halo {
+  int32_t rx, ry, rz;
+  int32_t lx, ly, lz;
+  int32_t n;
+};
+
+// LDS bytes of a pair-pass block: (x, y), (z, type) [+ df] per staged record
+__host__ __device__ inline size_t tile_lds_bytes(const Halo &h, bool with_df) {
+  return (size_t)h.n * (with_df ? 40 : 32);
+}
 
 // Loads through a uniform base pointer with a 32-bit element index, so the
 // address is an SGPR base + 32-bit VGPR offset (no 64-bit VALU arithmetic).
@@ -185,6 +204,48 @@ __device__ __forceinline__ bool map_site_lb(const Geom &g, int64_t lb, Site &st)
 }
 
 __device__ __forceinline__ bool map_site(const Geom &g, Site &st) { return map_site_lb(g, xcd_block(), st); }
+
+// Pair-pass tile lb: wave w of the block takes parity w & 1 and z-half w >> 1,
+// its 64 lanes 8 cells along x x 4 rows along y x 2 planes along z, so the
+// stencil row (parity) is wave-uniform.
+struct TileSite {
+  int32_t x0, y0, z0;  // tile origin, local cells / rows / planes
+  int32_t tx, ty, tz;  // this thread's cell in the tile
+  int s;               // parity (wave-uniform)
+  bool active;         // inside the local box
+  int64_t i;           // device index
+  int32_t c;           // LDS record of this site
+};
+
+__device__ __forceinline__ TileSite map_tile(const Geom &g, const Halo &h, int64_t lb) {
+  TileSite t;
+  const int32_t txi = (int32_t)(lb % g.ntx);
+  const int64_t r = lb / g.ntx;
+  const int32_t tyi = (int32_t)(r % g.nty), tzi = (int32_t)(r / g.nty);
+  t.x0 = txi * kTileX, t.y0 = tyi * kTileY, t.z0 = tzi * kTileZ;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+  t.s = __builtin_amdgcn_readfirstlane(w & 1);
+  t.tx = lane & 7;
+  t.ty = (lane >> 3) & 3;
+  t.tz = (w >> 1) * 2 + (lane >> 5);
+  t.active = lb < g.n_tiles && t.x0 + t.tx < g.bxh && t.y0 + t.ty < g.by && t.z0 + t.tz < g.bz;
+  t.i = dev_index(g, t.s, t.x0 + t.tx + g.gxh, t.y0 + t.ty + g.gy, t.z0 + t.tz + g.gz);
+  t.c = (((t.tz + h.rz) * h.ly + (t.ty + h.ry)) * 2 + t.s) * h.lx + (t.tx + h.rx);
+  return t;
+}
+
+// LDS record idx -> device index of the ext site it mirrors, or -1 outside
+// the ghost-extended lattice.
+__device__ __forceinline__ int64_t tile_record_src(const Geom &g, const Halo &h, const TileSite &t, int32_t idx) {
+  const int32_t lxh = idx % h.lx;
+  const int32_t r = idx / h.lx;
+  const int32_t s = r & 1;
+  const int32_t yz = r >> 1;
+  const int32_t ly = yz % h.ly, lz = yz / h.ly;
+  const int32_t xh = t.x0 + lxh - h.rx + g.gxh, ye = t.y0 + ly - h.ry + g.gy, ze = t.z0 + lz - h.rz + g.gz;
+  if (xh < 0 || xh >= g.exh || ye < 0 || ye >= g.ext_y || ze < 0 || ze >= g.ext_z) return -1;
+  return dev_index(g, s, xh, ye, ze);
+}
 
 }  // namespace mmd
 

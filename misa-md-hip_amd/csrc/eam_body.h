@@ -19,6 +19,8 @@ namespace mmd {
 #define MMD_PAIR_UNROLL 4
 #endif
 constexpr int kUnroll = MMD_PAIR_UNROLL;  // stencil entries whose loads are in flight together
+// stencil rows are read up to index nk + kUnroll - 2 (rows are zero-padded to kMaxStencil)
+static_assert(kUnroll <= 8, "stencil padding too short for the unroll");
 
 // 1/sqrt(d2) for d2 > 0: hardware reciprocal-square-root seed refined by two
 // Newton steps to full FP64 accuracy (a few ulp), instead of the IEEE sqrt and
@@ -65,79 +67,127 @@ __device__ __forceinline__ SegS seg_s(double s2, const TabMixed &t) {
 }
 
 // ------------------------------------------------------------- pair terms
+//
+// The pair functions are branch-free so the table loads of all kUnroll pairs
+// of a trip are issued together: control flow around a pair's loads makes the
+// compiler wait for them before the next pair's loads are issued.
+//   * rho'_{ti} of unlike pairs is loaded through a selected index (lanes with
+//     like pairs read element 0, one cache line for the whole wave);
+//   * the mixed path evaluates every lane with the r^2-keyed tables (index
+//     clamped into the table) and reports whether the pair lies in the range
+//     where they meet the accuracy bound; the few pairs outside it are redone
+//     with the FP64 r-spline in a separate, wave-uniformly guarded pass.
 
-// rho_{tj}(r) for r^2 = d2
-template <int PREC>
-__device__ __forceinline__ double rho_pair(const TabF64 &tf, const TabMixed &tm, int tj, double d2) {
-  if (PREC == 1 && d2 < tm.s_tail && d2 >= tm.s_head) {
-    const SegS sg = seg_s(d2, tm);
-    return mix_value(tm.rho, (uint32_t)(tj * tm.n_s + sg.m), sg.q);
-  }
+// rho_{tj}(r), r^2 = d2, FP64 r-keyed spline
+__device__ __forceinline__ double rho_f64(const TabF64 &tf, int tj, double d2) {
   const double r = d2 * rsqrt64(d2);
   const Seg sg = seg_f64(r, tf.r0, tf.inv_dr, tf.n_r);
   return spl_value(tf.rho, (uint32_t)(tj * tf.n_r + sg.m), sg.p);
 }
 
+// rho_{tj}(r) from the r^2-keyed table; ok = pair inside the table's checked range
+__device__ __forceinline__ double rho_mixed(const TabMixed &tm, int tj, double d2, bool &ok) {
+  ok = d2 < tm.s_tail && d2 >= tm.s_head;
+  const SegS sg = seg_s(d2, tm);
+  return mix_value(tm.rho, (uint32_t)(tj * tm.n_s + sg.m), sg.q);
+}
+
 // fpair = -(df_i rho'_{tj} + df_j rho'_{ti} + phi'_{ti tj}) / r (the scalar that
 // multiplies del = x_i - x_j); phi receives phi_{ti tj}(r) when WANT_E.
-// rho'_{tj} and phi'_{ti tj} come from one combined per-type-pair record;
-// rho'_{ti} is gathered separately, by unlike-type pairs only.
-template <int PREC, bool WANT_E>
-__device__ __forceinline__ double fpair(const TabF64 &tf, const TabMixed &tm, int ti, int tj, double d2, double dfi,
-                                        double dfj, double &phi) {
-  const int tp = ti * tf.nt + tj;
-  if (PREC == 1 && d2 < tm.s_tail && d2 >= tm.s_head) {
-    // r^2-keyed: d/ds tables, -(dpsi/dr)/r = -2 dpsi/ds
-    const SegS sg = seg_s(d2, tm);
-    const uint32_t rp = (uint32_t)(tp * tm.n_s + sg.m);
-    const double2 b0 = ld(tm.fb, 2 * rp), b1 = ld(tm.fb, 2 * rp + 1);
-    const float2 q2 = ld(tm.fq, rp);
-    const double q = sg.q;
-    const double rhojp = fma(q, fma(q, (double)q2.x, b0.y), b0.x);
-    const double phip = fma(q, fma(q, (double)q2.y, b1.y), b1.x);
-    double rhoip = rhojp;
-    if (ti != tj) rhoip = mix_deriv(tm.rho, (uint32_t)(ti * tm.n_s + sg.m), q);
-    if (WANT_E) phi = mix_value(tm.phi, rp, q);
-    return -2.0 * (dfi * rhojp + dfj * rhoip + phip);
-  }
+// rho'_{tj} and phi'_{ti tj} come from one combined per-type-pair record.
+template <bool WANT_E>
+__device__ __forceinline__ double fpair_f64(const TabF64 &tf, int ti, int tj, double d2, double dfi, double dfj,
+                                            double &phi) {
   const double rinv = rsqrt64(d2);
   const double r = d2 * rinv;
   const Seg sg = seg_f64(r, tf.r0, tf.inv_dr, tf.n_r);
-  const uint32_t rp = (uint32_t)(tp * tf.n_r + sg.m);
+  const uint32_t rp = (uint32_t)((ti * tf.nt + tj) * tf.n_r + sg.m);
+  const bool unlike = ti != tj;
+  const uint32_t ra = unlike ? (uint32_t)(ti * tf.n_r + sg.m) : 0u;
   const double2 c0 = ld(tf.fc, 3 * rp), c1 = ld(tf.fc, 3 * rp + 1), c2 = ld(tf.fc, 3 * rp + 2);
+  const double2 ua = ld(tf.rho.d01, ra);
+  const double uc = ld(tf.rho.d2, ra);
   const double p = sg.p;
   const double rhojp = (c0.x * p + c0.y) * p + c1.x;
   const double phip = (c1.y * p + c2.x) * p + c2.y;
-  double rhoip = rhojp;
-  if (ti != tj) rhoip = spl_deriv(tf.rho, (uint32_t)(ti * tf.n_r + sg.m), p);
+  const double rhoip = unlike ? (ua.x * p + ua.y) * p + uc : rhojp;
   if (WANT_E) phi = spl_value(tf.phi, rp, p);
   return -(dfi * rhojp + dfj * rhoip + phip) * rinv;
 }
 
+// Mixed path: d/ds tables, -(dpsi/dr)/r = -2 dpsi/ds, no square root or divide.
+template <bool WANT_E>
+__device__ __forceinline__ double fpair_mixed(const TabMixed &tm, int ti, int tj, double d2, double dfi, double dfj,
+                                              double &phi, bool &ok) {
+  ok = d2 < tm.s_tail && d2 >= tm.s_head;
+  const SegS sg = seg_s(d2, tm);
+  const uint32_t rp = (uint32_t)((ti * tm.nt + tj) * tm.n_s + sg.m);
+  const bool unlike = ti != tj;
+  const uint32_t ra = unlike ? (uint32_t)(ti * tm.n_s + sg.m) : 0u;
+  const double2 b0 = ld(tm.fb, 2 * rp), b1 = ld(tm.fb, 2 * rp + 1);
+  const float2 q2 = ld(tm.fq, rp);
+  const double2 ua = ld(tm.rho.db, ra);
+  const float uc = ld(tm.rho.dc, ra);
+  const double q = sg.q;
+  const double rhojp = fma(q, fma(q, (double)q2.x, b0.y), b0.x);
+  const double phip = fma(q, fma(q, (double)q2.y, b1.y), b1.x);
+  const double rhoip = unlike ? fma(q, fma(q, (double)uc, ua.y), ua.x) : rhojp;
+  if (WANT_E) phi = mix_value(tm.phi, rp, q);
+  return -2.0 * (dfi * rhojp + dfj * rhoip + phip);
+}
+
 // --------------------------------------------------------------- kernels
 //
-// One thread per local site, kUnroll stencil entries per trip: all neighbour
-// loads of a trip are issued before any is used, pairs are evaluated without
-// per-pair branches (out-of-cutoff and vacant neighbours are masked), and a
-// trip whose pairs are all outside the cutoff for the whole wave is skipped.
-// The stencil rows are padded with zero offsets past nk.  A neighbour is one
-// 32-byte (x, y, z, type) record: two 16-byte loads.
+// One block per tile of kTileX x kTileY x kTileZ cells (both sub-lattices),
+// one thread per site.  The block first stages the tile plus its stencil halo
+// in LDS — (x, y) and (z, type) of every record as two 16-byte arrays, and df
+// for the force pass — with coalesced row loads, so a neighbour costs LDS
+// reads instead of vector-memory instructions; only the spline tables are
+// gathered from memory.  Stencil entries are LDS offsets (wave-uniform: a wave
+// holds one parity), kUnroll per trip with the tables' loads of a trip issued
+// together; pairs are masked without per-pair branches and a trip with no
+// in-cutoff pair in the whole wave is skipped.  Records outside the
+// ghost-extended lattice are staged as vacancies.  The body starts with a
+// barrier so a persistent block can restage for its next tile.
+
+template <bool WITH_DF>
+__device__ __forceinline__ void stage_tile(const Geom &g, const Halo &h, const TileSite &t, int64_t lb,
+                                           const double4 *__restrict__ P, const double *__restrict__ DF,
+                                           double2 *lxy, double2 *lzt, double *ldf) {
+  __syncthreads();
+  if (lb < g.n_tiles) {
+    for (int32_t idx = threadIdx.x; idx < h.n; idx += kBlock) {
+      const int64_t src = tile_record_src(g, h, t, idx);
+      double4 q = make_double4(0.0, 0.0, 0.0, -1.0);
+      double f = 0.0;
+      if (src >= 0) {
+        q = ld(P, (uint32_t)src);
+        if (WITH_DF) f = ld(DF, (uint32_t)src);
+      }
+      lxy[idx] = make_double2(q.x, q.y);
+      lzt[idx] = make_double2(q.z, q.w);
+      if (WITH_DF) ldf[idx] = f;
+    }
+  }
+  __syncthreads();
+}
 
 template <int PREC, bool WANT_E>
-__device__ __forceinline__ void rho_df_block(const Geom &g, int64_t lb, const int32_t *__restrict__ off,
-                                             int32_t nk_even, int32_t nk_odd, const double4 *__restrict__ P,
-                                             double *__restrict__ RHO, double *__restrict__ DF, const TabF64 &tf,
-                                             const TabMixed &tm, double rc2, double *__restrict__ e_part,
-                                             double *lds) {
+__device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64_t lb,
+                                             const int32_t *__restrict__ off, int32_t nk_even, int32_t nk_odd,
+                                             const double4 *__restrict__ P, double *__restrict__ RHO,
+                                             double *__restrict__ DF, const TabF64 &tf, const TabMixed &tm, double rc2,
+                                             double *__restrict__ e_part, double2 *lds, double *red) {
+  const TileSite t = map_tile(g, h, lb);
+  double2 *lxy = lds, *lzt = lds + h.n;
+  stage_tile<false>(g, h, t, lb, P, nullptr, lxy, lzt, nullptr);
   double e_loc = 0.0;
-  Site st;
-  if (map_site_lb(g, lb, st) && st.active) {
-    const uint32_t i = (uint32_t)st.i;
-    const double4 pi = ld(P, i);
-    const int t0 = (int)pi.w;
+  if (t.active) {
+    const double2 cxy = lxy[t.c], czt = lzt[t.c];
+    const int t0 = (int)czt.y;
     if (t0 >= 0) {
-      const int32_t *o = off + st.s * kMaxStencil;
-      const int nk = st.s ? nk_odd : nk_even;
+      const int32_t *o = off + t.s * kMaxStencil;
+      const int nk = t.s ? nk_odd : nk_even;
       const double d2_safe = 0.25 * rc2;
       double acc = 0.0;
       for (int k = 0; k < nk; k += kUnroll) {
@@ -145,9 +195,10 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, int64_t lb, const in
         double d2[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
-          const double4 pj = ld(P, i + (uint32_t)o[k + u]);
-          tj[u] = (int)pj.w;
-          const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+          const int32_t j = t.c + o[k + u];
+          const double2 pxy = lxy[j], pzt = lzt[j];
+          tj[u] = (int)pzt.y;
+          const double dx = cxy.x - pxy.x, dy = cxy.y - pxy.y, dz = czt.x - pzt.x;
           d2[u] = dx * dx + dy * dy + dz * dz;
         }
         bool any = false;
@@ -158,13 +209,30 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, int64_t lb, const in
           any |= in;
         }
         if (__ballot(any) == 0) continue;
+        double v[kUnroll];
+        bool redo = false;
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
           const bool in = d2[u] >= 0.0;
-          const double v = rho_pair<PREC>(tf, tm, in ? tj[u] : 0, in ? d2[u] : d2_safe);
-          acc += in ? v : 0.0;
+          if (PREC == 1) {
+            bool ok;
+            v[u] = rho_mixed(tm, in ? tj[u] : 0, in ? d2[u] : d2_safe, ok);
+            redo |= in && !ok;
+          } else {
+            v[u] = rho_f64(tf, in ? tj[u] : 0, in ? d2[u] : d2_safe);
+          }
         }
+        if (PREC == 1 && __ballot(redo) != 0) {
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) {
+            const bool in = d2[u] >= 0.0;
+            if (in && !(d2[u] < tm.s_tail && d2[u] >= tm.s_head)) v[u] = rho_f64(tf, tj[u], d2[u]);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) acc += d2[u] >= 0.0 ? v[u] : 0.0;
       }
+      const uint32_t i = (uint32_t)t.i;
       RHO[i] = acc;
       const Seg se = seg_f64(acc, tf.rho0, tf.inv_drho, tf.n_rho);
       const uint32_t re = (uint32_t)(t0 * tf.n_rho + se.m);
@@ -173,28 +241,30 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, int64_t lb, const in
     }
   }
   if (WANT_E) {
-    const double b = block_sum(e_loc, lds);
+    const double b = block_sum(e_loc, red);
     if (threadIdx.x == 0) e_part[lb] = b;
   }
 }
 
 template <int PREC, bool WANT_E>
-__device__ __forceinline__ void force_block(const Geom &g, int64_t lb, const int32_t *__restrict__ off,
+__device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_t lb, const int32_t *__restrict__ off,
                                             int32_t nk_even, int32_t nk_odd, const double4 *__restrict__ P,
                                             const double *__restrict__ DF, double *__restrict__ FX,
                                             double *__restrict__ FY, double *__restrict__ FZ, const TabF64 &tf,
-                                            const TabMixed &tm, double rc2, double *__restrict__ e_part,
-                                            double *lds) {
+                                            const TabMixed &tm, double rc2, double *__restrict__ e_part, double2 *lds,
+                                            double *red) {
+  const TileSite t = map_tile(g, h, lb);
+  double2 *lxy = lds, *lzt = lds + h.n;
+  double *ldf = reinterpret_cast<double *>(lds + 2 * h.n);
+  stage_tile<true>(g, h, t, lb, P, DF, lxy, lzt, ldf);
   double e_loc = 0.0;
-  Site st;
-  if (map_site_lb(g, lb, st) && st.active) {
-    const uint32_t i = (uint32_t)st.i;
-    const double4 pi = ld(P, i);
-    const int t0 = (int)pi.w;
+  if (t.active) {
+    const double2 cxy = lxy[t.c], czt = lzt[t.c];
+    const int t0 = (int)czt.y;
     if (t0 >= 0) {
-      const int32_t *o = off + st.s * kMaxStencil;
-      const int nk = st.s ? nk_odd : nk_even;
-      const double dfi = ld(DF, i);
+      const int32_t *o = off + t.s * kMaxStencil;
+      const int nk = t.s ? nk_odd : nk_even;
+      const double dfi = ldf[t.c];
       const double d2_safe = 0.25 * rc2;
       double ax = 0.0, ay = 0.0, az = 0.0, ep = 0.0;
       for (int k = 0; k < nk; k += kUnroll) {
@@ -202,13 +272,13 @@ __device__ __forceinline__ void force_block(const Geom &g, int64_t lb, const int
         double dx[kUnroll], dy[kUnroll], dz[kUnroll], d2[kUnroll], dfj[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
-          const uint32_t j = i + (uint32_t)o[k + u];
-          const double4 pj = ld(P, j);
-          tj[u] = (int)pj.w;
-          dx[u] = pi.x - pj.x;
-          dy[u] = pi.y - pj.y;
-          dz[u] = pi.z - pj.z;
-          dfj[u] = ld(DF, j);
+          const int32_t j = t.c + o[k + u];
+          const double2 pxy = lxy[j], pzt = lzt[j];
+          tj[u] = (int)pzt.y;
+          dx[u] = cxy.x - pxy.x;
+          dy[u] = cxy.y - pxy.y;
+          dz[u] = czt.x - pzt.x;
+          dfj[u] = ldf[j];
           d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
         }
         bool any = false;
@@ -219,18 +289,39 @@ __device__ __forceinline__ void force_block(const Geom &g, int64_t lb, const int
           any |= in;
         }
         if (__ballot(any) == 0) continue;
+        double fp[kUnroll], phi[kUnroll];
+        bool redo = false;
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
           const bool in = d2[u] >= 0.0;
-          double phi = 0.0;
-          double fp = fpair<PREC, WANT_E>(tf, tm, t0, in ? tj[u] : t0, in ? d2[u] : d2_safe, dfi, dfj[u], phi);
-          fp = in ? fp : 0.0;
-          ax = fma(dx[u], fp, ax);
-          ay = fma(dy[u], fp, ay);
-          az = fma(dz[u], fp, az);
-          if (WANT_E) ep += in ? phi : 0.0;
+          phi[u] = 0.0;
+          if (PREC == 1) {
+            bool ok;
+            fp[u] = fpair_mixed<WANT_E>(tm, t0, in ? tj[u] : t0, in ? d2[u] : d2_safe, dfi, dfj[u], phi[u], ok);
+            redo |= in && !ok;
+          } else {
+            fp[u] = fpair_f64<WANT_E>(tf, t0, in ? tj[u] : t0, in ? d2[u] : d2_safe, dfi, dfj[u], phi[u]);
+          }
+        }
+        if (PREC == 1 && __ballot(redo) != 0) {
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) {
+            const bool in = d2[u] >= 0.0;
+            if (in && !(d2[u] < tm.s_tail && d2[u] >= tm.s_head))
+              fp[u] = fpair_f64<WANT_E>(tf, t0, tj[u], d2[u], dfi, dfj[u], phi[u]);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+          const bool in = d2[u] >= 0.0;
+          const double f = in ? fp[u] : 0.0;
+          ax = fma(dx[u], f, ax);
+          ay = fma(dy[u], f, ay);
+          az = fma(dz[u], f, az);
+          if (WANT_E) ep += in ? phi[u] : 0.0;
         }
       }
+      const uint32_t i = (uint32_t)t.i;
       FX[i] = ax;
       FY[i] = ay;
       FZ[i] = az;
@@ -238,7 +329,7 @@ __device__ __forceinline__ void force_block(const Geom &g, int64_t lb, const int
     }
   }
   if (WANT_E) {
-    const double b = block_sum(e_loc, lds);
+    const double b = block_sum(e_loc, red);
     if (threadIdx.x == 0) e_part[lb] = b;
   }
 }

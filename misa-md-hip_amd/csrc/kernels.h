@@ -8,17 +8,20 @@
 
 namespace mmd {
 
+// row-mapped site blocks (streaming passes), pair-pass tile blocks; both multiples of the 8 XCDs
 int64_t site_blocks(const Geom &g);
+int64_t tile_blocks(const Geom &g);
 
-// EAM passes
-hipError_t launch_rho_df(hipStream_t st, const Geom &g, const int32_t *off, int32_t nk_even, int32_t nk_odd,
-                         const double4 *P, double *RHO, double *DF, const TabF64 &tf, const TabMixed &tm, double rc2,
-                         int prec, double *e_part);
+// EAM passes; off = [2][kMaxStencil] LDS offsets of the stencil within the tile (stencil_lds_offsets)
+hipError_t launch_rho_df(hipStream_t st, const Geom &g, const Halo &h, const int32_t *off, int32_t nk_even,
+                         int32_t nk_odd, const double4 *P, double *RHO, double *DF, const TabF64 &tf,
+                         const TabMixed &tm, double rc2, int prec, double *e_part);
 hipError_t launch_df(hipStream_t st, const Geom &g, const int8_t *T, const double *RHO, double *DF,
                      const TabF64 &tf);
-hipError_t launch_force(hipStream_t st, const Geom &g, const int32_t *off, int32_t nk_even, int32_t nk_odd,
-                        const double4 *P, const double *DF, double *FX, double *FY, double *FZ, const TabF64 &tf,
-                        const TabMixed &tm, double rc2, int prec, double *e_part);
+hipError_t launch_force(hipStream_t st, const Geom &g, const Halo &h, const int32_t *off, int32_t nk_even,
+                        int32_t nk_odd, const double4 *P, const double *DF, double *FX, double *FY, double *FZ,
+                        const TabF64 &tf, const TabMixed &tm, double rc2, int prec, double *e_part);
+hipError_t pair_kernels_allow_lds(size_t bytes);
 
 // layout conversion between the reference layout (host view) and the device layout;
 // positions: three staged reference-layout arrays <-> (x, y, z, type) records

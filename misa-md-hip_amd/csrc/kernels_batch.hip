@@ -45,25 +45,28 @@ __global__ __launch_bounds__(kBlock) void kb_ghost(BatchLists L, uint32_t fields
 }
 
 template <int PREC, bool WANT_E>
-__global__ __launch_bounds__(kBlock) void kb_rho_df(BatchLists L, TabF64 tf, TabMixed tm, double rc2) {
-  __shared__ double lds[kWavesPerBlock];
-  const Range r = xcd_range(*L.n_site_items);
+__global__ __launch_bounds__(kBlock) void kb_rho_df(BatchLists L, Halo h, TabF64 tf, TabMixed tm, double rc2) {
+  extern __shared__ double2 lds_tile[];
+  __shared__ double red[kWavesPerBlock];
+  const Range r = xcd_range(*L.n_tile_items);
   for (int it = r.begin; it < r.end; it += r.stride) {
-    const int2 w = L.site_items[it];
+    const int2 w = L.tile_items[it];
     const BatchDom &d = L.dom[w.x];
-    rho_df_block<PREC, WANT_E>(d.g, w.y, d.off, d.nk_even, d.nk_odd, d.pos, d.rho, d.df, tf, tm, rc2, d.e_rho, lds);
+    rho_df_block<PREC, WANT_E>(d.g, h, w.y, d.off, d.nk_even, d.nk_odd, d.pos, d.rho, d.df, tf, tm, rc2, d.e_rho,
+                               lds_tile, red);
   }
 }
 
 template <int PREC, bool WANT_E>
-__global__ __launch_bounds__(kBlock) void kb_force(BatchLists L, TabF64 tf, TabMixed tm, double rc2) {
-  __shared__ double lds[kWavesPerBlock];
-  const Range r = xcd_range(*L.n_site_items);
+__global__ __launch_bounds__(kBlock) void kb_force(BatchLists L, Halo h, TabF64 tf, TabMixed tm, double rc2) {
+  extern __shared__ double2 lds_tile[];
+  __shared__ double red[kWavesPerBlock];
+  const Range r = xcd_range(*L.n_tile_items);
   for (int it = r.begin; it < r.end; it += r.stride) {
-    const int2 w = L.site_items[it];
+    const int2 w = L.tile_items[it];
     const BatchDom &d = L.dom[w.x];
-    force_block<PREC, WANT_E>(d.g, w.y, d.off, d.nk_even, d.nk_odd, d.pos, d.df, d.fx, d.fy, d.fz, tf, tm, rc2,
-                              d.e_force, lds);
+    force_block<PREC, WANT_E>(d.g, h, w.y, d.off, d.nk_even, d.nk_odd, d.pos, d.df, d.fx, d.fy, d.fz, tf, tm, rc2,
+                              d.e_force, lds_tile, red);
   }
 }
 
@@ -80,8 +83,8 @@ __global__ __launch_bounds__(kBlock) void kb_energy(BatchLists L, int which) {
   const int n = *L.n_reps;
   for (int q = blockIdx.x; q < n; q += gridDim.x) {
     const BatchDom &d = L.dom[L.reps[q]];
-    double pe = ordered_sum_b(d.e_rho, d.n_blocks);
-    pe += ordered_sum_b(d.e_force, d.n_blocks);
+    double pe = ordered_sum_b(d.e_rho, d.n_tiles);
+    pe += ordered_sum_b(d.e_force, d.n_tiles);
     const double kin = ordered_sum_b(d.e_ke, d.n_blocks);
     const double PE = block_sum(pe, lds);
     const double KE = block_sum(kin, lds);
@@ -116,38 +119,52 @@ hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, uint32_t fiel
   return hipGetLastError();
 }
 
-hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, const TabF64 &tf, const TabMixed &tm, double rc2,
-                               int prec, bool want_e) {
+hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, const Halo &h, const TabF64 &tf,
+                               const TabMixed &tm, double rc2, int prec, bool want_e) {
   const dim3 grid(kBatchGrid), block(kBlock);
+  const size_t sh = tile_lds_bytes(h, false);
   if (prec == 0) {
     if (want_e)
-      kb_rho_df<0, true><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+      kb_rho_df<0, true><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
     else
-      kb_rho_df<0, false><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+      kb_rho_df<0, false><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
   } else {
     if (want_e)
-      kb_rho_df<1, true><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+      kb_rho_df<1, true><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
     else
-      kb_rho_df<1, false><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+      kb_rho_df<1, false><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, const TabF64 &tf, const TabMixed &tm, double rc2,
-                              int prec, bool want_e) {
+hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, const Halo &h, const TabF64 &tf,
+                              const TabMixed &tm, double rc2, int prec, bool want_e) {
   const dim3 grid(kBatchGrid), block(kBlock);
+  const size_t sh = tile_lds_bytes(h, true);
   if (prec == 0) {
     if (want_e)
-      kb_force<0, true><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+      kb_force<0, true><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
     else
-      kb_force<0, false><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+      kb_force<0, false><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
   } else {
     if (want_e)
-      kb_force<1, true><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+      kb_force<1, true><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
     else
-      kb_force<1, false><<<grid, block, 0, st>>>(L, tf, tm, rc2);
+      kb_force<1, false><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
   }
   return hipGetLastError();
+}
+
+hipError_t batch_kernels_allow_lds(size_t bytes) {
+  const void *fns[] = {reinterpret_cast<const void *>(kb_rho_df<0, true>), reinterpret_cast<const void *>(kb_rho_df<0, false>),
+                       reinterpret_cast<const void *>(kb_rho_df<1, true>), reinterpret_cast<const void *>(kb_rho_df<1, false>),
+                       reinterpret_cast<const void *>(kb_force<0, true>), reinterpret_cast<const void *>(kb_force<0, false>),
+                       reinterpret_cast<const void *>(kb_force<1, true>), reinterpret_cast<const void *>(kb_force<1, false>)};
+  for (const void *f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_batch_energy(hipStream_t st, const BatchLists &L, int which) {

@@ -8,22 +8,25 @@
 namespace mmd {
 
 template <int PREC, bool WANT_E>
-__global__ __launch_bounds__(kBlock) void k_rho_df(Geom g, const int32_t *__restrict__ off, int32_t nk_even,
+__global__ __launch_bounds__(kBlock) void k_rho_df(Geom g, Halo h, const int32_t *__restrict__ off, int32_t nk_even,
                                                      int32_t nk_odd, const double4 *__restrict__ P,
                                                      double *__restrict__ RHO, double *__restrict__ DF, TabF64 tf,
                                                      TabMixed tm, double rc2, double *__restrict__ e_part) {
-  __shared__ double lds[kWavesPerBlock];
-  rho_df_block<PREC, WANT_E>(g, xcd_block(), off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part, lds);
+  extern __shared__ double2 lds_tile[];
+  __shared__ double red[kWavesPerBlock];
+  rho_df_block<PREC, WANT_E>(g, h, xcd_block(), off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part, lds_tile, red);
 }
 
 template <int PREC, bool WANT_E>
-__global__ __launch_bounds__(kBlock) void k_force(Geom g, const int32_t *__restrict__ off, int32_t nk_even,
+__global__ __launch_bounds__(kBlock) void k_force(Geom g, Halo h, const int32_t *__restrict__ off, int32_t nk_even,
                                                     int32_t nk_odd, const double4 *__restrict__ P,
                                                     const double *__restrict__ DF, double *__restrict__ FX,
                                                     double *__restrict__ FY, double *__restrict__ FZ, TabF64 tf,
                                                     TabMixed tm, double rc2, double *__restrict__ e_part) {
-  __shared__ double lds[kWavesPerBlock];
-  force_block<PREC, WANT_E>(g, xcd_block(), off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part, lds);
+  extern __shared__ double2 lds_tile[];
+  __shared__ double red[kWavesPerBlock];
+  force_block<PREC, WANT_E>(g, h, xcd_block(), off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part, lds_tile,
+                            red);
 }
 
 // standalone df (src/kernels/hip_kernels.hip.cpp:21-40)
@@ -45,20 +48,23 @@ int64_t site_blocks(const Geom &g) {
   return (nb + kXcds - 1) / kXcds * kXcds;
 }
 
-hipError_t launch_rho_df(hipStream_t st, const Geom &g, const int32_t *off, int32_t nk_even, int32_t nk_odd,
-                         const double4 *P, double *RHO, double *DF, const TabF64 &tf, const TabMixed &tm, double rc2,
-                         int prec, double *e_part) {
-  const dim3 grid((unsigned)site_blocks(g)), block(kBlock);
+int64_t tile_blocks(const Geom &g) { return (g.n_tiles + kXcds - 1) / kXcds * kXcds; }
+
+hipError_t launch_rho_df(hipStream_t st, const Geom &g, const Halo &h, const int32_t *off, int32_t nk_even,
+                         int32_t nk_odd, const double4 *P, double *RHO, double *DF, const TabF64 &tf,
+                         const TabMixed &tm, double rc2, int prec, double *e_part) {
+  const dim3 grid((unsigned)tile_blocks(g)), block(kBlock);
+  const size_t sh = tile_lds_bytes(h, false);
   if (prec == 0) {
     if (e_part)
-      k_rho_df<0, true><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<0, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
     else
-      k_rho_df<0, false><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<0, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
   } else {
     if (e_part)
-      k_rho_df<1, true><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<1, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
     else
-      k_rho_df<1, false><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<1, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
   }
   return hipGetLastError();
 }
@@ -69,22 +75,36 @@ hipError_t launch_df(hipStream_t st, const Geom &g, const int8_t *T, const doubl
   return hipGetLastError();
 }
 
-hipError_t launch_force(hipStream_t st, const Geom &g, const int32_t *off, int32_t nk_even, int32_t nk_odd,
-                        const double4 *P, const double *DF, double *FX, double *FY, double *FZ, const TabF64 &tf,
-                        const TabMixed &tm, double rc2, int prec, double *e_part) {
-  const dim3 grid((unsigned)site_blocks(g)), block(kBlock);
+hipError_t launch_force(hipStream_t st, const Geom &g, const Halo &h, const int32_t *off, int32_t nk_even,
+                        int32_t nk_odd, const double4 *P, const double *DF, double *FX, double *FY, double *FZ,
+                        const TabF64 &tf, const TabMixed &tm, double rc2, int prec, double *e_part) {
+  const dim3 grid((unsigned)tile_blocks(g)), block(kBlock);
+  const size_t sh = tile_lds_bytes(h, true);
   if (prec == 0) {
     if (e_part)
-      k_force<0, true><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
+      k_force<0, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
     else
-      k_force<0, false><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
+      k_force<0, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
   } else {
     if (e_part)
-      k_force<1, true><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
+      k_force<1, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
     else
-      k_force<1, false><<<grid, block, 0, st>>>(g, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
+      k_force<1, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
   }
   return hipGetLastError();
+}
+
+// Dynamic LDS beyond the default 64 KB must be enabled per kernel.
+hipError_t pair_kernels_allow_lds(size_t bytes) {
+  const void *fns[] = {reinterpret_cast<const void *>(k_rho_df<0, true>), reinterpret_cast<const void *>(k_rho_df<0, false>),
+                       reinterpret_cast<const void *>(k_rho_df<1, true>), reinterpret_cast<const void *>(k_rho_df<1, false>),
+                       reinterpret_cast<const void *>(k_force<0, true>), reinterpret_cast<const void *>(k_force<0, false>),
+                       reinterpret_cast<const void *>(k_force<1, true>), reinterpret_cast<const void *>(k_force<1, false>)};
+  for (const void *f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace mmd

@@ -61,8 +61,8 @@ struct RepLayout {
   mmd_domain d;
   Geom g;
   double L[3];
-  int64_t nb, n_ext_blocks;
-  size_t pos, vx, vy, vz, type, off, fx, fy, fz, rho, df, e_rho, e_force, e_ke, energy, bytes;
+  int64_t nb, nt, n_ext_blocks;
+  size_t pos, vx, vy, vz, type, fx, fy, fz, rho, df, e_rho, e_force, e_ke, energy, bytes;
 };
 
 int rep_layout(int32_t cells, int32_t ghost, double a, RepLayout &R) {
@@ -75,6 +75,7 @@ int rep_layout(int32_t cells, int32_t ghost, double a, RepLayout &R) {
   if (rc) return rc;
   R.g = make_geom(R.d);
   R.nb = site_blocks(R.g);
+  R.nt = tile_blocks(R.g);
   R.n_ext_blocks = (R.g.n_ext + kBlock - 1) / kBlock;
   const size_t n = (size_t)R.g.n_ext;
   size_t o = 0;
@@ -86,10 +87,9 @@ int rep_layout(int32_t cells, int32_t ghost, double a, RepLayout &R) {
   R.pos = put(32 * n);
   R.vx = put(8 * n), R.vy = put(8 * n), R.vz = put(8 * n);
   R.type = put(n);
-  R.off = put(sizeof(int32_t) * 2 * kMaxStencil);
   R.fx = put(8 * n), R.fy = put(8 * n), R.fz = put(8 * n);
   R.rho = put(8 * n), R.df = put(8 * n);
-  R.e_rho = put(8 * (size_t)R.nb), R.e_force = put(8 * (size_t)R.nb), R.e_ke = put(8 * (size_t)R.nb);
+  R.e_rho = put(8 * (size_t)R.nt), R.e_force = put(8 * (size_t)R.nt), R.e_ke = put(8 * (size_t)R.nb);
   R.energy = put(4 * sizeof(double));
   R.bytes = o;
   return MMD_OK;
@@ -221,12 +221,15 @@ struct SimExec final : Executor {
 
 struct ListBuf {
   int2 *h_site = nullptr, *d_site = nullptr;
+  int2 *h_tile = nullptr, *d_tile = nullptr;
   int2 *h_ext = nullptr, *d_ext = nullptr;
-  size_t cap_site = 0, cap_ext = 0;
+  size_t cap_site = 0, cap_tile = 0, cap_ext = 0;
   int *h_reps = nullptr, *d_reps = nullptr;  // [max_replicas]
-  int *h_n = nullptr, *d_n = nullptr;        // {n_site, n_ext, n_reps}
+  int *h_n = nullptr, *d_n = nullptr;        // {n_site, n_tile, n_ext, n_reps}
 
-  BatchLists lists(const BatchDom *dom) const { return {dom, d_site, d_n, d_ext, d_n + 1, d_reps, d_n + 2}; }
+  BatchLists lists(const BatchDom *dom) const {
+    return {dom, d_site, d_n, d_tile, d_n + 1, d_ext, d_n + 2, d_reps, d_n + 3};
+  }
 };
 
 template <typename T>
@@ -245,10 +248,12 @@ int grow(T **h, T **d, size_t *cap, size_t need) {
 
 void free_lists(ListBuf &b) {
   if (b.h_site) (void)hipHostFree(b.h_site);
+  if (b.h_tile) (void)hipHostFree(b.h_tile);
   if (b.h_ext) (void)hipHostFree(b.h_ext);
   if (b.h_reps) (void)hipHostFree(b.h_reps);
   if (b.h_n) (void)hipHostFree(b.h_n);
   if (b.d_site) (void)hipFree(b.d_site);
+  if (b.d_tile) (void)hipFree(b.d_tile);
   if (b.d_ext) (void)hipFree(b.d_ext);
   if (b.d_reps) (void)hipFree(b.d_reps);
   if (b.d_n) (void)hipFree(b.d_n);
@@ -267,6 +272,8 @@ struct mmd_server {
   size_t arena_bytes = 0;
   std::vector<int32_t> st3[2];  // stencil (dx doubled, dy, dz) per centre parity
   int32_t nk[2] = {0, 0};
+  Halo halo{};                  // pair-pass LDS tile of the stencil
+  int32_t *d_off_lds = nullptr; // [2][kMaxStencil] stencil LDS offsets, shared by all replicas
   BatchDom *h_dom = nullptr, *d_dom = nullptr;  // [max_replicas], indexed by slot
   ListBuf step, prime;
   std::vector<int> step_set;  // active replicas the step lists were built for
@@ -344,13 +351,6 @@ int load_one(mmd_server *S, int r, hipStream_t ws, char *&stage, size_t &stage_c
         VX[d] = h.vx[i], VY[d] = h.vy[i], VZ[d] = h.vz[i];
         T[d] = h.type[i];
       }
-  int32_t *off = reinterpret_cast<int32_t *>(stage + L.off);
-  std::memset(off, 0, sizeof(int32_t) * 2 * kMaxStencil);
-  for (int p = 0; p < 2; p++)
-    for (int32_t k = 0; k < S->nk[p]; k++) {
-      const int32_t *e = &S->st3[p][3 * (size_t)k];
-      off[p * kMaxStencil + k] = stencil_dev_offset(p, e[0], e[1], e[2], L.d);
-    }
   char *base = S->arena + rep.off;
   HIP_TRY(hipMemcpyAsync(base, stage, L.fx, hipMemcpyHostToDevice, ws));
   HIP_TRY(launch_fill_f64(ws, reinterpret_cast<double *>(base + L.fx), (int64_t)((L.bytes - L.fx) / 8), 0.0));
@@ -397,33 +397,39 @@ void worker_main(mmd_server *S) {
 
 int fill_lists(mmd_server *S, ListBuf &b, const std::vector<int> &reps, bool is_step) {
   HIP_TRY(hipStreamSynchronize(S->cs));  // the pinned images may still feed an earlier copy
-  size_t n_site = 0, n_ext = 0;
+  size_t n_site = 0, n_tile = 0, n_ext = 0;
   std::vector<RepLayout> lay(reps.size());
   for (size_t k = 0; k < reps.size(); k++) {
     const Rep &r = (*S->R)[(size_t)reps[k]];
     int rc = rep_layout(r.cells, S->o.ghost_cells, S->o.lattice_const, lay[k]);
     if (rc) return rc;
     n_site += (size_t)lay[k].nb;
+    n_tile += (size_t)lay[k].nt;
     n_ext += (size_t)lay[k].n_ext_blocks;
   }
-  if (n_site >= (size_t)INT32_MAX || n_ext >= (size_t)INT32_MAX) return fail(MMD_E_RANGE, "serving: work list too long");
-  const int2 *old_site = b.d_site, *old_ext = b.d_ext;
+  if (n_site >= (size_t)INT32_MAX || n_tile >= (size_t)INT32_MAX || n_ext >= (size_t)INT32_MAX)
+    return fail(MMD_E_RANGE, "serving: work list too long");
+  const int2 *old_site = b.d_site, *old_tile = b.d_tile, *old_ext = b.d_ext;
   int rc = grow(&b.h_site, &b.d_site, &b.cap_site, n_site);
+  if (!rc) rc = grow(&b.h_tile, &b.d_tile, &b.cap_tile, n_tile);
   if (!rc) rc = grow(&b.h_ext, &b.d_ext, &b.cap_ext, n_ext);
   if (rc) return rc;
-  if (is_step && (b.d_site != old_site || b.d_ext != old_ext)) drop_graphs(S);  // graphs hold the list pointers
-  size_t is = 0, ie = 0;
+  if (is_step && (b.d_site != old_site || b.d_tile != old_tile || b.d_ext != old_ext))
+    drop_graphs(S);  // graphs hold the list pointers
+  size_t is = 0, it = 0, ie = 0;
   for (size_t k = 0; k < reps.size(); k++) {
     const int slot = (*S->R)[(size_t)reps[k]].slot;
     for (int64_t lb = 0; lb < lay[k].nb; lb++) b.h_site[is++] = make_int2(slot, (int)lb);
+    for (int64_t tb = 0; tb < lay[k].nt; tb++) b.h_tile[it++] = make_int2(slot, (int)tb);
     for (int64_t eb = 0; eb < lay[k].n_ext_blocks; eb++) b.h_ext[ie++] = make_int2(slot, (int)eb);
     b.h_reps[k] = slot;
   }
-  b.h_n[0] = (int)n_site, b.h_n[1] = (int)n_ext, b.h_n[2] = (int)reps.size();
+  b.h_n[0] = (int)n_site, b.h_n[1] = (int)n_tile, b.h_n[2] = (int)n_ext, b.h_n[3] = (int)reps.size();
   if (n_site) HIP_TRY(hipMemcpyAsync(b.d_site, b.h_site, n_site * sizeof(int2), hipMemcpyHostToDevice, S->cs));
+  if (n_tile) HIP_TRY(hipMemcpyAsync(b.d_tile, b.h_tile, n_tile * sizeof(int2), hipMemcpyHostToDevice, S->cs));
   if (n_ext) HIP_TRY(hipMemcpyAsync(b.d_ext, b.h_ext, n_ext * sizeof(int2), hipMemcpyHostToDevice, S->cs));
   if (!reps.empty()) HIP_TRY(hipMemcpyAsync(b.d_reps, b.h_reps, reps.size() * sizeof(int), hipMemcpyHostToDevice, S->cs));
-  HIP_TRY(hipMemcpyAsync(b.d_n, b.h_n, 3 * sizeof(int), hipMemcpyHostToDevice, S->cs));
+  HIP_TRY(hipMemcpyAsync(b.d_n, b.h_n, 4 * sizeof(int), hipMemcpyHostToDevice, S->cs));
   return MMD_OK;
 }
 
@@ -433,9 +439,9 @@ int step_passes(mmd_server *S, const BatchLists &L, bool want_e) {
   const int prec = S->o.precision;
   HIP_TRY(launch_batch_kick(S->cs, L, M, h, dt, true, false));
   HIP_TRY(launch_batch_ghost(S->cs, L, MMD_F_X));
-  HIP_TRY(launch_batch_rho_df(S->cs, L, S->tab->tf, S->tab->tm, rc2, prec, want_e));
+  HIP_TRY(launch_batch_rho_df(S->cs, L, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e));
   HIP_TRY(launch_batch_ghost(S->cs, L, MMD_F_DF));
-  HIP_TRY(launch_batch_force(S->cs, L, S->tab->tf, S->tab->tm, rc2, prec, want_e));
+  HIP_TRY(launch_batch_force(S->cs, L, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e));
   HIP_TRY(launch_batch_kick(S->cs, L, M, h, dt, false, want_e));
   if (want_e) HIP_TRY(launch_batch_energy(S->cs, L, 0));
   return MMD_OK;
@@ -465,7 +471,7 @@ struct GpuExec final : Executor {
     d.rho = reinterpret_cast<double *>(base + L.rho);
     d.df = reinterpret_cast<double *>(base + L.df);
     d.type = reinterpret_cast<int8_t *>(base + L.type);
-    d.off = reinterpret_cast<const int32_t *>(base + L.off);
+    d.off = S->d_off_lds;
     d.nk_even = S->nk[0], d.nk_odd = S->nk[1];
     d.e_rho = reinterpret_cast<double *>(base + L.e_rho);
     d.e_force = reinterpret_cast<double *>(base + L.e_force);
@@ -473,6 +479,7 @@ struct GpuExec final : Executor {
     d.energy = reinterpret_cast<double *>(base + L.energy);
     for (int k = 0; k < 3; k++) d.L[k] = L.L[k];
     d.n_blocks = L.nb;
+    d.n_tiles = L.nt;
     HIP_TRY(hipMemcpyAsync(S->d_dom + rep.slot, &d, sizeof(BatchDom), hipMemcpyHostToDevice, S->cs));
     {
       std::lock_guard<std::mutex> lk(S->mu);
@@ -504,9 +511,9 @@ struct GpuExec final : Executor {
     const double rc2 = S->o.cutoff * S->o.cutoff;
     const int prec = S->o.precision;
     HIP_TRY(launch_batch_ghost(S->cs, L, MMD_F_X));
-    HIP_TRY(launch_batch_rho_df(S->cs, L, S->tab->tf, S->tab->tm, rc2, prec, true));
+    HIP_TRY(launch_batch_rho_df(S->cs, L, S->halo, S->tab->tf, S->tab->tm, rc2, prec, true));
     HIP_TRY(launch_batch_ghost(S->cs, L, MMD_F_DF));
-    HIP_TRY(launch_batch_force(S->cs, L, S->tab->tf, S->tab->tm, rc2, prec, true));
+    HIP_TRY(launch_batch_force(S->cs, L, S->halo, S->tab->tf, S->tab->tm, rc2, prec, true));
     HIP_TRY(launch_batch_kick(S->cs, L, S->tab->masses, 0.0, S->o.dt, false, true));
     HIP_TRY(launch_batch_energy(S->cs, L, 1));
     return fetch_energies(reps);
@@ -631,6 +638,7 @@ void server_free(mmd_server *S) {
   if (S->h_dom) (void)hipHostFree(S->h_dom);
   if (S->d_dom) (void)hipFree(S->d_dom);
   if (S->h_e) (void)hipHostFree(S->h_e);
+  if (S->d_off_lds) (void)hipFree(S->d_off_lds);
   if (S->arena) (void)hipFree(S->arena);
   if (S->tab) (void)mmd_ctx_destroy(S->tab);
   delete S;
@@ -727,6 +735,24 @@ extern "C" int mmd_server_create(const mmd_server_opts *opts, const mmd_eam_tabl
                            opts->cutoff + opts->skin, opts->ghost_cells));
       }
     }
+    {
+      const int32_t *st3[2] = {S->st3[0].data(), S->st3[1].data()};
+      S->halo = make_halo(st3, S->nk);
+      if ((rc = check_tile_lds(opts->device, S->halo))) return bail(rc);
+      const size_t need = tile_lds_bytes(S->halo, true);
+      if (need > 65536 && batch_kernels_allow_lds(need) != hipSuccess)
+        return bail(fail(MMD_E_RANGE, "mmd_server_create: LDS tile of %zu bytes refused", need));
+      std::vector<int32_t> off(2 * kMaxStencil, 0);
+      for (int p = 0; p < 2; p++)
+        for (int32_t k = 0; k < S->nk[p]; k++) {
+          const int32_t *e = &S->st3[p][3 * (size_t)k];
+          off[(size_t)p * kMaxStencil + k] = stencil_lds_offset(p, e[0], e[1], e[2], S->halo);
+        }
+      if (hipMalloc((void **)&S->d_off_lds, off.size() * sizeof(int32_t)) != hipSuccess)
+        return bail(fail(MMD_E_NOMEM, "mmd_server_create: stencil offsets"));
+      if (hipMemcpy(S->d_off_lds, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(MMD_E_HIP, "mmd_server_create: stencil offsets upload"));
+    }
     size_t free_b = 0, total_b = 0;
     HIP_TRY(hipMemGetInfo(&free_b, &total_b));
     size_t want = (size_t)((double)free_b * opts->hbm_fraction);
@@ -748,8 +774,8 @@ extern "C" int mmd_server_create(const mmd_server_opts *opts, const mmd_eam_tabl
     for (ListBuf *b : {&S->step, &S->prime}) {
       if (hipHostMalloc((void **)&b->h_reps, m * sizeof(int), hipHostMallocDefault) != hipSuccess ||
           hipMalloc((void **)&b->d_reps, m * sizeof(int)) != hipSuccess ||
-          hipHostMalloc((void **)&b->h_n, 3 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-          hipMalloc((void **)&b->d_n, 3 * sizeof(int)) != hipSuccess)
+          hipHostMalloc((void **)&b->h_n, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+          hipMalloc((void **)&b->d_n, 4 * sizeof(int)) != hipSuccess)
         return bail(fail(MMD_E_NOMEM, "mmd_server_create: work lists"));
     }
     for (int w = 0; w < opts->workers; w++) S->workers.emplace_back(worker_main, S);
