@@ -465,9 +465,16 @@ static SplHost pack_f64(Arena &A, const double *knots, int nfun, int n, double d
 }
 
 struct MixHost {
-  size_t vb, vc, db, dc;
+  size_t vb, vc, dp;
   double s_lo, s_hi;  // range where the table meets the accuracy bound
 };
+
+// 16-byte derivative record: FP64 base, FP32 linear and quadratic terms
+struct DerivRec {
+  double base;
+  float b1, b2;
+};
+static_assert(sizeof(DerivRec) == 16, "derivative record must be one 16-byte load");
 
 // Accuracy bounds of the r^2-keyed tables, relative error against the FP64
 // r-spline they resample, per pair term: values 1e-9; the pair-force term
@@ -481,10 +488,10 @@ constexpr double kMixedTolForce = 2e-8;
 // knot values and exact derivatives of g(sqrt(s)) at uniform s, cubic Hermite
 // in s; FP64 base and linear terms, FP32 quadratic and cubic terms.
 static MixHost pack_mixed(Arena &A, const double *knots, int nfun, int n_r, double r0, double dr, int n_s, double s0,
-                          double ds, double tol_deriv, std::vector<double> *db_out = nullptr,
-                          std::vector<float> *dc_out = nullptr) {
-  std::vector<double> c7((size_t)n_r * 7), h(n_s), hp(n_s), vb((size_t)nfun * n_s * 2), db(vb.size());
-  std::vector<float> vc((size_t)nfun * n_s * 2), dc((size_t)nfun * n_s);
+                          double ds, double tol_deriv, std::vector<DerivRec> *dp_out = nullptr) {
+  std::vector<double> c7((size_t)n_r * 7), h(n_s), hp(n_s), vb((size_t)nfun * n_s * 2);
+  std::vector<float> vc((size_t)nfun * n_s * 2);
+  std::vector<DerivRec> dp((size_t)nfun * n_s);
   const double inv_dr = 1.0 / dr;
   for (int f = 0; f < nfun; f++) {
     spline7(knots + (size_t)f * n_r, n_r, dr, c7.data());
@@ -506,8 +513,9 @@ static MixHost pack_mixed(Arena &A, const double *knots, int nfun, int n_r, doub
       const size_t r = (size_t)f * n_s + m;
       vb[2 * r] = h[m], vb[2 * r + 1] = a1;
       vc[2 * r] = (float)a2, vc[2 * r + 1] = (float)a3;
-      db[2 * r] = hp[m], db[2 * r + 1] = 2.0 * a2 / ds;
-      dc[r] = (float)(3.0 * a3 / ds);
+      dp[r].base = hp[m];
+      dp[r].b1 = (float)(2.0 * a2 / ds);
+      dp[r].b2 = (float)(3.0 * a3 / ds);
     }
   }
   // Dense check of every segment against the FP64 spline (the same FP32
@@ -532,7 +540,7 @@ static MixHost pack_mixed(Arena &A, const double *knots, int nfun, int n_r, doub
         const double g_ex = spline7_deriv(c7.data(), n_r, r0, inv_dr, r) / (2.0 * r);
         const float c = std::fma((float)q, vc[2 * rec + 1], vc[2 * rec]);
         const double h = std::fma(q, std::fma(q, (double)c, vb[2 * rec + 1]), vb[2 * rec]);
-        const double g = std::fma(q, std::fma(q, (double)dc[rec], db[2 * rec + 1]), db[2 * rec]);
+        const double g = std::fma(q, std::fma(q, (double)dp[rec].b2, (double)dp[rec].b1), dp[rec].base);
         if (std::fabs(h - h_ex) > kMixedTolValue * (std::fabs(h_ex) + 1e-3 * hmax) ||
             std::fabs(g - g_ex) > tol_deriv * (std::fabs(g_ex) + 1e-3 * gmax))
           ok[(size_t)m] = 0;
@@ -542,15 +550,13 @@ static MixHost pack_mixed(Arena &A, const double *knots, int nfun, int n_r, doub
   int lo = n_s / 2, hi = n_s / 2;
   while (lo > 0 && ok[(size_t)lo - 1]) lo--;
   while (hi < n_s - 1 && ok[(size_t)hi]) hi++;
-  if (db_out) *db_out = db;
-  if (dc_out) *dc_out = dc;
+  if (dp_out) *dp_out = dp;
   MixHost o;
   o.s_lo = s0 + ds * (double)lo;
   o.s_hi = s0 + ds * (double)hi;
   o.vb = A.put(vb.data(), vb.size() * 8);
   o.vc = A.put(vc.data(), vc.size() * 4);
-  o.db = A.put(db.data(), db.size() * 8);
-  o.dc = A.put(dc.data(), dc.size() * 4);
+  o.dp = A.put(dp.data(), dp.size() * sizeof(DerivRec));
   return o;
 }
 
@@ -597,8 +603,7 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
     const int n_s = c->mixed_knots > 0 ? c->mixed_knots : std::max(256, (nr + 3) / 4);
     const double rmax = t->r0 + t->dr * (double)(nr - 1);
     const double s0 = t->r0 * t->r0, ds = (rmax * rmax - s0) / (double)(n_s - 1);
-    std::vector<double> rdb, pdb;
-    std::vector<float> rdc, pdc;
+    std::vector<DerivRec> rdp, pdp;
     // largest |F'| over the central 80% of the embedding tables
     double fpmax = 0.02;
     {
@@ -608,21 +613,19 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
         for (int m = nrho / 10; m < nrho - nrho / 10; m++) fpmax = std::max(fpmax, std::fabs(ce[(size_t)m * 7 + 2]));
       }
     }
-    const MixHost m_rho = pack_mixed(A, t->rho_r, nt, nr, t->r0, t->dr, n_s, s0, ds, kMixedTolForce / fpmax, &rdb, &rdc);
-    const MixHost m_phi = pack_mixed(A, t->phi_r, nt * nt, nr, t->r0, t->dr, n_s, s0, ds, kMixedTolForce, &pdb, &pdc);
+    const MixHost m_rho = pack_mixed(A, t->rho_r, nt, nr, t->r0, t->dr, n_s, s0, ds, kMixedTolForce / fpmax, &rdp);
+    const MixHost m_phi = pack_mixed(A, t->phi_r, nt * nt, nr, t->r0, t->dr, n_s, s0, ds, kMixedTolForce, &pdp);
     const size_t h_fc = pack_force_f64(A, t->rho_r, t->phi_r, nt, nr, t->dr);
-    // mixed force records per type pair (a, b): (rho_b dbase, b1), (phi_ab dbase, b1), (rho_b b2, phi_ab b2)
-    std::vector<double> fb((size_t)nt * nt * n_s * 4);
-    std::vector<float> fq((size_t)nt * nt * n_s * 2);
+    // mixed force records per type pair (a, b): rho_b and phi_ab derivative records side by side
+    std::vector<DerivRec> fb((size_t)nt * nt * n_s * 2);
     for (int a = 0; a < nt; a++)
       for (int b = 0; b < nt; b++)
         for (int m = 0; m < n_s; m++) {
-          const size_t r = ((size_t)a * nt + b) * n_s + m, rr = (size_t)b * n_s + m;
-          fb[4 * r + 0] = rdb[2 * rr], fb[4 * r + 1] = rdb[2 * rr + 1];
-          fb[4 * r + 2] = pdb[2 * r], fb[4 * r + 3] = pdb[2 * r + 1];
-          fq[2 * r + 0] = rdc[rr], fq[2 * r + 1] = pdc[r];
+          const size_t r = ((size_t)a * nt + b) * n_s + m;
+          fb[2 * r] = rdp[(size_t)b * n_s + m];
+          fb[2 * r + 1] = pdp[r];
         }
-    const size_t h_fb = A.put(fb.data(), fb.size() * 8), h_fq = A.put(fq.data(), fq.size() * 4);
+    const size_t h_fb = A.put(fb.data(), fb.size() * sizeof(DerivRec));
 
     HIP_TRY(hipStreamSynchronize(c->stream));
     ctx_drop_graphs(c);
@@ -643,8 +646,7 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
       SplMixed s;
       s.vb = reinterpret_cast<const double2 *>(dev + h.vb);
       s.vc = reinterpret_cast<const float2 *>(dev + h.vc);
-      s.db = reinterpret_cast<const double2 *>(dev + h.db);
-      s.dc = reinterpret_cast<const float *>(dev + h.dc);
+      s.dp = reinterpret_cast<const double2 *>(dev + h.dp);
       return s;
     };
     TabF64 &tf = c->tf;
@@ -655,7 +657,6 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
     TabMixed &tm = c->tm;
     tm.rho = M(m_rho), tm.phi = M(m_phi);
     tm.fb = reinterpret_cast<const double2 *>(dev + h_fb);
-    tm.fq = reinterpret_cast<const float2 *>(dev + h_fq);
     tm.nt = nt, tm.n_s = n_s, tm.s0 = s0, tm.inv_ds = 1.0 / ds;
     tm.s_head = std::max(m_rho.s_lo, m_phi.s_lo);
     tm.s_tail = std::min(m_rho.s_hi, m_phi.s_hi);

@@ -49,9 +49,9 @@ struct Halo {
   int32_t n;
 };
 
-// LDS bytes of a pair-pass block: (x, y), (z, type) [+ df] per staged record
+// LDS bytes of a pair-pass block: (x, y), z or (z, df), type per staged record
 __host__ __device__ inline size_t tile_lds_bytes(const Halo &h, bool with_df) {
-  return (size_t)h.n * (with_df ? 40 : 32);
+  return (size_t)h.n * (with_df ? 33 : 25);
 }
 
 // Loads through a uniform base pointer with a 32-bit element index, so the
@@ -96,24 +96,23 @@ __device__ __forceinline__ double spl_deriv(const SplF64 &s, uint32_t rec, doubl
 // Mixed-precision spline keyed by s = r^2 (no square root or division per
 // pair).  Per knot, with q the fraction of the segment:
 //   value = base + q * (a1 + q * c),   c = a2 + q * a3   (c in FP32)
-//   d/ds  = dbase + q * (b1 + q * b2)                     (b2 stored as FP32)
-// The FP64 base and linear terms carry the leading digits; the FP32 terms are
-// O(ds^2) corrections, so their rounding stays far below the interpolation
-// error of the resampled table.
+//   d/ds  = dbase + q * (b1 + q * b2)                     (b1, b2 in FP32)
+// The FP64 base (and the value's linear term) carry the leading digits; the
+// FP32 terms are corrections of relative size ds * f''/f' (a few per cent)
+// or smaller, so their rounding stays far below the table's interpolation
+// error.  A derivative record is 16 bytes — one load: (dbase, bits of (b1, b2)).
 struct SplMixed {
   const double2 *vb;  // (base, a1)
   const float2 *vc;   // (a2, a3)
-  const double2 *db;  // (dbase, b1)
-  const float *dc;    // b2
+  const double2 *dp;  // (dbase, (b1, b2) as two floats)
 };
 
 struct TabMixed {
   SplMixed rho;  // [nt][n_s]
   SplMixed phi;  // [nt*nt][n_s]
-  // force records per type pair (a, b) and knot: fb[2r] = (rho_b dbase, b1),
-  // fb[2r+1] = (phi_ab dbase, b1), fq[r] = (rho_b b2, phi_ab b2)
+  // force records per type pair (a, b) and knot: fb[2r] = rho_b derivative
+  // record, fb[2r+1] = phi_ab derivative record (two 16-byte loads per pair)
   const double2 *fb;  // [nt*nt][n_s][2]
-  const float2 *fq;   // [nt*nt][n_s]
   int32_t nt, n_s;
   double s0, inv_ds;
   // The reference spline's first and last three segments use low-order end
@@ -129,9 +128,10 @@ __device__ __forceinline__ double mix_value(const SplMixed &s, uint32_t rec, dou
   return fma(q, fma(q, (double)cc, b.y), b.x);
 }
 
-__device__ __forceinline__ double mix_deriv(const SplMixed &s, uint32_t rec, double q) {
-  const double2 b = ld(s.db, rec);
-  return fma(q, fma(q, (double)ld(s.dc, rec), b.y), b.x);
+// derivative from one 16-byte record
+__device__ __forceinline__ double mix_deriv_rec(double2 d, double q) {
+  const float2 b = __builtin_bit_cast(float2, d.y);
+  return fma(q, fma(q, (double)b.y, (double)b.x), d.x);
 }
 
 struct Masses {

@@ -125,13 +125,11 @@ __device__ __forceinline__ double fpair_mixed(const TabMixed &tm, int ti, int tj
   const bool unlike = ti != tj;
   const uint32_t ra = unlike ? (uint32_t)(ti * tm.n_s + sg.m) : 0u;
   const double2 b0 = ld(tm.fb, 2 * rp), b1 = ld(tm.fb, 2 * rp + 1);
-  const float2 q2 = ld(tm.fq, rp);
-  const double2 ua = ld(tm.rho.db, ra);
-  const float uc = ld(tm.rho.dc, ra);
+  const double2 ua = ld(tm.rho.dp, ra);
   const double q = sg.q;
-  const double rhojp = fma(q, fma(q, (double)q2.x, b0.y), b0.x);
-  const double phip = fma(q, fma(q, (double)q2.y, b1.y), b1.x);
-  const double rhoip = unlike ? fma(q, fma(q, (double)uc, ua.y), ua.x) : rhojp;
+  const double rhojp = mix_deriv_rec(b0, q);
+  const double phip = mix_deriv_rec(b1, q);
+  const double rhoip = unlike ? mix_deriv_rec(ua, q) : rhojp;
   if (WANT_E) phi = mix_value(tm.phi, rp, q);
   return -2.0 * (dfi * rhojp + dfj * rhoip + phip);
 }
@@ -140,8 +138,8 @@ __device__ __forceinline__ double fpair_mixed(const TabMixed &tm, int ti, int tj
 //
 // One block per tile of kTileX x kTileY x kTileZ cells (both sub-lattices),
 // one thread per site.  The block first stages the tile plus its stencil halo
-// in LDS — (x, y) and (z, type) of every record as two 16-byte arrays, and df
-// for the force pass — with coalesced row loads, so a neighbour costs LDS
+// in LDS — (x, y) and (z, df) of every record as 16-byte arrays (z alone for
+// the rho pass) plus the int8 types — with coalesced row loads, so a neighbour costs LDS
 // reads instead of vector-memory instructions; only the spline tables are
 // gathered from memory.  Stencil entries are LDS offsets (wave-uniform: a wave
 // holds one parity), kUnroll per trip with the tables' loads of a trip issued
@@ -150,10 +148,12 @@ __device__ __forceinline__ double fpair_mixed(const TabMixed &tm, int ti, int tj
 // ghost-extended lattice are staged as vacancies.  The body starts with a
 // barrier so a persistent block can restage for its next tile.
 
+// LDS tile layout (n records): lxy = (x, y) [n], then lzd = (z, df) [n] for
+// the force pass or lz = z [n] for the rho pass, then the int8 types [n].
 template <bool WITH_DF>
 __device__ __forceinline__ void stage_tile(const Geom &g, const Halo &h, const TileSite &t, int64_t lb,
                                            const double4 *__restrict__ P, const double *__restrict__ DF,
-                                           double2 *lxy, double2 *lzt, double *ldf) {
+                                           double2 *lxy, void *lz_or_lzd, int8_t *lt) {
   __syncthreads();
   if (lb < g.n_tiles) {
     for (int32_t idx = threadIdx.x; idx < h.n; idx += kBlock) {
@@ -165,8 +165,11 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const Halo &h, const T
         if (WITH_DF) f = ld(DF, (uint32_t)src);
       }
       lxy[idx] = make_double2(q.x, q.y);
-      lzt[idx] = make_double2(q.z, q.w);
-      if (WITH_DF) ldf[idx] = f;
+      if (WITH_DF)
+        static_cast<double2 *>(lz_or_lzd)[idx] = make_double2(q.z, f);
+      else
+        static_cast<double *>(lz_or_lzd)[idx] = q.z;
+      lt[idx] = (int8_t)q.w;
     }
   }
   __syncthreads();
@@ -179,12 +182,15 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
                                              double *__restrict__ DF, const TabF64 &tf, const TabMixed &tm, double rc2,
                                              double *__restrict__ e_part, double2 *lds, double *red) {
   const TileSite t = map_tile(g, h, lb);
-  double2 *lxy = lds, *lzt = lds + h.n;
-  stage_tile<false>(g, h, t, lb, P, nullptr, lxy, lzt, nullptr);
+  double2 *lxy = lds;
+  double *lz = reinterpret_cast<double *>(lds + h.n);
+  int8_t *lt = reinterpret_cast<int8_t *>(lz + h.n);
+  stage_tile<false>(g, h, t, lb, P, nullptr, lxy, lz, lt);
   double e_loc = 0.0;
   if (t.active) {
-    const double2 cxy = lxy[t.c], czt = lzt[t.c];
-    const int t0 = (int)czt.y;
+    const double2 cxy = lxy[t.c];
+    const double cz = lz[t.c];
+    const int t0 = lt[t.c];
     if (t0 >= 0) {
       const int32_t *o = off + t.s * kMaxStencil;
       const int nk = t.s ? nk_odd : nk_even;
@@ -196,9 +202,9 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
           const int32_t j = t.c + o[k + u];
-          const double2 pxy = lxy[j], pzt = lzt[j];
-          tj[u] = (int)pzt.y;
-          const double dx = cxy.x - pxy.x, dy = cxy.y - pxy.y, dz = czt.x - pzt.x;
+          const double2 pxy = lxy[j];
+          tj[u] = lt[j];
+          const double dx = cxy.x - pxy.x, dy = cxy.y - pxy.y, dz = cz - lz[j];
           d2[u] = dx * dx + dy * dy + dz * dz;
         }
         bool any = false;
@@ -254,17 +260,17 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
                                             const TabMixed &tm, double rc2, double *__restrict__ e_part, double2 *lds,
                                             double *red) {
   const TileSite t = map_tile(g, h, lb);
-  double2 *lxy = lds, *lzt = lds + h.n;
-  double *ldf = reinterpret_cast<double *>(lds + 2 * h.n);
-  stage_tile<true>(g, h, t, lb, P, DF, lxy, lzt, ldf);
+  double2 *lxy = lds, *lzd = lds + h.n;
+  int8_t *lt = reinterpret_cast<int8_t *>(lds + 2 * h.n);
+  stage_tile<true>(g, h, t, lb, P, DF, lxy, lzd, lt);
   double e_loc = 0.0;
   if (t.active) {
-    const double2 cxy = lxy[t.c], czt = lzt[t.c];
-    const int t0 = (int)czt.y;
+    const double2 cxy = lxy[t.c], czd = lzd[t.c];
+    const int t0 = lt[t.c];
     if (t0 >= 0) {
       const int32_t *o = off + t.s * kMaxStencil;
       const int nk = t.s ? nk_odd : nk_even;
-      const double dfi = ldf[t.c];
+      const double dfi = czd.y;
       const double d2_safe = 0.25 * rc2;
       double ax = 0.0, ay = 0.0, az = 0.0, ep = 0.0;
       for (int k = 0; k < nk; k += kUnroll) {
@@ -273,12 +279,12 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
           const int32_t j = t.c + o[k + u];
-          const double2 pxy = lxy[j], pzt = lzt[j];
-          tj[u] = (int)pzt.y;
+          const double2 pxy = lxy[j], pzd = lzd[j];
+          tj[u] = lt[j];
           dx[u] = cxy.x - pxy.x;
           dy[u] = cxy.y - pxy.y;
-          dz[u] = czt.x - pzt.x;
-          dfj[u] = ldf[j];
+          dz[u] = czd.x - pzd.x;
+          dfj[u] = pzd.y;
           d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
         }
         bool any = false;

@@ -83,12 +83,14 @@ def test_alone_mix_and_single_domain_identical(tab, prec):
         r_alone = res[0]
         # in a mix: others of other sizes, running before, during and after the probe
         mixed = _keep(9)
+        # a long-running replica is resident when the probe arrives, others come and go around it
         others = [dict(cells=c, temperature=t, steps=s, seed=100 + k, arrival_s=a)
-                  for k, (c, t, s, a) in enumerate([(14, 300.0, 23, 0.0), (6, 1500.0, 5, 0.0), (11, 1200.0, 17, 0.0),
+                  for k, (c, t, s, a) in enumerate([(14, 300.0, 4000, 0.0), (6, 1500.0, 5, 0.0), (11, 1200.0, 17, 0.0),
                                                     (8, 600.0, 9, 0.05), (12, 800.0, 30, 0.08)])]
         res, st = srv.run(others[:3] + [dict(probe, arrival_s=0.02, keep=mixed)] + others[3:])
         assert all(r["status"] == 0 for r in res), res
-        r_mix = res[3]
+        r_mix, long_ = res[3], res[0]
+        assert long_["t_ready"] < r_mix["t_done"] and r_mix["t_ready"] < long_["t_done"], (long_, r_mix)
         assert st["max_active"] >= 2
     _same(alone, mixed, "alone vs mix")
     for k in ("epot0", "ekin0", "epot", "ekin"):
