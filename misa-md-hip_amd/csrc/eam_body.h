@@ -68,39 +68,32 @@ __device__ __forceinline__ SegS seg_s(double s2, const TabMixed &t) {
 
 // ------------------------------------------------------------- pair terms
 //
-// The pair functions are branch-free so the table loads of all kUnroll pairs
-// of a trip are issued together: control flow around a pair's loads makes the
-// compiler wait for them before the next pair's loads are issued.
-//   * rho'_{ti} of unlike pairs is loaded through a selected index (lanes with
-//     like pairs read element 0, one cache line for the whole wave);
-//   * the mixed path evaluates every lane with the r^2-keyed tables (index
-//     clamped into the table) and reports whether the pair lies in the range
-//     where they meet the accuracy bound; the few pairs outside it are redone
-//     with the FP64 r-spline in a separate, wave-uniformly guarded pass.
+// A trip's kUnroll pairs are evaluated in three phases — table indices, then
+// every table load, then the arithmetic — with no control flow around a
+// pair's loads, so all of a trip's loads are in flight together (a branch or
+// an early use makes the compiler wait for each pair's loads before issuing
+// the next pair's).  rho'_{ti} is always loaded from the rho table at the
+// centre type: for like pairs it holds the same coefficients as the rho'_{tj}
+// part of the pair record, so no select is needed.  The mixed path evaluates
+// every lane with the r^2-keyed tables (index clamped into the table) and
+// reports whether the pair lies in the range where they meet the accuracy
+// bound; the few pairs outside it are redone with the FP64 r-spline in a
+// separate, wave-uniformly guarded pass.
 
-// rho_{tj}(r), r^2 = d2, FP64 r-keyed spline
+// FP64 r-keyed spline: rho_{tj}(r), r^2 = d2
 __device__ __forceinline__ double rho_f64(const TabF64 &tf, int tj, double d2) {
   const double r = d2 * rsqrt64(d2);
   const Seg sg = seg_f64(r, tf.r0, tf.inv_dr, tf.n_r);
   return spl_value(tf.rho, (uint32_t)(tj * tf.n_r + sg.m), sg.p);
 }
 
-// rho_{tj}(r) from the r^2-keyed table; ok = pair inside the table's checked range
-__device__ __forceinline__ double rho_mixed(const TabMixed &tm, int tj, double d2, bool &ok) {
-  ok = d2 < tm.s_tail && d2 >= tm.s_head;
-  const SegS sg = seg_s(d2, tm);
-  return mix_value(tm.rho, (uint32_t)(tj * tm.n_s + sg.m), sg.q);
-}
-
 // fpair = -(df_i rho'_{tj} + df_j rho'_{ti} + phi'_{ti tj}) / r (the scalar that
 // multiplies del = x_i - x_j); phi receives phi_{ti tj}(r) when WANT_E.
-// rho'_{tj} and phi'_{ti tj} come from one combined per-type-pair record.
 template <bool WANT_E>
 __device__ __forceinline__ double fpair_f64(const TabF64 &tf, int ti, int tj, double d2, double dfi, double dfj,
                                             double &phi) {
   const double rinv = rsqrt64(d2);
-  const double r = d2 * rinv;
-  const Seg sg = seg_f64(r, tf.r0, tf.inv_dr, tf.n_r);
+  const Seg sg = seg_f64(d2 * rinv, tf.r0, tf.inv_dr, tf.n_r);
   const uint32_t rp = (uint32_t)((ti * tf.nt + tj) * tf.n_r + sg.m);
   const bool unlike = ti != tj;
   const uint32_t ra = unlike ? (uint32_t)(ti * tf.n_r + sg.m) : 0u;
@@ -115,23 +108,93 @@ __device__ __forceinline__ double fpair_f64(const TabF64 &tf, int ti, int tj, do
   return -(dfi * rhojp + dfj * rhoip + phip) * rinv;
 }
 
-// Mixed path: d/ds tables, -(dpsi/dr)/r = -2 dpsi/ds, no square root or divide.
-template <bool WANT_E>
-__device__ __forceinline__ double fpair_mixed(const TabMixed &tm, int ti, int tj, double d2, double dfi, double dfj,
-                                              double &phi, bool &ok) {
-  ok = d2 < tm.s_tail && d2 >= tm.s_head;
-  const SegS sg = seg_s(d2, tm);
-  const uint32_t rp = (uint32_t)((ti * tm.nt + tj) * tm.n_s + sg.m);
-  const bool unlike = ti != tj;
-  const uint32_t ra = unlike ? (uint32_t)(ti * tm.n_s + sg.m) : 0u;
-  const double2 b0 = ld(tm.fb, 2 * rp), b1 = ld(tm.fb, 2 * rp + 1);
-  const double2 ua = ld(tm.rho.dp, ra);
-  const double q = sg.q;
-  const double rhojp = mix_deriv_rec(b0, q);
-  const double phip = mix_deriv_rec(b1, q);
-  const double rhoip = unlike ? mix_deriv_rec(ua, q) : rhojp;
-  if (WANT_E) phi = mix_value(tm.phi, rp, q);
-  return -2.0 * (dfi * rhojp + dfj * rhoip + phip);
+// Phased evaluation of a trip (mixed): d/ds tables, -(dpsi/dr)/r = -2 dpsi/ds,
+// no square root or divide; ok[u] = pair inside the tables' checked range.
+template <bool WANT_E, int U>
+__device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, int ti, const int *tj, const double *d2,
+                                                 double dfi, const double *dfj, double *fp, double *phi, bool *ok) {
+  double q[U];
+  uint32_t rp[U], ra[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    ok[u] = d2[u] < tm.s_tail && d2[u] >= tm.s_head;
+    const SegS sg = seg_s(d2[u], tm);
+    q[u] = sg.q;
+    rp[u] = (uint32_t)((ti * tm.nt + tj[u]) * tm.n_s + sg.m);
+    ra[u] = (uint32_t)(ti * tm.n_s + sg.m);
+  }
+  double2 b0[U], b1[U], ua[U], vb[U];
+  float2 vc[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    b0[u] = ld(tm.fb, 2 * rp[u]);
+    b1[u] = ld(tm.fb, 2 * rp[u] + 1);
+    ua[u] = ld(tm.rho.dp, ra[u]);
+    if (WANT_E) {
+      vb[u] = ld(tm.phi.vb, rp[u]);
+      vc[u] = ld(tm.phi.vc, rp[u]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const double rhojp = mix_deriv_rec(b0[u], q[u]);
+    const double phip = mix_deriv_rec(b1[u], q[u]);
+    const double rhoip = mix_deriv_rec(ua[u], q[u]);
+    if (WANT_E) {
+      const float cc = fmaf((float)q[u], vc[u].y, vc[u].x);
+      phi[u] = fma(q[u], fma(q[u], (double)cc, vb[u].y), vb[u].x);
+    }
+    fp[u] = -2.0 * (dfi * rhojp + dfj[u] * rhoip + phip);
+  }
+}
+
+// Phased rho of a trip (mixed): rho_{tj}(s) with the checked-range flags.
+template <int U>
+__device__ __forceinline__ void trip_rho_mixed(const TabMixed &tm, const int *tj, const double *d2, double *v,
+                                               bool *ok) {
+  double q[U];
+  uint32_t rec[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    ok[u] = d2[u] < tm.s_tail && d2[u] >= tm.s_head;
+    const SegS sg = seg_s(d2[u], tm);
+    q[u] = sg.q;
+    rec[u] = (uint32_t)(tj[u] * tm.n_s + sg.m);
+  }
+  double2 vb[U];
+  float2 vc[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    vb[u] = ld(tm.rho.vb, rec[u]);
+    vc[u] = ld(tm.rho.vc, rec[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const float cc = fmaf((float)q[u], vc[u].y, vc[u].x);
+    v[u] = fma(q[u], fma(q[u], (double)cc, vb[u].y), vb[u].x);
+  }
+}
+
+// Phased rho of a trip (FP64)
+template <int U>
+__device__ __forceinline__ void trip_rho_f64(const TabF64 &tf, const int *tj, const double *d2, double *v) {
+  double p[U];
+  uint32_t rec[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const double r = d2[u] * rsqrt64(d2[u]);
+    const Seg sg = seg_f64(r, tf.r0, tf.inv_dr, tf.n_r);
+    p[u] = sg.p;
+    rec[u] = (uint32_t)(tj[u] * tf.n_r + sg.m);
+  }
+  double2 a[U], b[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    a[u] = ld(tf.rho.v34, rec[u]);
+    b[u] = ld(tf.rho.v56, rec[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < U; u++) v[u] = ((a[u].x * p[u] + a[u].y) * p[u] + b[u].x) * p[u] + b[u].y;
 }
 
 // --------------------------------------------------------------- kernels
@@ -216,17 +279,22 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
         }
         if (__ballot(any) == 0) continue;
         double v[kUnroll];
+        int tq[kUnroll];
+        double dq[kUnroll];
         bool redo = false;
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
           const bool in = d2[u] >= 0.0;
-          if (PREC == 1) {
-            bool ok;
-            v[u] = rho_mixed(tm, in ? tj[u] : 0, in ? d2[u] : d2_safe, ok);
-            redo |= in && !ok;
-          } else {
-            v[u] = rho_f64(tf, in ? tj[u] : 0, in ? d2[u] : d2_safe);
-          }
+          tq[u] = in ? tj[u] : 0;
+          dq[u] = in ? d2[u] : d2_safe;
+        }
+        if (PREC == 1) {
+          bool ok[kUnroll];
+          trip_rho_mixed<kUnroll>(tm, tq, dq, v, ok);
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) redo |= d2[u] >= 0.0 && !ok[u];
+        } else {
+          trip_rho_f64<kUnroll>(tf, tq, dq, v);
         }
         if (PREC == 1 && __ballot(redo) != 0) {
 #pragma unroll
@@ -297,15 +365,27 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
         if (__ballot(any) == 0) continue;
         double fp[kUnroll], phi[kUnroll];
         bool redo = false;
+        if (PREC == 1) {
+          int tq[kUnroll];
+          double dq[kUnroll];
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-          const bool in = d2[u] >= 0.0;
-          phi[u] = 0.0;
-          if (PREC == 1) {
-            bool ok;
-            fp[u] = fpair_mixed<WANT_E>(tm, t0, in ? tj[u] : t0, in ? d2[u] : d2_safe, dfi, dfj[u], phi[u], ok);
-            redo |= in && !ok;
-          } else {
+          for (int u = 0; u < kUnroll; u++) {
+            const bool in = d2[u] >= 0.0;
+            phi[u] = 0.0;
+            tq[u] = in ? tj[u] : t0;
+            dq[u] = in ? d2[u] : d2_safe;
+          }
+          bool ok[kUnroll];
+          trip_force_mixed<WANT_E, kUnroll>(tm, t0, tq, dq, dfi, dfj, fp, phi, ok);
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) redo |= d2[u] >= 0.0 && !ok[u];
+        } else {
+          // FP64: pair by pair (its large tables are gathered from L2; this
+          // measured faster than phased loads)
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) {
+            const bool in = d2[u] >= 0.0;
+            phi[u] = 0.0;
             fp[u] = fpair_f64<WANT_E>(tf, t0, in ? tj[u] : t0, in ? d2[u] : d2_safe, dfi, dfj[u], phi[u]);
           }
         }
