@@ -249,10 +249,10 @@ def replica_trace(n, seed, rate, t0=0.0, steps=200):
                  arrival_s=float(arr[i])) for i in range(n)]
 
 
-def serving(n_replicas, load, seed=77, burst=256, steps=200):
+def serving(n_replicas, load, seed=77, burst=256, steps=200, use_graph=True):
     tab = mmd.synth_tables(n_types=3, rc=RC, jitter=0.1, seed=seed)
     with mmd.Server(tab, precision=mmd.PREC_MIXED, a=A, cutoff=RC, skin=SKIN, dt=DT, sigma=0.05, max_replicas=64,
-                    hbm_fraction=0.5, chunk_steps=10, workers=16) as srv:
+                    hbm_fraction=0.5, chunk_steps=10, workers=16, use_graph=use_graph) as srv:
         # capacity: closed-loop burst of the same replica mix, all present at t = 0
         res, st = srv.run(replica_trace(burst, seed + 1, 0.0, steps=steps))  # queue never empties until the tail
         t_start = min(r["t_ready"] for r in res)
@@ -292,6 +292,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-replicas", type=int, default=2000)
     ap.add_argument("--serve-load", type=float, default=0.75)
+    ap.add_argument("--serve-no-graph", action="store_true", help="launch the batched step directly (profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only time a device-to-device streaming copy of BYTES (HBM bandwidth / counter calibration)")
     args = ap.parse_args()
@@ -354,7 +355,7 @@ def main():
         out["mixed"] = mixed_vs_fp64(wd, args.steps, args.warmup)
 
     if not args.no_serve:
-        out["serving"] = serving(args.serve_replicas, args.serve_load)
+        out["serving"] = serving(args.serve_replicas, args.serve_load, use_graph=not args.serve_no_graph)
         out["serving"]["capacity_vs_fp64_path"] = round(out["serving"]["capacity_atom_steps_per_s"] / value, 3)
 
     if not args.no_cpu:
