@@ -249,15 +249,22 @@ def replica_trace(n, seed, rate, t0=0.0, steps=200):
                  arrival_s=float(arr[i])) for i in range(n)]
 
 
-def serving(n_replicas, load, seed=77, burst=256, steps=200, use_graph=True):
+def serving(n_replicas, load, seed=77, burst=512, steps=200, use_graph=True):
     tab = mmd.synth_tables(n_types=3, rc=RC, jitter=0.1, seed=seed)
     with mmd.Server(tab, precision=mmd.PREC_MIXED, a=A, cutoff=RC, skin=SKIN, dt=DT, sigma=0.05, max_replicas=64,
                     hbm_fraction=0.5, chunk_steps=10, workers=16, use_graph=use_graph) as srv:
-        # capacity: closed-loop burst of the same replica mix, all present at t = 0
-        res, st = srv.run(replica_trace(burst, seed + 1, 0.0, steps=steps))  # queue never empties until the tail
-        t_start = min(r["t_ready"] for r in res)
-        t_end = max(r["t_done"] for r in res)
-        capacity = st["atom_steps"] / (t_end - t_start)
+        # capacity: a burst of the same replica mix, all present at t = 0, measured while the admission queue
+        # is non-empty (from the first replica resident to the last admission), each replica's atom-steps
+        # spread evenly over its residency
+        res, st = srv.run(replica_trace(burst, seed + 1, 0.0, steps=steps))
+        t0 = min(r["t_ready"] for r in res)
+        t1 = max(r["t_admit"] for r in res)
+        work = 0.0
+        for r in res:
+            span = r["t_done"] - r["t_ready"]
+            overlap = max(0.0, min(r["t_done"], t1) - max(r["t_ready"], t0))
+            work += r["n_atoms"] * steps * (overlap / span if span > 0 else 0.0)
+        capacity = work / (t1 - t0)
         reps = replica_trace(n_replicas, seed, 1.0, steps=steps)
         mean_work = np.mean([2 * r["cells"] ** 3 * r["steps"] for r in reps])
         rate = load * capacity / mean_work

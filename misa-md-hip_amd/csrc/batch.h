@@ -1,8 +1,6 @@
 // Batched (multi-replica) device descriptors and launchers used by the
-// serving loop.  One launch covers every active replica: a persistent grid
-// walks a device work list of (replica, logical block) items, so a captured
-// HIP graph stays valid while the mix of replicas changes (only the work list
-// in device memory is rewritten between steps).  Each item runs exactly the
+// serving loop.  One launch covers every active replica: its blocks walk a
+// device work list of (replica, block) items.  Each item runs exactly the
 // per-block code of the single-domain kernels, and energies are reduced per
 // replica in a fixed order, so a replica's results do not depend on which
 // other replicas share the launch.
@@ -10,6 +8,8 @@
 #define MMD_BATCH_H
 
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "device_common.h"
 
@@ -42,15 +42,15 @@ struct BatchLists {
   const int *n_reps;
 };
 
-// grid of the persistent batched kernels (a multiple of the 8 XCDs)
-constexpr int kBatchGrid = 2048;
+// blocks per launch: a multiple of the 8 XCDs; one block per item when it covers the list
+inline int batch_blocks(int64_t items) { return (int)((std::max<int64_t>(items, 1) + 7) / 8 * 8); }
 
-hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, const Masses &M, double h, double dt, bool drift,
-                             bool want_ke);
-hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, uint32_t fields);
-hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, const Halo &h, const TabF64 &tf,
+hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, int blocks, const Masses &M, double h, double dt,
+                             bool drift, bool want_ke);
+hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, int blocks, uint32_t fields);
+hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, int blocks, const Halo &h, const TabF64 &tf,
                                const TabMixed &tm, double rc2, int prec, bool want_e);
-hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, const Halo &h, const TabF64 &tf,
+hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, int blocks, const Halo &h, const TabF64 &tf,
                               const TabMixed &tm, double rc2, int prec, bool want_e);
 hipError_t batch_kernels_allow_lds(size_t bytes);
 // per-replica {potential, kinetic} -> energy[2 * which], which = 0 (step) or 1 (admission)

@@ -1,6 +1,9 @@
-// Persistent batched kernels of the serving loop (see batch.h).  The work
-// list is split into one contiguous range per XCD, so the blocks of an XCD
-// walk neighbouring blocks of the same replica and share its halo in L2.
+// Batched kernels of the serving loop (see batch.h).  The work list is split
+// into one contiguous range per XCD (blocks are dispatched to the 8 XCDs
+// round-robin, so blockIdx.x % 8 names the XCD), so the blocks of an XCD take
+// neighbouring blocks of the same replica and share its halo in L2.  The
+// serving loop launches one block per item (grid = items rounded up to a
+// multiple of 8); a smaller grid makes the blocks loop over their items.
 #include "batch.h"
 #include "eam_body.h"
 #include "kernels.h"
@@ -97,9 +100,9 @@ __global__ __launch_bounds__(kBlock) void kb_energy(BatchLists L, int which) {
 
 }  // namespace
 
-hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, const Masses &M, double h, double dt, bool drift,
-                             bool want_ke) {
-  const dim3 grid(kBatchGrid), block(kBlock);
+hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, int blocks, const Masses &M, double h, double dt,
+                             bool drift, bool want_ke) {
+  const dim3 grid(blocks), block(kBlock);
   if (drift) {
     if (want_ke)
       kb_kick<true, true><<<grid, block, 0, st>>>(L, M, h, dt);
@@ -114,14 +117,14 @@ hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, const Masses &
   return hipGetLastError();
 }
 
-hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, uint32_t fields) {
-  kb_ghost<<<dim3(kBatchGrid), dim3(kBlock), 0, st>>>(L, fields);
+hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, int blocks, uint32_t fields) {
+  kb_ghost<<<dim3(blocks), dim3(kBlock), 0, st>>>(L, fields);
   return hipGetLastError();
 }
 
-hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, const Halo &h, const TabF64 &tf,
+hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, int blocks, const Halo &h, const TabF64 &tf,
                                const TabMixed &tm, double rc2, int prec, bool want_e) {
-  const dim3 grid(kBatchGrid), block(kBlock);
+  const dim3 grid(blocks), block(kBlock);
   const size_t sh = tile_lds_bytes(h, false);
   if (prec == 0) {
     if (want_e)
@@ -137,9 +140,9 @@ hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, const Halo &
   return hipGetLastError();
 }
 
-hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, const Halo &h, const TabF64 &tf,
+hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, int blocks, const Halo &h, const TabF64 &tf,
                               const TabMixed &tm, double rc2, int prec, bool want_e) {
-  const dim3 grid(kBatchGrid), block(kBlock);
+  const dim3 grid(blocks), block(kBlock);
   const size_t sh = tile_lds_bytes(h, true);
   if (prec == 0) {
     if (want_e)

@@ -226,6 +226,7 @@ struct ListBuf {
   size_t cap_site = 0, cap_tile = 0, cap_ext = 0;
   int *h_reps = nullptr, *d_reps = nullptr;  // [max_replicas]
   int *h_n = nullptr, *d_n = nullptr;        // {n_site, n_tile, n_ext, n_reps}
+  int g_site = 8, g_tile = 8, g_ext = 8;     // blocks per launch: one per item
 
   BatchLists lists(const BatchDom *dom) const {
     return {dom, d_site, d_n, d_tile, d_n + 1, d_ext, d_n + 2, d_reps, d_n + 3};
@@ -409,13 +410,11 @@ int fill_lists(mmd_server *S, ListBuf &b, const std::vector<int> &reps, bool is_
   }
   if (n_site >= (size_t)INT32_MAX || n_tile >= (size_t)INT32_MAX || n_ext >= (size_t)INT32_MAX)
     return fail(MMD_E_RANGE, "serving: work list too long");
-  const int2 *old_site = b.d_site, *old_tile = b.d_tile, *old_ext = b.d_ext;
   int rc = grow(&b.h_site, &b.d_site, &b.cap_site, n_site);
   if (!rc) rc = grow(&b.h_tile, &b.d_tile, &b.cap_tile, n_tile);
   if (!rc) rc = grow(&b.h_ext, &b.d_ext, &b.cap_ext, n_ext);
   if (rc) return rc;
-  if (is_step && (b.d_site != old_site || b.d_tile != old_tile || b.d_ext != old_ext))
-    drop_graphs(S);  // graphs hold the list pointers
+  if (is_step) drop_graphs(S);  // graphs hold the list pointers and the grid sizes
   size_t is = 0, it = 0, ie = 0;
   for (size_t k = 0; k < reps.size(); k++) {
     const int slot = (*S->R)[(size_t)reps[k]].slot;
@@ -425,6 +424,7 @@ int fill_lists(mmd_server *S, ListBuf &b, const std::vector<int> &reps, bool is_
     b.h_reps[k] = slot;
   }
   b.h_n[0] = (int)n_site, b.h_n[1] = (int)n_tile, b.h_n[2] = (int)n_ext, b.h_n[3] = (int)reps.size();
+  b.g_site = batch_blocks((int64_t)n_site), b.g_tile = batch_blocks((int64_t)n_tile), b.g_ext = batch_blocks((int64_t)n_ext);
   if (n_site) HIP_TRY(hipMemcpyAsync(b.d_site, b.h_site, n_site * sizeof(int2), hipMemcpyHostToDevice, S->cs));
   if (n_tile) HIP_TRY(hipMemcpyAsync(b.d_tile, b.h_tile, n_tile * sizeof(int2), hipMemcpyHostToDevice, S->cs));
   if (n_ext) HIP_TRY(hipMemcpyAsync(b.d_ext, b.h_ext, n_ext * sizeof(int2), hipMemcpyHostToDevice, S->cs));
@@ -433,16 +433,16 @@ int fill_lists(mmd_server *S, ListBuf &b, const std::vector<int> &reps, bool is_
   return MMD_OK;
 }
 
-int step_passes(mmd_server *S, const BatchLists &L, bool want_e) {
+int step_passes(mmd_server *S, const BatchLists &L, const ListBuf &B, bool want_e) {
   const double dt = S->o.dt, h = 0.5 * S->o.dt, rc2 = S->o.cutoff * S->o.cutoff;
   const Masses &M = S->tab->masses;
   const int prec = S->o.precision;
-  HIP_TRY(launch_batch_kick(S->cs, L, M, h, dt, true, false));
-  HIP_TRY(launch_batch_ghost(S->cs, L, MMD_F_X));
-  HIP_TRY(launch_batch_rho_df(S->cs, L, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e));
-  HIP_TRY(launch_batch_ghost(S->cs, L, MMD_F_DF));
-  HIP_TRY(launch_batch_force(S->cs, L, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e));
-  HIP_TRY(launch_batch_kick(S->cs, L, M, h, dt, false, want_e));
+  HIP_TRY(launch_batch_kick(S->cs, L, B.g_site, M, h, dt, true, false));
+  HIP_TRY(launch_batch_ghost(S->cs, L, B.g_ext, MMD_F_X));
+  HIP_TRY(launch_batch_rho_df(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e));
+  HIP_TRY(launch_batch_ghost(S->cs, L, B.g_ext, MMD_F_DF));
+  HIP_TRY(launch_batch_force(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e));
+  HIP_TRY(launch_batch_kick(S->cs, L, B.g_site, M, h, dt, false, want_e));
   if (want_e) HIP_TRY(launch_batch_energy(S->cs, L, 0));
   return MMD_OK;
 }
@@ -510,11 +510,12 @@ struct GpuExec final : Executor {
     const BatchLists L = S->prime.lists(S->d_dom);
     const double rc2 = S->o.cutoff * S->o.cutoff;
     const int prec = S->o.precision;
-    HIP_TRY(launch_batch_ghost(S->cs, L, MMD_F_X));
-    HIP_TRY(launch_batch_rho_df(S->cs, L, S->halo, S->tab->tf, S->tab->tm, rc2, prec, true));
-    HIP_TRY(launch_batch_ghost(S->cs, L, MMD_F_DF));
-    HIP_TRY(launch_batch_force(S->cs, L, S->halo, S->tab->tf, S->tab->tm, rc2, prec, true));
-    HIP_TRY(launch_batch_kick(S->cs, L, S->tab->masses, 0.0, S->o.dt, false, true));
+    const ListBuf &B = S->prime;
+    HIP_TRY(launch_batch_ghost(S->cs, L, B.g_ext, MMD_F_X));
+    HIP_TRY(launch_batch_rho_df(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, true));
+    HIP_TRY(launch_batch_ghost(S->cs, L, B.g_ext, MMD_F_DF));
+    HIP_TRY(launch_batch_force(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, true));
+    HIP_TRY(launch_batch_kick(S->cs, L, B.g_site, S->tab->masses, 0.0, S->o.dt, false, true));
     HIP_TRY(launch_batch_energy(S->cs, L, 1));
     return fetch_energies(reps);
   }
@@ -545,7 +546,7 @@ struct GpuExec final : Executor {
         if (!ge) {
           hipGraph_t gr;
           HIP_TRY(hipStreamBeginCapture(S->cs, hipStreamCaptureModeThreadLocal));
-          rc = step_passes(S, L, want_e);
+          rc = step_passes(S, L, S->step, want_e);
           hipError_t ee = hipStreamEndCapture(S->cs, &gr);
           if (rc) {
             if (ee == hipSuccess) (void)hipGraphDestroy(gr);
@@ -557,7 +558,7 @@ struct GpuExec final : Executor {
           HIP_TRY(ie);
         }
         HIP_TRY(hipGraphLaunch(ge, S->cs));
-      } else if ((rc = step_passes(S, L, want_e))) {
+      } else if ((rc = step_passes(S, L, S->step, want_e))) {
         return rc;
       }
     }
