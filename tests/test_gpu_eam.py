@@ -75,6 +75,34 @@ def test_rho_df_force_fp64_match_oracle(name):
     assert np.all(out.rho[vac] == 0) and np.all(out.fx[vac] == 0)  # vacancies untouched
 
 
+@pytest.mark.parametrize("prec", [mmd.PREC_FP64, mmd.PREC_MIXED])
+def test_wide_stencil_large_lds_tile(prec):
+    """A 7.5 A stencil (168 entries, reach 3 cells each way) needs a 92 KB
+    LDS tile per force block, beyond the default 64 KB: exercises the opt-in
+    LDS path.  FP64 is held to the oracle bar, mixed to its force bound."""
+    c = W.case(cells=(10, 9, 8), n_types=3, sigma=0.05, temperature=900.0, vacancy_frac=0.01, seed=5, skin=1.9)
+    assert len(c["st3"]) == 168
+    ref, _ = _oracle_pass(c)
+    with _ctx(c, prec) as ctx:
+        ctx.rho_df(RC)
+        ctx.md_setup(c["box"], A, 1.0, RC)
+        ctx.ghost_refresh(mmd.F_DF)
+        ctx.force(RC)
+        out = c["st"].copy()
+        ctx.download(out, mmd.F_RHO | mmd.F_DF | mmd.F_F)
+    li = out.local_index()
+    occ = li[out.type[li] >= 0]
+    F = np.stack([out.fx[occ], out.fy[occ], out.fz[occ]], 1)
+    Fr = np.stack([ref.fx[occ], ref.fy[occ], ref.fz[occ]], 1)
+    if prec == mmd.PREC_FP64:
+        assert np.allclose(out.rho[occ], ref.rho[occ], rtol=1e-12, atol=0)
+        assert np.abs(F - Fr).max() <= 1e-11 * np.abs(Fr).max()
+    else:
+        assert np.allclose(out.rho[occ], ref.rho[occ], rtol=1e-8, atol=0)
+        rms = np.sqrt((Fr ** 2).sum(1).mean())
+        assert np.linalg.norm(F - Fr, axis=1).max() <= 1e-6 * rms
+
+
 def test_standalone_df_matches_oracle():
     c = W.case(**CASES["bcc12_3type_vac"])
     ref, T = _oracle_pass(c)
