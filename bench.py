@@ -2,9 +2,10 @@
 """MISA-MD EAM step throughput on one MI355X.
 
 Headline workload (the FP64 metric): 128x128x128 BCC cells (4,194,304 atoms),
-a = 2.855 A, 3 atom types drawn 90/5/5 from the seed with 9 synthetic pair
-tables (A, B, C jittered +-10%, phi symmetric), 5000-point splines on
-r in [0.5, 5.6] A, Gaussian displacement 0.05 A, Maxwell velocities at 800 K,
+a = 2.855 A, 3 atom types drawn 90/5/5 from the seed, tables of the
+Finnis-Sinclair BCC Fe potential (embedding strength per type and pair term
+per type pair scaled by +-10%, phi symmetric), 5000-point splines on
+r in [0.5, 5.6] A (cutoff 5.6 A for every pass), Gaussian displacement 0.05 A, Maxwell velocities at 800 K,
 velocity Verlet with dt = 1 fs.  Everything is device resident: ghost
 refresh, rho + df, ghost df, force and integration run on the GPU, energies
 are recorded every 10 steps.  One step = one full velocity-Verlet step over
@@ -58,7 +59,7 @@ def make_workload(cells, seed=2024, temperature=800.0, vacancy_frac=0.0, interst
                             interstitial_frac=interstitial_frac, seed=seed)
     cap = int(interstitial_frac * 2 * cells ** 3 * 1.5) + 16
     dom, box, st, itl = mmd.lattice_init(spec, itl_cap=cap)
-    tab = mmd.synth_tables(n_types=3, rc=RC, jitter=0.1, seed=seed)
+    tab = mmd.fs_tables(n_types=3, rc=RC, jitter=0.1, seed=seed)
     st3 = mmd.stencil_build(A, RC + SKIN, 0)
     return dict(dom=dom, box=box, st=st, itl=itl, tab=tab, st3=st3, even=mmd.stencil_linear(dom, st3),
                 odd=mmd.stencil_linear(dom, -st3), wide3=mmd.wide_stencil(A, RC, SKIN))
@@ -250,7 +251,7 @@ def replica_trace(n, seed, rate, t0=0.0, steps=200):
 
 
 def serving(n_replicas, load, seed=77, burst=512, steps=200, use_graph=True):
-    tab = mmd.synth_tables(n_types=3, rc=RC, jitter=0.1, seed=seed)
+    tab = mmd.fs_tables(n_types=3, rc=RC, jitter=0.1, seed=seed)
     with mmd.Server(tab, precision=mmd.PREC_MIXED, a=A, cutoff=RC, skin=SKIN, dt=DT, sigma=0.05, max_replicas=64,
                     hbm_fraction=0.5, chunk_steps=10, workers=16, use_graph=use_graph) as srv:
         # capacity: a burst of the same replica mix, all present at t = 0, measured while the admission queue

@@ -193,6 +193,24 @@ int mmd_energy(mmd_ctx *ctx, double *epot, double *ekin);
 int mmd_itl_set(mmd_ctx *ctx, int32_t n, const double *x, const double *y, const double *z, const double *vx,
                 const double *vy, const double *vz, const int8_t *type, const int32_t *wide3, int32_t n_wide,
                 int32_t cap);
+/* Wigner-Seitz re-assignment between the lattice and the interstitial list of
+ * a single periodic domain (start = 0), after mmd_md_setup and mmd_itl_set.
+ * No counterpart in the reference plugin, which skips off-lattice atoms
+ * (src/kernels/kernel_itl.hpp:44-46, :52-54); it stands in for MISA-MD core's
+ * lattice <-> interstitial bookkeeping between steps.  Rules:
+ *   1. in increasing local site order (z, y, doubled x), a lattice atom whose
+ *      nearest site (its Wigner-Seitz cell; ties to the corner sub-lattice) is
+ *      not its own joins the end of the list with every field; its site
+ *      becomes a vacancy (type -1; v, f, rho, df zeroed; position kept);
+ *   2. a list entry whose nearest site, wrapped into the box, is vacant moves
+ *      into it (position shifted by the periodic image); the lowest list index
+ *      wins a contested site; the entries that moved leave the list, the rest
+ *      keep their order.
+ * Forces, rho and df travel with the atoms, so a run can continue without a
+ * new force pass.  The ghost shell's positions, types and df are refreshed.
+ * MMD_E_RANGE, with nothing changed, when the list would exceed its capacity.
+ * *n_to_itl / *n_to_lattice receive the counts of each transition. */
+int mmd_itl_migrate(mmd_ctx *ctx, int32_t *n_to_itl, int32_t *n_to_lattice);
 /* Copies interstitial fields to the host (any pointer may be NULL). */
 int mmd_itl_get(mmd_ctx *ctx, int32_t *n, double *x, double *y, double *z, double *vx, double *vy, double *vz,
                 double *fx, double *fy, double *fz, double *rho, double *df, int8_t *type);
@@ -327,6 +345,25 @@ typedef struct {
 } mmd_synth_params;
 int mmd_synth_tables(const mmd_synth_params *p, double *rho_r, double *emb, double *phi_r, double *r0, double *dr,
                      double *rho0, double *drho);
+
+/* Finnis-Sinclair tables for BCC Fe (Phil. Mag. A 50 (1984) 45): pair
+ * V(r) = (r-c)^2 (c0 + c1 r + c2 r^2) below c = 3.40 A, density
+ * (r-d)^2 + 1.8 (r-d)^3/d below d = 3.569745 A, F(rho) = -A sqrt(rho); zero
+ * beyond, so every pair inside the passes' cutoff is evaluated while the
+ * lattice stays mechanically stable (a = 2.8665 A, E_coh = 4.28 eV).  For
+ * n_types > 1, A per type and the pair term per unordered type pair are scaled
+ * by 1 +- jitter (seeded).  Knots on r0 + i*dr up to rc, rho = i*drho. */
+typedef struct {
+  int32_t n_types;
+  double rc, jitter;
+  uint64_t seed;
+  int32_t n_r;
+  double r0;
+  int32_t n_rho;
+  double rho_max;
+} mmd_fs_params;
+int mmd_fs_tables(const mmd_fs_params *p, double *rho_r, double *emb, double *phi_r, double *r0, double *dr,
+                  double *rho0, double *drho);
 
 /* EAM tables from a DYNAMO setfl file (LAMMPS eam/alloy format), the on-disk
  * source of the tables hip_pot_init copies (src/hip_arch_acc_imp.cpp:146-150).

@@ -684,8 +684,9 @@ static int check_ready(mmd_ctx *c, const char *fn, bool need_tables = true) {
   if (!c->has_domain) return fail(MMD_E_STATE, "%s: domain not set", fn);
   if (!c->has_stencil) return fail(MMD_E_STATE, "%s: stencil not set", fn);
   if (need_tables && !c->has_tables) return fail(MMD_E_STATE, "%s: tables not set", fn);
-  if (need_tables && c->max_type >= c->n_types)
-    return fail(MMD_E_INVALID, "%s: atom type %d has no table (n_types = %d)", fn, c->max_type, c->n_types);
+  const int32_t mt = std::max(c->max_type, c->n_itl > 0 ? c->itl_max_type : -1);
+  if (need_tables && mt >= c->n_types)
+    return fail(MMD_E_INVALID, "%s: atom type %d has no table (n_types = %d)", fn, mt, c->n_types);
   return MMD_OK;
 }
 

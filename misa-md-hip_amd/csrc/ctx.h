@@ -70,6 +70,7 @@ struct mmd_ctx {
 
   // interstitials (off-lattice atoms), see itl.hip
   int32_t n_itl = 0, itl_cap = 0, n_wide = 0;
+  int32_t itl_max_type = -1;  // largest interstitial type (checked with max_type)
   double *ix = nullptr, *iy = nullptr, *iz = nullptr;
   double *ivx = nullptr, *ivy = nullptr, *ivz = nullptr;
   double *ifx = nullptr, *ify = nullptr, *ifz = nullptr;

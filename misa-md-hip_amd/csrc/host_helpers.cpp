@@ -216,6 +216,47 @@ extern "C" int mmd_synth_tables(const mmd_synth_params *p, double *rho_r, double
   return MMD_OK;
 }
 
+// Finnis-Sinclair EAM for BCC Fe (M. W. Finnis and J. E. Sinclair, Phil. Mag.
+// A 50 (1984) 45, Table 1): pair V(r) = (r-c)^2 (c0 + c1 r + c2 r^2) for
+// r < c, density phi(r) = (r-d)^2 + beta (r-d)^3 / d for r < d, embedding
+// F(rho) = -A sqrt(rho); equilibrium a = 2.8665 A, cohesive energy 4.28 eV.
+extern "C" int mmd_fs_tables(const mmd_fs_params *p, double *rho_r, double *emb, double *phi_r, double *r0,
+                             double *dr, double *rho0, double *drho) {
+  constexpr double d = 3.569745, A = 1.828905, beta = 1.8, c = 3.40;
+  constexpr double c0 = 1.2371147, c1 = -0.3592185, c2 = -0.0385607;
+  if (!p || p->n_types < 1 || p->n_types > 8 || p->n_r < 5 || p->n_rho < 5 || !(p->rc >= d) || !(p->r0 > 0.0) ||
+      !(p->rc > p->r0) || !(p->rho_max > 0.0) || !(p->jitter >= 0.0 && p->jitter < 0.5) || !rho_r || !emb || !phi_r)
+    return fail(MMD_E_INVALID, "mmd_fs_tables: bad argument (the table range must reach d = %g A)", d);
+  const int nt = p->n_types, nr = p->n_r, nrho = p->n_rho;
+  const double delta_r = (p->rc - p->r0) / (double)(nr - 1);
+  const double delta_rho = p->rho_max / (double)(nrho - 1);
+  auto jit = [&](uint64_t id, uint64_t stream) { return 1.0 + p->jitter * (2.0 * rng_u01(p->seed, id, stream) - 1.0); };
+  for (int t = 0; t < nt; t++) {
+    for (int i = 0; i < nr; i++) {
+      const double q = p->r0 + (double)i * delta_r - d;
+      rho_r[(size_t)t * nr + i] = q < 0.0 ? q * q * (1.0 + beta * q / d) : 0.0;
+    }
+    const double At = A * jit((uint64_t)t, 110);
+    for (int i = 0; i < nrho; i++) emb[(size_t)t * nrho + i] = -At * std::sqrt((double)i * delta_rho);
+  }
+  for (int a = 0; a < nt; a++)
+    for (int b = a; b < nt; b++) {
+      const double s = jit((uint64_t)(a * nt + b), 111);
+      for (int i = 0; i < nr; i++) {
+        const double r = p->r0 + (double)i * delta_r;
+        const double q = r - c;
+        const double v = q < 0.0 ? s * q * q * (c0 + c1 * r + c2 * r * r) : 0.0;
+        phi_r[((size_t)a * nt + b) * nr + i] = v;
+        phi_r[((size_t)b * nt + a) * nr + i] = v;
+      }
+    }
+  if (r0) *r0 = p->r0;
+  if (dr) *dr = delta_r;
+  if (rho0) *rho0 = 0.0;
+  if (drho) *drho = delta_rho;
+  return MMD_OK;
+}
+
 // ------------------------------------------------------ initial conditions
 
 static constexpr double kBoltzmann = 8.617333262e-5;  // eV/K

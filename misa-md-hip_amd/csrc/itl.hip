@@ -16,6 +16,7 @@
 // bins filled with atomics and are then sorted, so their order is fixed.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -89,29 +90,45 @@ __device__ __forceinline__ bool li_pair(const Geom &g, const ItlDev &I, const do
   return d2 < I.rc2;
 }
 
+// Nearest BCC site of a position (global lattice coordinates, x doubled): the
+// closer of the nearest corner-sub-lattice and body-sub-lattice sites, a tie
+// going to the corner, i.e. the site whose Wigner-Seitz cell holds the point.
+// The squared distances are written with fma() so that no contraction choice
+// of the compiler can change the decision (the oracle uses the same
+// expression, oracle/eam_oracle.c orc_anchor).
+__device__ __forceinline__ void ws_site(double a, double px, double py, double pz, long long &X, long long &Y,
+                                        long long &Z) {
+  const double ux = px / a, uy = py / a, uz = pz / a;
+  const double cx = floor(ux + 0.5), cy = floor(uy + 0.5), cz = floor(uz + 0.5);
+  const double fx = floor(ux), fy = floor(uy), fz = floor(uz);
+  const double ex = ux - cx, ey = uy - cy, ez = uz - cz;
+  const double gx = ux - (fx + 0.5), gy = uy - (fy + 0.5), gz = uz - (fz + 0.5);
+  const double dc = fma(ex, ex, fma(ey, ey, ez * ez));
+  const double db = fma(gx, gx, fma(gy, gy, gz * gz));
+  if (dc <= db) {
+    X = 2 * (long long)cx, Y = (long long)cy, Z = (long long)cz;
+  } else {
+    X = 2 * (long long)fx + 1, Y = (long long)fy, Z = (long long)fz;
+  }
+}
+
+// floor division image of v in [0, n): v - q n, q returned
+__device__ __forceinline__ long long wrap_img64(long long &v, long long n) {
+  const long long q = v / n - (v % n < 0 ? 1 : 0);
+  v -= q * n;
+  return q;
+}
+
 // anchor (nearest lattice site), wrapped into the box, and bins
 __global__ __launch_bounds__(kBlock) void k_itl_prepare(Geom g, ItlDev I) {
   const int k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= I.n) return;
   double px = I.x[k], py = I.y[k], pz = I.z[k];
-  const double ux = px / I.a, uy = py / I.a, uz = pz / I.a;
-  const double cx = floor(ux + 0.5), cy = floor(uy + 0.5), cz = floor(uz + 0.5);
-  const double bx = floor(ux) + 0.5, by = floor(uy) + 0.5, bz = floor(uz) + 0.5;
-  const double dc = (ux - cx) * (ux - cx) + (uy - cy) * (uy - cy) + (uz - cz) * (uz - cz);
-  const double db = (ux - bx) * (ux - bx) + (uy - by) * (uy - by) + (uz - bz) * (uz - bz);
   long long X, Y, Z;
-  if (dc <= db) {
-    X = 2 * (long long)cx, Y = (long long)cy, Z = (long long)cz;
-  } else {
-    X = 2 * (long long)floor(ux) + 1, Y = (long long)floor(uy), Z = (long long)floor(uz);
-  }
-  const long long nX = 2 * g.bxh, nY = g.by, nZ = g.bz;
-  long long q = X / nX - (X % nX < 0 ? 1 : 0);
-  X -= q * nX, px -= (double)q * I.Lx;
-  q = Y / nY - (Y % nY < 0 ? 1 : 0);
-  Y -= q * nY, py -= (double)q * I.Ly;
-  q = Z / nZ - (Z % nZ < 0 ? 1 : 0);
-  Z -= q * nZ, pz -= (double)q * I.Lz;
+  ws_site(I.a, px, py, pz, X, Y, Z);
+  px = fma(-(double)wrap_img64(X, 2 * (long long)g.bxh), I.Lx, px);
+  py = fma(-(double)wrap_img64(Y, (long long)g.by), I.Ly, py);
+  pz = fma(-(double)wrap_img64(Z, (long long)g.bz), I.Lz, pz);
   I.ax[k] = (int)X, I.ay[k] = (int)Y, I.az[k] = (int)Z;
   I.px[k] = px, I.py[k] = py, I.pz[k] = pz;
   // bin of the wrapped position (positions may sit a little outside [0, L))
@@ -346,6 +363,208 @@ __global__ __launch_bounds__(kBlock) void k_itl_kick(ItlDev I, double *vx, doubl
   }
 }
 
+// ------------------------------------------------------------- migration
+//
+// Wigner-Seitz re-assignment between the lattice and the interstitial list
+// (rules: a lattice atom whose nearest site is not its own joins the list, in
+// increasing local site order, and leaves a vacancy; a list entry whose
+// nearest site is vacant moves into it, the lowest index winning a contested
+// site, and the list is compacted in order).  Every decision is integer work
+// on positions read as stored, so the result is bit-identical to the oracle's
+// sequential restatement.
+
+struct MigDev {
+  Geom g;
+  double4 *P;
+  int8_t *T;
+  double *VX, *VY, *VZ, *FX, *FY, *FZ, *RHO, *DF;   // lattice (ext, device layout)
+  double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz;  // interstitial list
+  double *rho, *df;
+  int8_t *type;
+  int32_t *blk;    // per-block leaver counts, then exclusive offsets
+  int32_t *claim;  // [n_local] lowest list index claiming a vacant site
+  int32_t *cand;   // [cap] claimed local site of each entry (-1: none), then keep flags
+  int32_t *cnt;    // [0] leavers, [1] entries kept
+  int64_t n_local;
+  int32_t n, n_all;
+  double a, Lx, Ly, Lz;
+};
+
+// device index of local site li (reference order: z, y, doubled x)
+__device__ __forceinline__ int64_t local_dev(const Geom &g, int64_t li, int32_t &xl, int32_t &yl, int32_t &zl) {
+  const int64_t nx = 2 * (int64_t)g.bxh;
+  xl = (int32_t)(li % nx);
+  const int64_t r = li / nx;
+  yl = (int32_t)(r % g.by);
+  zl = (int32_t)(r / g.by);
+  return dev_index(g, xl & 1, (xl >> 1) + g.gxh, yl + g.gy, zl + g.gz);
+}
+
+// Does the atom on local site li belong to another site's cell?
+__device__ __forceinline__ bool mig_leaves(const MigDev &M, int64_t li, int64_t &e) {
+  e = -1;
+  if (li >= M.n_local) return false;
+  int32_t xl, yl, zl;
+  e = local_dev(M.g, li, xl, yl, zl);
+  if (M.T[e] < 0) return false;
+  const double4 p = M.P[e];
+  long long X, Y, Z;
+  ws_site(M.a, p.x, p.y, p.z, X, Y, Z);
+  return X != xl || Y != yl || Z != zl;
+}
+
+// exclusive prefix sum of v over the block (kBlock threads); total in tot
+__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t &tot, int32_t *lds) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  int32_t incl = v;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int32_t t = __shfl_up(incl, d, kWave);
+    if (lane >= d) incl += t;
+  }
+  if (lane == kWave - 1) lds[w] = incl;
+  __syncthreads();
+  int32_t off = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < kWavesPerBlock; k++) {
+    const int32_t c = lds[k];
+    off += k < w ? c : 0;
+    all += c;
+  }
+  __syncthreads();
+  tot = all;
+  return off + incl - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mig_count(MigDev M) {
+  __shared__ int32_t lds[kWavesPerBlock];
+  int64_t e;
+  const bool f = mig_leaves(M, (int64_t)blockIdx.x * kBlock + threadIdx.x, e);
+  int32_t tot;
+  (void)block_excl_scan(f ? 1 : 0, tot, lds);
+  if (threadIdx.x == 0) M.blk[blockIdx.x] = tot;
+}
+
+// one block: exclusive scan of v[0..n) in place, total into *total
+__global__ __launch_bounds__(kBlock) void k_mig_scan(int32_t *v, int64_t n, int32_t *total) {
+  __shared__ int32_t lds[kWavesPerBlock];
+  int32_t carry = 0;
+  for (int64_t c0 = 0; c0 < n; c0 += kBlock) {
+    const int64_t i = c0 + threadIdx.x;
+    const int32_t x = i < n ? v[i] : 0;
+    int32_t tot;
+    const int32_t ex = block_excl_scan(x, tot, lds);
+    if (i < n) v[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+// leavers join the list at n + (their rank in site order); their sites empty
+__global__ __launch_bounds__(kBlock) void k_mig_append(MigDev M) {
+  __shared__ int32_t lds[kWavesPerBlock];
+  int64_t e;
+  const bool f = mig_leaves(M, (int64_t)blockIdx.x * kBlock + threadIdx.x, e);
+  int32_t tot;
+  const int32_t r = block_excl_scan(f ? 1 : 0, tot, lds);
+  if (!f) return;
+  const int32_t k = M.n + M.blk[blockIdx.x] + r;
+  double4 p = M.P[e];
+  M.x[k] = p.x, M.y[k] = p.y, M.z[k] = p.z;
+  M.vx[k] = M.VX[e], M.vy[k] = M.VY[e], M.vz[k] = M.VZ[e];
+  M.fx[k] = M.FX[e], M.fy[k] = M.FY[e], M.fz[k] = M.FZ[e];
+  M.rho[k] = M.RHO[e], M.df[k] = M.DF[e];
+  M.type[k] = M.T[e];
+  p.w = -1.0;
+  M.P[e] = p;
+  M.T[e] = -1;
+  M.VX[e] = 0.0, M.VY[e] = 0.0, M.VZ[e] = 0.0;
+  M.FX[e] = 0.0, M.FY[e] = 0.0, M.FZ[e] = 0.0;
+  M.RHO[e] = 0.0, M.DF[e] = 0.0;
+}
+
+// nearest site of entry k wrapped into the box; returns the local index and
+// the periodic image
+__device__ __forceinline__ int64_t mig_target(const MigDev &M, int32_t k, long long &qx, long long &qy,
+                                              long long &qz) {
+  long long X, Y, Z;
+  ws_site(M.a, M.x[k], M.y[k], M.z[k], X, Y, Z);
+  qx = wrap_img64(X, 2 * (long long)M.g.bxh);
+  qy = wrap_img64(Y, (long long)M.g.by);
+  qz = wrap_img64(Z, (long long)M.g.bz);
+  return (Z * M.g.by + Y) * (2 * (long long)M.g.bxh) + X;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mig_claim(MigDev M) {
+  const int32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= M.n_all) return;
+  long long qx, qy, qz;
+  const int64_t li = mig_target(M, k, qx, qy, qz);
+  int32_t xl, yl, zl;
+  const int64_t e = local_dev(M.g, li, xl, yl, zl);
+  if (M.T[e] < 0) {
+    M.cand[k] = (int32_t)li;
+    atomicMin(&M.claim[li], k);
+  } else {
+    M.cand[k] = -1;
+  }
+}
+
+// winners move into their site; cand[k] becomes the keep flag
+__global__ __launch_bounds__(kBlock) void k_mig_fill(MigDev M) {
+  const int32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= M.n_all) return;
+  const int32_t li = M.cand[k];
+  if (li < 0 || M.claim[li] != k) {
+    M.cand[k] = 1;
+    return;
+  }
+  long long qx, qy, qz;
+  (void)mig_target(M, k, qx, qy, qz);
+  int32_t xl, yl, zl;
+  const int64_t e = local_dev(M.g, li, xl, yl, zl);
+  const int8_t t = M.type[k];
+  M.P[e] = make_double4(fma(-(double)qx, M.Lx, M.x[k]), fma(-(double)qy, M.Ly, M.y[k]),
+                        fma(-(double)qz, M.Lz, M.z[k]), (double)t);
+  M.T[e] = t;
+  M.VX[e] = M.vx[k], M.VY[e] = M.vy[k], M.VZ[e] = M.vz[k];
+  M.FX[e] = M.fx[k], M.FY[e] = M.fy[k], M.FZ[e] = M.fz[k];
+  M.RHO[e] = M.rho[k], M.DF[e] = M.df[k];
+  M.cand[k] = 0;
+}
+
+// one block: in-place, order-preserving compaction of the list by the keep
+// flags.  Each chunk is staged through LDS, so all of its loads have landed
+// before any store of the chunk (stores go to indices <= the chunk's own).
+__global__ __launch_bounds__(kBlock) void k_mig_compact(MigDev M) {
+  constexpr int kF = 11;
+  __shared__ double vals[kF][kBlock];
+  __shared__ int8_t typ[kBlock];
+  __shared__ int32_t lds[kWavesPerBlock];
+  double *const arr[kF] = {M.x, M.y, M.z, M.vx, M.vy, M.vz, M.fx, M.fy, M.fz, M.rho, M.df};
+  int32_t base = 0;
+  for (int32_t c0 = 0; c0 < M.n_all; c0 += kBlock) {
+    const int32_t k = c0 + threadIdx.x;
+    const int32_t keep = k < M.n_all ? M.cand[k] : 0;
+    if (keep) {
+#pragma unroll
+      for (int f = 0; f < kF; f++) vals[f][threadIdx.x] = arr[f][k];
+      typ[threadIdx.x] = M.type[k];
+    }
+    int32_t tot;
+    const int32_t r = block_excl_scan(keep, tot, lds);  // its barriers order the loads above before the stores
+    if (keep) {
+      const int32_t d = base + r;
+#pragma unroll
+      for (int f = 0; f < kF; f++) arr[f][d] = vals[f][threadIdx.x];
+      M.type[d] = typ[threadIdx.x];
+    }
+    base += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) M.cnt[1] = base;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------- host
@@ -363,6 +582,10 @@ struct ItlState {
   int32_t *bins = nullptr;  // counts followed by lists
   int32_t *err = nullptr;
   int32_t nbx = 0, nby = 0, nbz = 0, max_d2q = 0;
+  // migration work arrays
+  int32_t *mig_blk = nullptr, *mig_claim = nullptr, *mig_cand = nullptr, *mig_cnt = nullptr;
+  int64_t mig_nblk = 0, mig_nlocal = 0;
+  int32_t mig_cap = 0;
 };
 
 static ItlState &state(mmd_ctx *c) {
@@ -385,6 +608,7 @@ void itl_free(mmd_ctx *c) {
     ItlState &s = state(c);
     dfree_t(s.ax), dfree_t(s.ay), dfree_t(s.az), dfree_t(s.nbr), dfree_t(s.nnb);
     dfree_t(s.px), dfree_t(s.py), dfree_t(s.pz), dfree_t(s.bins), dfree_t(s.err);
+    dfree_t(s.mig_blk), dfree_t(s.mig_claim), dfree_t(s.mig_cand), dfree_t(s.mig_cnt);
     delete static_cast<ItlState *>(c->itl_state);
     c->itl_state = nullptr;
   }
@@ -552,9 +776,12 @@ extern "C" int mmd_itl_set(mmd_ctx *c, int32_t n, const double *x, const double 
     const int s1 = dX & 1;
     max_d2q = std::max(max_d2q, dX * dX + (2 * dY + s1) * (2 * dY + s1) + (2 * dZ + s1) * (2 * dZ + s1));
   }
-  for (int32_t k = 0; k < n; k++)
+  int32_t max_type = -1;
+  for (int32_t k = 0; k < n; k++) {
     if (type[k] < 0 || type[k] >= kMaxTypes || (c->has_tables && type[k] >= c->n_types))
       return fail(MMD_E_INVALID, "mmd_itl_set: interstitial %d has type %d", k, type[k]);
+    max_type = std::max<int32_t>(max_type, type[k]);
+  }
   ITL_TRY(hipStreamSynchronize(c->stream));
   ctx_drop_graphs(c);
   const bool realloc = cap > c->itl_cap || n_wide > c->n_wide || !c->d_wide;
@@ -583,6 +810,7 @@ extern "C" int mmd_itl_set(mmd_ctx *c, int32_t n, const double *x, const double 
   s.max_d2q = max_d2q;
   c->n_wide = n_wide;
   c->n_itl = n;
+  c->itl_max_type = max_type;
   c->itl_stale = true;
   if (n_wide > 0) ITL_TRY(hipMemcpy(c->d_wide, wide3, (size_t)n_wide * 3 * sizeof(int32_t), hipMemcpyHostToDevice));
   ITL_TRY(launch_fill_i32(c->stream, s.err, 1, 0));
@@ -625,5 +853,90 @@ extern "C" int mmd_itl_get(mmd_ctx *c, int32_t *n, double *x, double *y, double 
     if (p.h) ITL_TRY(hipMemcpyAsync(p.h, p.d, b, hipMemcpyDeviceToHost, c->stream));
   if (type) ITL_TRY(hipMemcpyAsync(type, c->itype, (size_t)c->n_itl, hipMemcpyDeviceToHost, c->stream));
   ITL_TRY(hipStreamSynchronize(c->stream));
+  return MMD_OK;
+}
+
+extern "C" int mmd_itl_migrate(mmd_ctx *c, int32_t *n_to_itl, int32_t *n_to_lattice) {
+  int rc = ctx_bind(c);
+  if (rc) return rc;
+  if (!c->has_domain || !c->md_ready)
+    return fail(MMD_E_STATE, "mmd_itl_migrate: needs the domain and mmd_md_setup (box, lattice constant)");
+  if (c->itl_cap < 1 || !c->d_wide || !c->itl_state)
+    return fail(MMD_E_STATE, "mmd_itl_migrate: call mmd_itl_set first (list capacity and wide stencil)");
+  if (c->dom.start[0] || c->dom.start[1] || c->dom.start[2])
+    return fail(MMD_E_STATE, "mmd_itl_migrate: a single periodic domain (start = 0) only");
+  const Geom &g = c->g;
+  const int64_t n_local = 2 * (int64_t)g.bxh * g.by * g.bz;
+  if (n_local >= ((int64_t)1 << 31)) return fail(MMD_E_RANGE, "mmd_itl_migrate: more than 2^31 local sites");
+  const int64_t nblk = (n_local + kBlock - 1) / kBlock;
+  ItlState &s = state(c);
+  if (s.mig_nlocal != n_local || s.mig_cap != c->itl_cap) {
+    ITL_TRY(hipStreamSynchronize(c->stream));
+    dfree_t(s.mig_blk), dfree_t(s.mig_claim), dfree_t(s.mig_cand), dfree_t(s.mig_cnt);
+    s.mig_nlocal = s.mig_nblk = 0, s.mig_cap = 0;
+    ITL_TRY(hipMalloc((void **)&s.mig_blk, (size_t)nblk * sizeof(int32_t)));
+    ITL_TRY(hipMalloc((void **)&s.mig_claim, (size_t)n_local * sizeof(int32_t)));
+    ITL_TRY(hipMalloc((void **)&s.mig_cand, (size_t)c->itl_cap * sizeof(int32_t)));
+    ITL_TRY(hipMalloc((void **)&s.mig_cnt, 2 * sizeof(int32_t)));
+    s.mig_nlocal = n_local, s.mig_nblk = nblk, s.mig_cap = c->itl_cap;
+  }
+  MigDev M{};
+  M.g = g;
+  M.P = c->pos, M.T = c->type;
+  M.VX = c->vx, M.VY = c->vy, M.VZ = c->vz, M.FX = c->fx, M.FY = c->fy, M.FZ = c->fz;
+  M.RHO = c->rho, M.DF = c->df;
+  M.x = c->ix, M.y = c->iy, M.z = c->iz, M.vx = c->ivx, M.vy = c->ivy, M.vz = c->ivz;
+  M.fx = c->ifx, M.fy = c->ify, M.fz = c->ifz, M.rho = c->irho, M.df = c->idf, M.type = c->itype;
+  M.blk = s.mig_blk, M.claim = s.mig_claim, M.cand = s.mig_cand, M.cnt = s.mig_cnt;
+  M.n_local = n_local;
+  M.n = c->n_itl;
+  M.a = c->md.lattice_const;
+  M.Lx = c->md.box_len[0], M.Ly = c->md.box_len[1], M.Lz = c->md.box_len[2];
+  // phase 1: count the leavers (site order), check the capacity before any change
+  k_mig_count<<<dim3((unsigned)nblk), kBlock, 0, c->stream>>>(M);
+  ITL_TRY(hipGetLastError());
+  k_mig_scan<<<1, kBlock, 0, c->stream>>>(s.mig_blk, nblk, s.mig_cnt);
+  ITL_TRY(hipGetLastError());
+  int32_t leave = 0;
+  ITL_TRY(hipMemcpyAsync(&leave, s.mig_cnt, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  ITL_TRY(hipStreamSynchronize(c->stream));
+  if ((int64_t)c->n_itl + leave > c->itl_cap)
+    return fail(MMD_E_RANGE, "mmd_itl_migrate: %d interstitials plus %d leaving the lattice exceed the capacity %d",
+                c->n_itl, leave, c->itl_cap);
+  if (leave > 0) {
+    k_mig_append<<<dim3((unsigned)nblk), kBlock, 0, c->stream>>>(M);
+    ITL_TRY(hipGetLastError());
+  }
+  // phase 2: entries whose nearest site is vacant fill it, lowest index first
+  M.n_all = c->n_itl + leave;
+  int32_t kept = M.n_all;
+  if (M.n_all > 0) {
+    const dim3 grid((unsigned)((M.n_all + kBlock - 1) / kBlock));
+    ITL_TRY(launch_fill_i32(c->stream, s.mig_claim, n_local, INT32_MAX));
+    k_mig_claim<<<grid, kBlock, 0, c->stream>>>(M);
+    ITL_TRY(hipGetLastError());
+    k_mig_fill<<<grid, kBlock, 0, c->stream>>>(M);
+    ITL_TRY(hipGetLastError());
+    k_mig_compact<<<1, kBlock, 0, c->stream>>>(M);
+    ITL_TRY(hipGetLastError());
+    ITL_TRY(hipMemcpyAsync(&kept, s.mig_cnt + 1, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  }
+  ITL_TRY(launch_ghost(c->stream, g, M.Lx, M.Ly, M.Lz, c->pos, c->type, c->df, MMD_F_X | MMD_F_DF));
+  ITL_TRY(hipStreamSynchronize(c->stream));
+  const int32_t filled = M.n_all - kept;
+  if (kept != c->n_itl) {
+    ctx_drop_graphs(c);  // launch grids follow the list length
+    // partials of blocks beyond the new grids would otherwise stay in the energy sums
+    ITL_TRY(launch_fill_f64(c->stream, c->e_itl, c->itl_part_n + c->itl_kick_n, 0.0));
+    ITL_TRY(hipStreamSynchronize(c->stream));
+  }
+  if (leave > 0 || filled > 0) {
+    const int32_t m = std::max(c->max_type, c->itl_max_type);
+    c->max_type = c->itl_max_type = m;
+  }
+  c->n_itl = kept;
+  c->itl_stale = true;
+  if (n_to_itl) *n_to_itl = leave;
+  if (n_to_lattice) *n_to_lattice = filled;
   return MMD_OK;
 }

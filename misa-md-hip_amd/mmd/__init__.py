@@ -70,6 +70,11 @@ class SynthParams(C.Structure):
                 ("r0", C.c_double), ("n_rho", C.c_int32), ("rho_max", C.c_double)]
 
 
+class FsParams(C.Structure):
+    _fields_ = [("n_types", C.c_int32), ("rc", C.c_double), ("jitter", C.c_double), ("seed", C.c_uint64),
+                ("n_r", C.c_int32), ("r0", C.c_double), ("n_rho", C.c_int32), ("rho_max", C.c_double)]
+
+
 class LatticeSpec(C.Structure):
     _fields_ = [("cells", C.c_int32 * 3), ("ghost_cells", C.c_int32), ("lattice_const", C.c_double),
                 ("n_types", C.c_int32), ("type_cum", C.c_double * 8), ("mass", C.c_double * 8),
@@ -156,6 +161,7 @@ def lib():
         "mmd_energy": (C.c_int, [P, _d, _d]),
         "mmd_itl_set": (C.c_int, [P, C.c_int32, _d, _d, _d, _d, _d, _d, _i8, _i32, C.c_int32, C.c_int32]),
         "mmd_itl_get": (C.c_int, [P, _i32, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _i8]),
+        "mmd_itl_migrate": (C.c_int, [P, _i32, _i32]),
         "mmd_timing_enable": (C.c_int, [P, C.c_int32]),
         "mmd_timing_get": (C.c_int, [P, _d, _i64]),
         "mmd_timing_reset": (C.c_int, [P]),
@@ -163,6 +169,7 @@ def lib():
         "mmd_stencil_build": (C.c_int32, [C.c_double, C.c_double, C.c_int32, _i32, C.c_int32]),
         "mmd_stencil_linear": (C.c_int, [C.POINTER(Domain), _i32, C.c_int32, _i64]),
         "mmd_synth_tables": (C.c_int, [C.POINTER(SynthParams), _d, _d, _d, _d, _d, _d, _d]),
+        "mmd_fs_tables": (C.c_int, [C.POINTER(FsParams), _d, _d, _d, _d, _d, _d, _d]),
         "mmd_lattice_domain": (C.c_int, [C.POINTER(LatticeSpec), C.POINTER(Domain), _d]),
         "mmd_lattice_init": (C.c_int, [C.POINTER(LatticeSpec), C.POINTER(HostSoa), C.c_int32, _i32,
                                        _d, _d, _d, _d, _d, _d, _i8]),
@@ -226,6 +233,18 @@ def synth_tables(n_types=1, rc=5.6, A=1.2, B=0.3, Cc=0.01, jitter=0.0, seed=1, n
     phi = np.zeros((n_types, n_types, n_r))
     vals = [C.c_double() for _ in range(4)]
     _check(lib().mmd_synth_tables(C.byref(p), ptr(rho_r), ptr(emb), ptr(phi), *[C.byref(v) for v in vals]))
+    return dict(n_types=n_types, n_r=n_r, r0=vals[0].value, dr=vals[1].value, n_rho=n_rho,
+                rho0=vals[2].value, drho=vals[3].value, rho_r=rho_r, emb=emb, phi_r=phi, rc=rc)
+
+
+def fs_tables(n_types=1, rc=5.6, jitter=0.0, seed=1, n_r=5000, r0=0.5, n_rho=5000, rho_max=30.0):
+    """Finnis-Sinclair BCC Fe tables (mmd_fs_tables) in the form synth_tables returns."""
+    p = FsParams(n_types, rc, jitter, seed, n_r, r0, n_rho, rho_max)
+    rho_r = np.zeros((n_types, n_r))
+    emb = np.zeros((n_types, n_rho))
+    phi = np.zeros((n_types, n_types, n_r))
+    vals = [C.c_double() for _ in range(4)]
+    _check(lib().mmd_fs_tables(C.byref(p), ptr(rho_r), ptr(emb), ptr(phi), *[C.byref(v) for v in vals]))
     return dict(n_types=n_types, n_r=n_r, r0=vals[0].value, dr=vals[1].value, n_rho=n_rho,
                 rho0=vals[2].value, drho=vals[3].value, rho_r=rho_r, emb=emb, phi_r=phi, rc=rc)
 
@@ -449,6 +468,13 @@ class Context:
         _check(lib().mmd_itl_get(self.h, C.byref(n), *[ptr(getattr(o, f)) for f in Interstitials.FIELDS],
                                  ptr(o.type, C.c_int8)))
         return o.take(n.value)
+
+    def itl_migrate(self):
+        """Wigner-Seitz re-assignment (mmd_itl_migrate); returns the counts
+        (lattice -> list, list -> lattice)."""
+        a, b = C.c_int32(0), C.c_int32(0)
+        _check(lib().mmd_itl_migrate(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def timing(self, on=True):
         _check(lib().mmd_timing_enable(self.h, 1 if on else 0))

@@ -406,20 +406,25 @@ void orc_md_run(const orc_domain *d, const int64_t *off_even, int32_t n_even, co
 
 void orc_anchor(double a, double px, double py, double pz, int64_t *X, int64_t *Y, int64_t *Z) {
   /* corner sub-lattice: (2i, j, k) at (i, j, k) a; body sub-lattice:
-   * (2i+1, j, k) at (i+1/2, j+1/2, k+1/2) a.  Pick the closer candidate. */
+   * (2i+1, j, k) at (i+1/2, j+1/2, k+1/2) a.  Pick the closer candidate; a tie
+   * goes to the corner.  The squared distances are written with fma() so the
+   * product library's device code, which uses the same expression, reaches
+   * the same decision bit for bit (the choice is integer bookkeeping). */
   const double ux = px / a, uy = py / a, uz = pz / a;
   const double cx = floor(ux + 0.5), cy = floor(uy + 0.5), cz = floor(uz + 0.5);
-  const double bx = floor(ux) + 0.5, by = floor(uy) + 0.5, bz = floor(uz) + 0.5;
-  const double dc = (ux - cx) * (ux - cx) + (uy - cy) * (uy - cy) + (uz - cz) * (uz - cz);
-  const double db = (ux - bx) * (ux - bx) + (uy - by) * (uy - by) + (uz - bz) * (uz - bz);
+  const double fx = floor(ux), fy = floor(uy), fz = floor(uz);
+  const double ex = ux - cx, ey = uy - cy, ez = uz - cz;
+  const double gx = ux - (fx + 0.5), gy = uy - (fy + 0.5), gz = uz - (fz + 0.5);
+  const double dc = fma(ex, ex, fma(ey, ey, ez * ez));
+  const double db = fma(gx, gx, fma(gy, gy, gz * gz));
   if (dc <= db) {
     *X = 2 * (int64_t)cx;
     *Y = (int64_t)cy;
     *Z = (int64_t)cz;
   } else {
-    *X = 2 * (int64_t)floor(ux) + 1;
-    *Y = (int64_t)floor(uy);
-    *Z = (int64_t)floor(uz);
+    *X = 2 * (int64_t)fx + 1;
+    *Y = (int64_t)fy;
+    *Z = (int64_t)fz;
   }
 }
 
@@ -600,4 +605,92 @@ void orc_md_run_itl(const orc_domain *d, const int64_t *off_even, int32_t n_even
       energies[2 * s + 1] = orc_kinetic(d, t, vx, vy, vz, type) + ke_itl * ORC_MVV2E;
     }
   }
+}
+
+/* ------------------------------------------------- interstitial migration */
+
+/* Wigner-Seitz re-assignment between the lattice and the interstitial list.
+ * MISA-MD core keeps off-lattice atoms apart from the lattice array (the
+ * plugin skips them, kernel_itl.hpp:44-46, :52-54); the core's own transition
+ * rules are not in the reference tree, so the rules are stated here:
+ *   1. in increasing local site order (z, then y, then doubled x), a lattice
+ *      atom whose nearest site (orc_anchor) is not its own site is appended to
+ *      the list with every field; its site becomes a vacancy (type -1; v, f,
+ *      rho, df zeroed; position kept);
+ *   2. in list order, an entry whose nearest site (wrapped into the periodic
+ *      box) is vacant moves into it, its position shifted by the periodic
+ *      image; visiting in increasing order means the lowest index wins a site
+ *      several entries claim.  The entries that moved leave the list; the rest
+ *      keep their order.
+ * Then the ghost shell's positions, types and df are refreshed.  A single
+ * periodic domain (start = 0) only.  Returns 0; -1 (nothing changed) when the
+ * list would exceed cap; -2 for a domain with start != 0. */
+int32_t orc_migrate(const orc_domain *d, double a, const double box_len[3], double *x, double *y, double *z,
+                    double *vx, double *vy, double *vz, double *fx, double *fy, double *fz, double *rho, double *df,
+                    int8_t *type, orc_itl *itl, int32_t cap, int32_t *n_to_itl, int32_t *n_to_lattice) {
+  if (d->start[0] || d->start[1] || d->start[2]) return -2;
+  const int64_t nX = d->box[0], nY = d->box[1], nZ = d->box[2];
+  int64_t leave = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    int32_t n = itl->n;
+    for (int64_t zl = 0; zl < nZ; zl++)
+      for (int64_t yl = 0; yl < nY; yl++)
+        for (int64_t xl = 0; xl < nX; xl++) {
+          const int64_t i = orc_lin(d, xl + d->ghost[0], yl + d->ghost[1], zl + d->ghost[2]);
+          if (type[i] < 0) continue;
+          int64_t X, Y, Z;
+          orc_anchor(a, x[i], y[i], z[i], &X, &Y, &Z);
+          if (X == xl && Y == yl && Z == zl) continue;
+          if (pass == 0) {
+            leave++;
+            continue;
+          }
+          const int32_t k = n++;
+          itl->x[k] = x[i], itl->y[k] = y[i], itl->z[k] = z[i];
+          itl->vx[k] = vx[i], itl->vy[k] = vy[i], itl->vz[k] = vz[i];
+          itl->fx[k] = fx[i], itl->fy[k] = fy[i], itl->fz[k] = fz[i];
+          itl->rho[k] = rho[i], itl->df[k] = df[i];
+          itl->type[k] = type[i];
+          type[i] = -1;
+          vx[i] = vy[i] = vz[i] = 0.0;
+          fx[i] = fy[i] = fz[i] = 0.0;
+          rho[i] = df[i] = 0.0;
+        }
+    if (pass == 0 && (int64_t)itl->n + leave > cap) return -1;
+    if (pass == 1) itl->n = n;
+  }
+  int32_t kept = 0, filled = 0;
+  for (int32_t k = 0; k < itl->n; k++) {
+    int64_t X, Y, Z, ix, iy, iz;
+    orc_anchor(a, itl->x[k], itl->y[k], itl->z[k], &X, &Y, &Z);
+    X = orc_wrap(X, nX, &ix);
+    Y = orc_wrap(Y, nY, &iy);
+    Z = orc_wrap(Z, nZ, &iz);
+    const int64_t i = orc_lin(d, X + d->ghost[0], Y + d->ghost[1], Z + d->ghost[2]);
+    if (type[i] < 0) {
+      x[i] = fma(-(double)ix, box_len[0], itl->x[k]);
+      y[i] = fma(-(double)iy, box_len[1], itl->y[k]);
+      z[i] = fma(-(double)iz, box_len[2], itl->z[k]);
+      vx[i] = itl->vx[k], vy[i] = itl->vy[k], vz[i] = itl->vz[k];
+      fx[i] = itl->fx[k], fy[i] = itl->fy[k], fz[i] = itl->fz[k];
+      rho[i] = itl->rho[k], df[i] = itl->df[k];
+      type[i] = itl->type[k];
+      filled++;
+      continue;
+    }
+    if (kept != k) {
+      itl->x[kept] = itl->x[k], itl->y[kept] = itl->y[k], itl->z[kept] = itl->z[k];
+      itl->vx[kept] = itl->vx[k], itl->vy[kept] = itl->vy[k], itl->vz[kept] = itl->vz[k];
+      itl->fx[kept] = itl->fx[k], itl->fy[kept] = itl->fy[k], itl->fz[kept] = itl->fz[k];
+      itl->rho[kept] = itl->rho[k], itl->df[kept] = itl->df[k];
+      itl->type[kept] = itl->type[k];
+    }
+    kept++;
+  }
+  itl->n = kept;
+  *n_to_itl = (int32_t)leave;
+  *n_to_lattice = filled;
+  orc_ghost_periodic_xt(d, box_len, x, y, z, type);
+  orc_ghost_periodic_df(d, df);
+  return 0;
 }

@@ -151,6 +151,13 @@ void orc_md_run_itl(const orc_domain *d, const int64_t *off_even, int32_t n_even
                     double *vy, double *vz, const int8_t *type, double *rho, double *df, double *fx, double *fy,
                     double *fz, orc_itl *itl, double *energies, int32_t nthreads);
 
+/* Wigner-Seitz migration between the lattice and the interstitial list of a
+ * single periodic domain (rules in eam_oracle.c); itl arrays hold cap
+ * entries.  Returns 0, -1 over capacity (nothing changed), -2 for start != 0. */
+int32_t orc_migrate(const orc_domain *d, double a, const double box_len[3], double *x, double *y, double *z,
+                    double *vx, double *vy, double *vz, double *fx, double *fy, double *fz, double *rho, double *df,
+                    int8_t *type, orc_itl *itl, int32_t cap, int32_t *n_to_itl, int32_t *n_to_lattice);
+
 #ifdef __cplusplus
 }
 #endif

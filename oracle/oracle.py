@@ -79,6 +79,8 @@ def lib():
         "orc_md_run_itl": (None, [D, _i64, C.c_int32, _i64, C.c_int32, _i32, C.c_int32, C.c_double, T, C.c_double,
                                   _d, C.c_double, C.c_int32, _d, _d, _d, _d, _d, _d, _i8, _d, _d, _d, _d, _d,
                                   C.POINTER(OrcItl), _d, C.c_int32]),
+        "orc_migrate": (C.c_int32, [D, C.c_double, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _i8,
+                                    C.POINTER(OrcItl), C.c_int32, _i32, _i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -241,3 +243,24 @@ def md_run_itl(dom, even, odd, wide3, a, tabs, rc, box_len, dt, nsteps, st, itl,
                          _p(st.z), _p(st.vx), _p(st.vy), _p(st.vz), _p(st.type, C.c_int8), _p(st.rho), _p(st.df),
                          _p(st.fx), _p(st.fy), _p(st.fz), C.byref(s), _p(en) if energies else None, nthreads)
     return en
+
+
+def migrate(dom, a, box_len, st, itl, cap):
+    """Wigner-Seitz migration (orc_migrate) on the host state st (changed in
+    place) and the list itl; returns (new list, leavers, fills).  Raises
+    OverflowError, with nothing changed, when the list would exceed cap."""
+    work = type(itl)(max(cap, 1))
+    for f in ("x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "rho", "df", "type"):
+        getattr(work, f)[: itl.n] = getattr(itl, f)[: itl.n]
+    s = _itl_struct(work)
+    s.n = itl.n
+    L = np.ascontiguousarray(box_len, float)
+    out, fil = C.c_int32(0), C.c_int32(0)
+    rc = lib().orc_migrate(C.byref(domain(dom)), a, _p(L), _p(st.x), _p(st.y), _p(st.z), _p(st.vx), _p(st.vy),
+                           _p(st.vz), _p(st.fx), _p(st.fy), _p(st.fz), _p(st.rho), _p(st.df),
+                           _p(st.type, C.c_int8), C.byref(s), cap, C.byref(out), C.byref(fil))
+    if rc == -1:
+        raise OverflowError("interstitial list would exceed its capacity")
+    if rc:
+        raise ValueError("orc_migrate: domain with start != 0")
+    return work.take(s.n), out.value, fil.value

@@ -22,7 +22,7 @@ TYPE_FRAC = (0.9, 0.05, 0.05)
 
 @pytest.fixture(scope="module")
 def tab():
-    return mmd.synth_tables(n_types=3, rc=RC, jitter=0.1, seed=11)
+    return mmd.fs_tables(n_types=3, rc=RC, jitter=0.1, seed=11)
 
 
 def _server(tab, prec, **kw):

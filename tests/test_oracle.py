@@ -135,6 +135,7 @@ def _run_oracle(c, want_e=False):
 
 def test_perfect_lattice_closed_form_and_zero_force():
     c = W.case(cells=(5, 5, 5), sigma=0.0, temperature=0.0)
+    c["tab"] = mmd.synth_tables(rc=RC)  # phi = 1.2 (rc-r)^2 (1 + 0.3/r), rho = (rc-r)^2 e^-r, F = -sqrt + 0.01 rho^2
     _run_oracle(c)
     st = c["st"]
     li = st.local_index()
@@ -144,6 +145,31 @@ def test_perfect_lattice_closed_form_and_zero_force():
     df_cf = -0.5 / np.sqrt(rho_cf) + 0.02 * rho_cf
     assert np.allclose(st.df[li], df_cf, rtol=1e-7)
     assert np.abs(np.stack([st.fx[li], st.fy[li], st.fz[li]])).max() < 1e-12
+
+
+def test_finnis_sinclair_fe_cohesive_energy():
+    """The Finnis-Sinclair Fe tables reproduce the published fit (Phil. Mag. A
+    50 (1984) 45): cohesive energy 4.28 eV at a = 2.8665 A, a closed-form
+    density over the two shells inside d, zero force on the perfect lattice,
+    and a minimum of the energy per atom at that lattice constant."""
+    d, beta = 3.569745, 1.8
+    e = {}
+    for a in (2.8465, 2.8665, 2.8865):
+        spec = mmd.lattice_spec((5, 5, 5), ghost_cells=3, a=a, n_types=1, sigma=0.0, temperature=0.0, seed=1)
+        dom, box, st, _ = mmd.lattice_init(spec)
+        st3 = mmd.stencil_build(a, RC + W.SKIN, 0)
+        c = dict(dom=dom, st=st, tab=mmd.fs_tables(rc=RC), masses=mmd.DEFAULT_MASSES,
+                 even=mmd.stencil_linear(dom, st3), odd=mmd.stencil_linear(dom, -st3))
+        _, E = _run_oracle(c, want_e=True)
+        li = st.local_index()
+        e[a] = E / li.size
+        if a == 2.8665:
+            q = np.array([np.sqrt(3) / 2 * a, a]) - d
+            rho_cf = (np.array([8, 6]) * q * q * (1 + beta * q / d)).sum()
+            assert np.allclose(st.rho[li], rho_cf, rtol=1e-12, atol=0)
+            assert np.abs(np.stack([st.fx[li], st.fy[li], st.fz[li]])).max() < 1e-11
+    assert abs(e[2.8665] + 4.28) < 2e-3
+    assert e[2.8665] < e[2.8465] and e[2.8665] < e[2.8865]
 
 
 @pytest.mark.parametrize("n_types,vac", [(1, 0.0), (3, 0.05)])

@@ -19,8 +19,8 @@ def case(cells=(6, 6, 6), n_types=1, sigma=0.03, temperature=600.0, vacancy_frac
                             n_blobs=n_blobs, blob_sigma=blob_sigma, seed=seed)
     cap = int(interstitial_frac * 2 * np.prod(cells)) + 16
     dom, box, st, itl = mmd.lattice_init(spec, itl_cap=cap)
-    tab = mmd.synth_tables(n_types=n_types, rc=RC, jitter=(0.1 if n_types > 1 else 0.0) if jitter is None else jitter,
-                           seed=seed, n_r=n_r, n_rho=n_rho)
+    tab = mmd.fs_tables(n_types=n_types, rc=RC, jitter=(0.1 if n_types > 1 else 0.0) if jitter is None else jitter,
+                        seed=seed, n_r=n_r, n_rho=n_rho)
     st3 = mmd.stencil_build(A, RC + skin, 0)
     even = mmd.stencil_linear(dom, st3)
     odd = mmd.stencil_linear(dom, -st3)
