@@ -88,7 +88,7 @@ def pairs_in_cutoff(w, sample=20000, seed=0):
     return tot / len(idx)
 
 
-def open_ctx(w, prec, energy_every=ENERGY_EVERY, state=None, itl=None):
+def open_ctx(w, prec, energy_every=ENERGY_EVERY, state=None, itl=None, cap=None):
     ctx = mmd.Context(0)
     ctx.set_domain(w["dom"])
     ctx.set_stencil(w["even"], w["odd"])
@@ -98,7 +98,7 @@ def open_ctx(w, prec, energy_every=ENERGY_EVERY, state=None, itl=None):
     ctx.md_setup(w["box"], A, DT, RC, energy_every=energy_every)
     it = itl if itl is not None else w["itl"]
     if it.n:
-        ctx.itl_set(it, w["wide3"], cap=max(it.n, 1))
+        ctx.itl_set(it, w["wide3"], cap=cap or max(it.n, 1))
     ctx.md_prime()
     return ctx
 
@@ -187,6 +187,7 @@ def mixed_vs_fp64(w, steps, warmup):
     f_ref_end = forces(ctx, w)
     ctx.close()
     nw1, pa1 = force_err(rmx["f_end"], f_ref_end)
+    mig = migration(w, rmx["state"], rmx["itl"])
     e64, emx = r64["energies"].sum(1), rmx["energies"].sum(1)
     rec = np.arange(ENERGY_EVERY - 1, steps, ENERGY_EVERY)
     d64 = e64[rec] - (r64["e0"][0] + r64["e0"][1])
@@ -201,7 +202,25 @@ def mixed_vs_fp64(w, steps, warmup):
             "energy_diff_per_atom_per_step_max": float(per_step.max()) if len(per_step) else None,
             "force_kernel_ms": round(mk[mmd.K_FORCE] / max(mn[mmd.K_FORCE], 1), 4),
             "rho_df_kernel_ms": round(mk[mmd.K_RHO] / max(mn[mmd.K_RHO], 1), 4),
-            "interstitial_ms_per_step": round(mk[mmd.K_ITL] / steps, 4)}
+            "interstitial_ms_per_step": round(mk[mmd.K_ITL] / steps, 4),
+            "migration": mig}
+
+
+def migration(w, state, itl, reps=5):
+    """Wigner-Seitz migration (mmd_itl_migrate) on the state a run ended in:
+    wall time of the first call (the transitions since the start) and of calls
+    with nothing left to move (two full-lattice passes plus a host sync)."""
+    ctx = open_ctx(w, mmd.PREC_MIXED, state=state, itl=itl, cap=itl.n + 65536)
+    ctx.sync()
+    t0 = time.perf_counter()
+    out, fill = ctx.itl_migrate()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        ctx.itl_migrate()
+    t2 = time.perf_counter()
+    ctx.close()
+    return {"to_interstitial": out, "to_lattice": fill, "first_call_ms": round((t1 - t0) * 1e3, 3),
+            "steady_call_ms": round((t2 - t1) * 1e3 / reps, 3)}
 
 
 def dropin_hooks(w, reps=3):

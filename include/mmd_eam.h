@@ -120,6 +120,12 @@ int mmd_set_stencil(mmd_ctx *ctx, const int64_t *even, int32_t n_even, const int
  * tables (and the mixed-precision r^2-keyed tables) on the device. */
 int mmd_set_tables(mmd_ctx *ctx, const mmd_eam_tables *tab);
 int mmd_set_precision(mmd_ctx *ctx, int32_t mode);
+/* Where MMD_PREC_MIXED evaluates pairs from its r^2-keyed tables, as squared
+ * distances: out[0..1] = [head, tail), out[2..5] = two interior holes
+ * [lo, hi) (empty as [0, 0)); pairs elsewhere use the FP64 spline.  No
+ * counterpart in the reference (FP64 only); for callers that check how much
+ * of their workload the fast path covers. */
+int mmd_mixed_ranges(mmd_ctx *ctx, double out[6]);
 
 /* ------------------------------------------------------------ data movement */
 

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <utility>
 #include <vector>
 
 #include "mmd_eam.h"
@@ -466,7 +467,7 @@ static SplHost pack_f64(Arena &A, const double *knots, int nfun, int n, double d
 
 struct MixHost {
   size_t vb, vc, dp;
-  double s_lo, s_hi;  // range where the table meets the accuracy bound
+  std::vector<char> ok;  // per segment: every function of the family meets the accuracy bounds
 };
 
 // 16-byte derivative record: FP64 base, FP32 linear and quadratic terms
@@ -481,8 +482,48 @@ static_assert(sizeof(DerivRec) == 16, "derivative record must be one 16-byte loa
 // phi' 2e-8 (the force on an atom cancels ~50x over its ~60 neighbours, so
 // the per-atom force error stays well inside 1e-6); rho' enters the force
 // multiplied by F'(rho), so its bound is 2e-8 / max|F'|.
+// Each bound is relative to |f| + kMixedFloor * max|f| (max over the table):
+// near a zero of a function or of its derivative the relative bound alone
+// would ask more than FP32 correction terms resolve (their absolute error
+// there is ~1e-10), while the absolute error the floor admits per pair is
+// orders below the per-atom force target.
 constexpr double kMixedTolValue = 1e-9;
 constexpr double kMixedTolForce = 2e-8;
+constexpr double kMixedFloor = 3e-2;
+
+// Where the mixed tables are used: the segments from the first to the last
+// good one, minus at most kMixedHoles interior runs of failing segments (runs
+// separated by the fewest good segments are merged until that many remain;
+// the merged good segments then use the FP64 spline too).
+static void mixed_ranges(const std::vector<char> &ok_a, const std::vector<char> &ok_b, int n_s, double s0, double ds,
+                         TabMixed &tm) {
+  auto good = [&](int m) { return ok_a[(size_t)m] && ok_b[(size_t)m]; };
+  int lo = 0, hi = n_s - 1;
+  while (lo < hi && !good(lo)) lo++;
+  while (hi > lo && !good(hi - 1)) hi--;
+  std::vector<std::pair<int, int>> runs;  // [first, last + 1) failing runs inside [lo, hi)
+  for (int m = lo; m < hi; m++) {
+    if (good(m)) continue;
+    if (!runs.empty() && runs.back().second == m)
+      runs.back().second = m + 1;
+    else
+      runs.push_back({m, m + 1});
+  }
+  while ((int)runs.size() > kMixedHoles) {
+    size_t best = 0;
+    for (size_t k = 1; k + 1 < runs.size(); k++)
+      if (runs[k + 1].first - runs[k].second < runs[best + 1].first - runs[best].second) best = k;
+    runs[best].second = runs[best + 1].second;
+    runs.erase(runs.begin() + (long)best + 1);
+  }
+  tm.s_head = s0 + ds * (double)lo;
+  tm.s_tail = s0 + ds * (double)hi;
+  for (int k = 0; k < kMixedHoles; k++) {
+    const bool used = k < (int)runs.size();
+    tm.hole_lo[k] = used ? s0 + ds * (double)runs[(size_t)k].first : 0.0;
+    tm.hole_hi[k] = used ? s0 + ds * (double)runs[(size_t)k].second : 0.0;
+  }
+}
 
 // Mixed-precision r^2-keyed family resampled from the FP64 r-keyed spline:
 // knot values and exact derivatives of g(sqrt(s)) at uniform s, cubic Hermite
@@ -519,11 +560,12 @@ static MixHost pack_mixed(Arena &A, const double *knots, int nfun, int n_r, doub
     }
   }
   // Dense check of every segment against the FP64 spline (the same FP32
-  // rounding as the device); the table is used only on the contiguous range of
-  // segments around the middle that meet the bounds, the rest falls back to
-  // the FP64 r-spline.  Near r -> 0 the uniform s grid is coarse in r and the
-  // end segments of the r-spline carry low-order kinks.
+  // rounding as the device).  Segments that miss the bounds fall back to the
+  // FP64 r-spline (mixed_ranges): near r -> 0 the uniform s grid is coarse in
+  // r, the end segments of the r-spline carry low-order kinks, and a function
+  // that is only C1 at an interior point has a kink there.
   std::vector<char> ok((size_t)n_s, 1);
+  ok[(size_t)n_s - 1] = 0;  // the last knot starts no segment
   for (int f = 0; f < nfun; f++) {
     spline7(knots + (size_t)f * n_r, n_r, dr, c7.data());
     double hmax = 0.0, gmax = 0.0;
@@ -541,19 +583,15 @@ static MixHost pack_mixed(Arena &A, const double *knots, int nfun, int n_r, doub
         const float c = std::fma((float)q, vc[2 * rec + 1], vc[2 * rec]);
         const double h = std::fma(q, std::fma(q, (double)c, vb[2 * rec + 1]), vb[2 * rec]);
         const double g = std::fma(q, std::fma(q, (double)dp[rec].b2, (double)dp[rec].b1), dp[rec].base);
-        if (std::fabs(h - h_ex) > kMixedTolValue * (std::fabs(h_ex) + 1e-3 * hmax) ||
-            std::fabs(g - g_ex) > tol_deriv * (std::fabs(g_ex) + 1e-3 * gmax))
+        if (std::fabs(h - h_ex) > kMixedTolValue * (std::fabs(h_ex) + kMixedFloor * hmax) ||
+            std::fabs(g - g_ex) > tol_deriv * (std::fabs(g_ex) + kMixedFloor * gmax))
           ok[(size_t)m] = 0;
       }
     }
   }
-  int lo = n_s / 2, hi = n_s / 2;
-  while (lo > 0 && ok[(size_t)lo - 1]) lo--;
-  while (hi < n_s - 1 && ok[(size_t)hi]) hi++;
   if (dp_out) *dp_out = dp;
   MixHost o;
-  o.s_lo = s0 + ds * (double)lo;
-  o.s_hi = s0 + ds * (double)hi;
+  o.ok = std::move(ok);
   o.vb = A.put(vb.data(), vb.size() * 8);
   o.vc = A.put(vc.data(), vc.size() * 4);
   o.dp = A.put(dp.data(), dp.size() * sizeof(DerivRec));
@@ -658,8 +696,7 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
     tm.rho = M(m_rho), tm.phi = M(m_phi);
     tm.fb = reinterpret_cast<const double2 *>(dev + h_fb);
     tm.nt = nt, tm.n_s = n_s, tm.s0 = s0, tm.inv_ds = 1.0 / ds;
-    tm.s_head = std::max(m_rho.s_lo, m_phi.s_lo);
-    tm.s_tail = std::min(m_rho.s_hi, m_phi.s_hi);
+    mixed_ranges(m_rho.ok, m_phi.ok, n_s, s0, ds, tm);
     for (int a = 0; a < kMaxTypes; a++) {
       c->masses.m[a] = a < nt ? t->mass[a] : 1.0;
       c->masses.inv_m[a] = 1.0 / c->masses.m[a];
@@ -677,6 +714,16 @@ extern "C" int mmd_set_precision(mmd_ctx *c, int32_t mode) {
   if (mode != MMD_PREC_FP64 && mode != MMD_PREC_MIXED) return fail(MMD_E_INVALID, "mmd_set_precision: mode %d", mode);
   if (c->prec != mode) ctx_drop_graphs(c);
   c->prec = mode;
+  return MMD_OK;
+}
+
+extern "C" int mmd_mixed_ranges(mmd_ctx *c, double out[6]) {
+  int rc = ctx_bind(c);
+  if (rc) return rc;
+  if (!out) return fail(MMD_E_INVALID, "mmd_mixed_ranges: null out");
+  if (!c->has_tables) return fail(MMD_E_STATE, "mmd_mixed_ranges: tables not set");
+  out[0] = c->tm.s_head, out[1] = c->tm.s_tail;
+  for (int k = 0; k < kMixedHoles; k++) out[2 + 2 * k] = c->tm.hole_lo[k], out[3 + 2 * k] = c->tm.hole_hi[k];
   return MMD_OK;
 }
 

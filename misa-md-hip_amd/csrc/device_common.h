@@ -115,11 +115,22 @@ struct TabMixed {
   const double2 *fb;  // [nt*nt][n_s][2]
   int32_t nt, n_s;
   double s0, inv_ds;
-  // The reference spline's first and last three segments use low-order end
-  // slopes, which leave second-derivative kinks a uniform s grid cannot follow:
-  // pairs with s < s_head or s >= s_tail evaluate the FP64 r-spline instead.
+  // Segments that miss the accuracy bounds evaluate the FP64 r-spline instead:
+  // those below s_head or from s_tail on (the reference spline's low-order end
+  // slopes leave kinks a uniform s grid cannot follow) and those in up to
+  // kMixedHoles interior holes [hole_lo, hole_hi) (a tabulated function that
+  // is only C1 somewhere, e.g. where a potential term reaches its own cutoff
+  // inside the table).  Unused holes are empty ([0, 0)).
   double s_head, s_tail;
+  double hole_lo[2], hole_hi[2];
 };
+constexpr int kMixedHoles = 2;
+
+// Is a pair at squared distance s evaluated from the mixed tables?
+__device__ __forceinline__ bool mix_in(const TabMixed &tm, double s) {
+  return s >= tm.s_head && s < tm.s_tail && !(s >= tm.hole_lo[0] && s < tm.hole_hi[0]) &&
+         !(s >= tm.hole_lo[1] && s < tm.hole_hi[1]);
+}
 
 __device__ __forceinline__ double mix_value(const SplMixed &s, uint32_t rec, double q) {
   const double2 b = ld(s.vb, rec);

@@ -109,7 +109,7 @@ __device__ __forceinline__ double fpair_f64(const TabF64 &tf, int ti, int tj, do
 }
 
 // Phased evaluation of a trip (mixed): d/ds tables, -(dpsi/dr)/r = -2 dpsi/ds,
-// no square root or divide; ok[u] = pair inside the tables' checked range.
+// no square root or divide; ok[u] = pair inside the tables' checked ranges.
 template <bool WANT_E, int U>
 __device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, int ti, const int *tj, const double *d2,
                                                  double dfi, const double *dfj, double *fp, double *phi, bool *ok) {
@@ -117,7 +117,7 @@ __device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, int ti, con
   uint32_t rp[U], ra[U];
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    ok[u] = d2[u] < tm.s_tail && d2[u] >= tm.s_head;
+    ok[u] = mix_in(tm, d2[u]);
     const SegS sg = seg_s(d2[u], tm);
     q[u] = sg.q;
     rp[u] = (uint32_t)((ti * tm.nt + tj[u]) * tm.n_s + sg.m);
@@ -156,7 +156,7 @@ __device__ __forceinline__ void trip_rho_mixed(const TabMixed &tm, const int *tj
   uint32_t rec[U];
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    ok[u] = d2[u] < tm.s_tail && d2[u] >= tm.s_head;
+    ok[u] = mix_in(tm, d2[u]);
     const SegS sg = seg_s(d2[u], tm);
     q[u] = sg.q;
     rec[u] = (uint32_t)(tj[u] * tm.n_s + sg.m);
@@ -300,7 +300,7 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
 #pragma unroll
           for (int u = 0; u < kUnroll; u++) {
             const bool in = d2[u] >= 0.0;
-            if (in && !(d2[u] < tm.s_tail && d2[u] >= tm.s_head)) v[u] = rho_f64(tf, tj[u], d2[u]);
+            if (in && !mix_in(tm, d2[u])) v[u] = rho_f64(tf, tj[u], d2[u]);
           }
         }
 #pragma unroll
@@ -393,7 +393,7 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
 #pragma unroll
           for (int u = 0; u < kUnroll; u++) {
             const bool in = d2[u] >= 0.0;
-            if (in && !(d2[u] < tm.s_tail && d2[u] >= tm.s_head))
+            if (in && !mix_in(tm, d2[u]))
               fp[u] = fpair_f64<WANT_E>(tf, t0, tj[u], d2[u], dfi, dfj[u], phi[u]);
           }
         }

@@ -162,6 +162,7 @@ def lib():
         "mmd_itl_set": (C.c_int, [P, C.c_int32, _d, _d, _d, _d, _d, _d, _i8, _i32, C.c_int32, C.c_int32]),
         "mmd_itl_get": (C.c_int, [P, _i32, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _i8]),
         "mmd_itl_migrate": (C.c_int, [P, _i32, _i32]),
+        "mmd_mixed_ranges": (C.c_int, [P, _d]),
         "mmd_timing_enable": (C.c_int, [P, C.c_int32]),
         "mmd_timing_get": (C.c_int, [P, _d, _i64]),
         "mmd_timing_reset": (C.c_int, [P]),
@@ -468,6 +469,12 @@ class Context:
         _check(lib().mmd_itl_get(self.h, C.byref(n), *[ptr(getattr(o, f)) for f in Interstitials.FIELDS],
                                  ptr(o.type, C.c_int8)))
         return o.take(n.value)
+
+    def mixed_ranges(self):
+        """Squared-distance ranges the mixed tables cover: (head, tail), [(lo, hi), (lo, hi)] holes."""
+        o = np.zeros(6)
+        _check(lib().mmd_mixed_ranges(self.h, ptr(o)))
+        return (o[0], o[1]), [(o[2], o[3]), (o[4], o[5])]
 
     def itl_migrate(self):
         """Wigner-Seitz re-assignment (mmd_itl_migrate); returns the counts

@@ -120,6 +120,8 @@ def test_standalone_df_matches_oracle():
 
 def test_perfect_lattice_on_device():
     c = W.case(cells=(16, 16, 16), sigma=0.0, temperature=0.0)
+    # the synthetic forms are non-zero out to rc, so every shell of the stencil counts
+    c["tab"] = mmd.synth_tables(rc=RC)
     with _ctx(c) as ctx:
         ctx.rho_df(RC)
         ctx.md_setup(c["box"], A, 1.0, RC)
@@ -272,6 +274,25 @@ def _force_err(F, Fr):
     return np.linalg.norm(F - Fr) / np.linalg.norm(Fr), d.max() / rms
 
 
+def _fallback_fraction(c, ranges):
+    """Share of the in-cutoff pairs (stencil pairs between occupied local
+    sites and their neighbours) whose squared distance lies outside the mixed
+    tables' ranges."""
+    st = c["st"]
+    li = st.local_index()
+    i = li[st.type[li] >= 0]
+    par = ((i % st.dom.ext[0]) - st.dom.ghost[0]) & 1
+    offs = np.where(par[:, None] == 0, c["even"][None, :], c["odd"][None, :])
+    j = i[:, None] + offs
+    d2 = (st.x[i, None] - st.x[j]) ** 2 + (st.y[i, None] - st.y[j]) ** 2 + (st.z[i, None] - st.z[j]) ** 2
+    s = d2[(d2 < RC * RC) & (st.type[j] >= 0)]
+    (head, tail), holes = ranges
+    inside = (s >= head) & (s < tail)
+    for lo, hi in holes:
+        inside &= ~((s >= lo) & (s < hi))
+    return 1.0 - inside.mean()
+
+
 @pytest.mark.parametrize("name", ["bcc12_3type_vac", "ragged_7x9x5"])
 def test_mixed_precision_within_target(name):
     c = W.case(**CASES[name])
@@ -285,14 +306,17 @@ def test_mixed_precision_within_target(name):
             o = c["st"].copy()
             ctx.download(o, mmd.F_RHO | mmd.F_DF | mmd.F_F)
             res[prec] = (o, ctx.energy())
+            ranges = ctx.mixed_ranges()
     a, b = res[mmd.PREC_FP64][0], res[mmd.PREC_MIXED][0]
     li = a.local_index()
     occ = li[a.type[li] >= 0]
+    # the fast path covers the pairs: at most 2% of in-cutoff pairs fall back to FP64
+    assert _fallback_fraction(c, ranges) < 0.02, ranges
     assert np.allclose(b.rho[occ], a.rho[occ], rtol=1e-9)
     Fa = np.stack([a.fx[occ], a.fy[occ], a.fz[occ]], 1)
     Fb = np.stack([b.fx[occ], b.fy[occ], b.fz[occ]], 1)
     normwise, per_atom = _force_err(Fb, Fa)
-    assert normwise <= 1e-6 and per_atom <= 1e-6, (normwise, per_atom)
+    assert 0.0 < normwise <= 1e-6 and per_atom <= 1e-6, (normwise, per_atom)
     (ea, _), (eb, _) = res[mmd.PREC_FP64][1], res[mmd.PREC_MIXED][1]
     assert abs(ea - eb) / len(occ) < 1e-9
 
