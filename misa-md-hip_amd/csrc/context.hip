@@ -439,27 +439,25 @@ struct Arena {
 };
 
 struct SplHost {
-  size_t v34, v56, d01, d2;
+  size_t val, d01, d2;
 };
 
 // FP64 spline family: spline7 coefficients split by width (see SplF64).
 static SplHost pack_f64(Arena &A, const double *knots, int nfun, int n, double dx) {
-  std::vector<double> c7((size_t)n * 7), v34((size_t)nfun * n * 2), v56(v34.size()), d01(v34.size()),
+  std::vector<double> c7((size_t)n * 7), val((size_t)nfun * n * 4), d01((size_t)nfun * n * 2),
       d2((size_t)nfun * n);
   for (int f = 0; f < nfun; f++) {
     spline7(knots + (size_t)f * n, n, dx, c7.data());
     for (int m = 0; m < n; m++) {
       const double *k = &c7[(size_t)m * 7];
       const size_t r = (size_t)f * n + m;
-      v34[2 * r] = k[3], v34[2 * r + 1] = k[4];
-      v56[2 * r] = k[5], v56[2 * r + 1] = k[6];
+      val[4 * r] = k[3], val[4 * r + 1] = k[4], val[4 * r + 2] = k[5], val[4 * r + 3] = k[6];
       d01[2 * r] = k[0], d01[2 * r + 1] = k[1];
       d2[r] = k[2];
     }
   }
   SplHost h;
-  h.v34 = A.put(v34.data(), v34.size() * 8);
-  h.v56 = A.put(v56.data(), v56.size() * 8);
+  h.val = A.put(val.data(), val.size() * 8);
   h.d01 = A.put(d01.data(), d01.size() * 8);
   h.d2 = A.put(d2.data(), d2.size() * 8);
   return h;
@@ -674,8 +672,7 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
     HIP_TRY(hipMemcpy(dev, A.buf.data(), A.buf.size(), hipMemcpyHostToDevice));
     auto F = [&](const SplHost &h) {
       SplF64 s;
-      s.v34 = reinterpret_cast<const double2 *>(dev + h.v34);
-      s.v56 = reinterpret_cast<const double2 *>(dev + h.v56);
+      s.val = reinterpret_cast<const double2 *>(dev + h.val);
       s.d01 = reinterpret_cast<const double2 *>(dev + h.d01);
       s.d2 = reinterpret_cast<const double *>(dev + h.d2);
       return s;

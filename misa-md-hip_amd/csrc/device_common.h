@@ -63,11 +63,13 @@ __device__ __forceinline__ T ld(const T *__restrict__ base, uint32_t idx) {
 
 // FP64 cubic spline of one function family (all types or type pairs back to
 // back, record = function * n + knot).  The seven coefficients per knot are
-// split by width so a lane's gather touches as few cache lines as possible:
-//   value = ((c3 p + c4) p + c5) p + c6   from v34 = (c3, c4), v56 = (c5, c6)
+// split into a value record and derivative arrays, so a lane's gather touches
+// as few cache lines as possible:
+//   value = ((c3 p + c4) p + c5) p + c6   from val[2r] = (c3, c4), val[2r+1] = (c5, c6)
 //   deriv = (c0 p + c1) p + c2            from d01 = (c0, c1), d2 = c2
+// (the value record's two 16-byte halves share a cache line)
 struct SplF64 {
-  const double2 *v34, *v56, *d01;
+  const double2 *val, *d01;
   const double *d2;
 };
 
@@ -84,7 +86,7 @@ struct TabF64 {
 };
 
 __device__ __forceinline__ double spl_value(const SplF64 &s, uint32_t rec, double p) {
-  const double2 a = ld(s.v34, rec), b = ld(s.v56, rec);
+  const double2 a = ld(s.val, 2 * rec), b = ld(s.val, 2 * rec + 1);
   return ((a.x * p + a.y) * p + b.x) * p + b.y;
 }
 

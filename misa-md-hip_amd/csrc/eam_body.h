@@ -190,8 +190,8 @@ __device__ __forceinline__ void trip_rho_f64(const TabF64 &tf, const int *tj, co
   double2 a[U], b[U];
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    a[u] = ld(tf.rho.v34, rec[u]);
-    b[u] = ld(tf.rho.v56, rec[u]);
+    a[u] = ld(tf.rho.val, 2 * rec[u]);
+    b[u] = ld(tf.rho.val, 2 * rec[u] + 1);
   }
 #pragma unroll
   for (int u = 0; u < U; u++) v[u] = ((a[u].x * p[u] + a[u].y) * p[u] + b[u].x) * p[u] + b[u].y;

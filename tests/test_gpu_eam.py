@@ -387,3 +387,23 @@ def test_mixed_precision_close_contacts():
     Fb = np.stack([b.fx[occ], b.fy[occ], b.fz[occ]], 1)
     normwise, per_atom = _force_err(Fb, Fa)
     assert normwise <= 1e-6 and per_atom <= 1e-6, (normwise, per_atom)
+
+
+@pytest.mark.parametrize("prec", [mmd.PREC_FP64, mmd.PREC_MIXED])
+def test_golden_fs_lattices_on_device(prec):
+    """Closed-form golden energies and densities (tests/golden/fs_bcc_lattice.json:
+    direct analytic shell sums of the Finnis-Sinclair Fe functions) through the
+    C ABI: FP64 to rounding, mixed within its 1e-9 value bound."""
+    tol = 1e-11 if prec == mmd.PREC_FP64 else 1e-9
+    for case in W.golden_fs_cases():
+        c = W.strained_lattice(case["lattice_const"], case["strain"])
+        with _ctx(c, prec=prec) as ctx:
+            ctx.md_setup(c["box"], A, 1.0, RC)
+            ep, _ = ctx.energy()
+            out = c["st"].copy()
+            ctx.download(out, mmd.F_RHO | mmd.F_F)
+        li = out.local_index()
+        n = li.size
+        assert np.allclose(out.rho[li], case["rho"], rtol=tol, atol=0), case["name"]
+        assert abs(ep / n - case["energy"]) < tol * abs(case["energy"]), (case["name"], ep / n)
+        assert np.abs(np.stack([out.fx[li], out.fy[li], out.fz[li]])).max() < 1e3 * tol, case["name"]

@@ -123,3 +123,25 @@ def brute_force(pos, types, L, nt_tab, rc):
     F = (d * fpair[:, :, None]).sum(1)
     E = Femb.sum() + 0.5 * np.where(mask, phiv, 0.0).sum()
     return rho, df, F, E
+
+
+def golden_fs_cases():
+    """Closed-form Finnis-Sinclair lattice energies (tests/golden/fs_bcc_lattice.json)."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fs_bcc_lattice.json")) as f:
+        return json.load(f)["cases"]
+
+
+def strained_lattice(a, strain, cells=(6, 6, 6)):
+    """Perfect single-type BCC lattice of constant a, every ext position and the
+    box scaled by (1 + strain) per axis, with Finnis-Sinclair tables."""
+    spec = mmd.lattice_spec(cells, ghost_cells=3, a=a, n_types=1, sigma=0.0, temperature=0.0, seed=1)
+    dom, box, st, itl = mmd.lattice_init(spec)
+    s = 1.0 + np.asarray(strain, float)
+    st.x *= s[0]
+    st.y *= s[1]
+    st.z *= s[2]
+    st3 = mmd.stencil_build(a, RC + SKIN, 0)
+    return dict(dom=dom, box=box * s, st=st, itl=itl, tab=mmd.fs_tables(rc=RC), st3=st3,
+                even=mmd.stencil_linear(dom, st3), odd=mmd.stencil_linear(dom, -st3), masses=mmd.DEFAULT_MASSES)
