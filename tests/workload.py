@@ -145,3 +145,48 @@ def strained_lattice(a, strain, cells=(6, 6, 6)):
     st3 = mmd.stencil_build(a, RC + SKIN, 0)
     return dict(dom=dom, box=box * s, st=st, itl=itl, tab=mmd.fs_tables(rc=RC), st3=st3,
                 even=mmd.stencil_linear(dom, st3), odd=mmd.stencil_linear(dom, -st3), masses=mmd.DEFAULT_MASSES)
+
+
+def write_fs_setfl(path, n_types=2, jitter=0.1, seed=3, nr=3000, dr=0.002, nrho=3000, drho=0.01):
+    """A DYNAMO setfl file of the Finnis-Sinclair Fe functions (embedding
+    strength per type and pair term per type pair scaled by 1 +- jitter),
+    values from the analytic forms; r * phi is tabulated, as the format asks."""
+    d, a_fs, beta, c = 3.569745, 1.828905, 1.8, 3.40
+    c0, c1, c2 = 1.2371147, -0.3592185, -0.0385607
+    rng = np.random.default_rng(seed)
+    amp = 1.0 + jitter * rng.uniform(-1, 1, n_types)
+    spair = 1.0 + jitter * rng.uniform(-1, 1, (n_types, n_types))
+    r = np.arange(nr) * dr
+    rho = np.arange(nrho) * drho
+    q = r - d
+    dens = np.where(q < 0, q * q * (1 + beta * q / d), 0.0)
+    p = r - c
+    pair = np.where(p < 0, p * p * (c0 + c1 * r + c2 * r * r), 0.0)
+    lines = ["Finnis-Sinclair Fe (Phil. Mag. A 50 (1984) 45)", "test fixture", "scaled per type",
+             f"{n_types} " + " ".join(f"E{t}" for t in range(n_types)), f"{nrho} {drho!r} {nr} {dr!r} {RC!r}"]
+
+    def emit(v):
+        for k in range(0, len(v), 5):
+            lines.append(" ".join("%.17g" % x for x in v[k:k + 5]))
+
+    for t in range(n_types):
+        lines.append(f"{26 + t} {mmd.DEFAULT_MASSES[t]!r} 2.855 bcc")
+        emit(-a_fs * amp[t] * np.sqrt(rho))
+        emit(dens)
+    for i in range(n_types):
+        for j in range(i + 1):
+            emit(r * pair * 0.5 * (spair[i, j] + spair[j, i]))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def ghost_sources(dom):
+    """Reference-order index of each ext record's periodic source (-1 for local records)."""
+    ex, ey, ez = dom.ext
+    gx, gy, gz = dom.ghost
+    bx, by, bz = dom.box
+    z, y, x = np.meshgrid(np.arange(ez), np.arange(ey), np.arange(ex), indexing="ij")
+    sx, sy, sz = (x - gx) % bx + gx, (y - gy) % by + gy, (z - gz) % bz + gz
+    src = ((sz * ey + sy) * ex + sx).ravel().astype(np.int64)
+    idx = np.arange(src.size)
+    return np.where(src == idx, -1, src)
