@@ -148,42 +148,70 @@ __device__ __forceinline__ int bin_of(const ItlDev &I, double p, double L, doubl
   return min((int)(w / bs), nb - 1);
 }
 
-// sorted list of interstitials closer than 2 rc (minimum image)
+// exclusive prefix sum of v over the wave; total in tot
+__device__ __forceinline__ int32_t wave_excl_scan(int32_t v, int32_t &tot) {
+  const int lane = threadIdx.x & (kWave - 1);
+  int32_t incl = v;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int32_t t = __shfl_up(incl, d, kWave);
+    if (lane >= d) incl += t;
+  }
+  tot = __shfl(incl, kWave - 1, kWave);
+  return incl - v;
+}
+
+// Sorted list of the interstitials closer than 2 rc (minimum image): one wave
+// per interstitial, a lane per neighbouring bin; candidates are gathered in
+// LDS and placed by rank (ids are distinct), so the list is in increasing id
+// order.  The 27 bins are those of the offsets -1..1 per axis, fewer along an
+// axis with fewer than 3 bins so that no bin is visited twice.
 __global__ __launch_bounds__(kBlock) void k_itl_nbr(ItlDev I) {
-  const int k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= I.n) return;
-  const int bx = bin_of(I, I.px[k], I.Lx, I.bsx, I.nbx), by = bin_of(I, I.py[k], I.Ly, I.bsy, I.nby),
-            bz = bin_of(I, I.pz[k], I.Lz, I.bsz, I.nbz);
-  int list[kItlNbrCap];
-  int cnt = 0;
-  const int rx0 = I.nbx >= 3 ? -1 : 0, rx1 = I.nbx >= 2 ? 1 : 0;
-  const int ry0 = I.nby >= 3 ? -1 : 0, ry1 = I.nby >= 2 ? 1 : 0;
-  const int rz0 = I.nbz >= 3 ? -1 : 0, rz1 = I.nbz >= 2 ? 1 : 0;
-  for (int oz = rz0; oz <= rz1; oz++)
-    for (int oy = ry0; oy <= ry1; oy++)
-      for (int ox = rx0; ox <= rx1; ox++) {
-        const int b = (wrap_n(bz + oz, I.nbz) * I.nby + wrap_n(by + oy, I.nby)) * I.nbx + wrap_n(bx + ox, I.nbx);
-        const int nb = min(I.bin_count[b], kBinCap);
-        for (int q = 0; q < nb; q++) {
-          const int m = I.bin_list[b * kBinCap + q];
-          if (m == k) continue;
-          const double dx = minimg(I.px[k] - I.px[m], I.Lx), dy = minimg(I.py[k] - I.py[m], I.Ly),
-                       dz = minimg(I.pz[k] - I.pz[m], I.Lz);
-          if (dx * dx + dy * dy + dz * dz >= I.rc2x4) continue;
-          if (cnt == kItlNbrCap) {
-            atomicOr(I.err, 2);
-            continue;
-          }
-          int pos = cnt++;
-          while (pos > 0 && list[pos - 1] > m) {
-            list[pos] = list[pos - 1];
-            pos--;
-          }
-          list[pos] = m;
-        }
-      }
-  I.nnb[k] = cnt;
-  for (int q = 0; q < cnt; q++) I.nbr[k * kItlNbrCap + q] = list[q];
+  __shared__ int32_t cand[kWavesPerBlock][kItlNbrCap];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int k = blockIdx.x * kWavesPerBlock + w;
+  const bool active = k < I.n;
+  int32_t found = 0, b = 0, nb = 0;
+  double pkx = 0.0, pky = 0.0, pkz = 0.0;
+  if (active && lane < 27) {
+    pkx = I.px[k], pky = I.py[k], pkz = I.pz[k];
+    const int ox = lane % 3 - 1, oy = (lane / 3) % 3 - 1, oz = lane / 9 - 1;
+    const bool vx = I.nbx >= 3 || (ox >= 0 && ox < I.nbx), vy = I.nby >= 3 || (oy >= 0 && oy < I.nby),
+               vz = I.nbz >= 3 || (oz >= 0 && oz < I.nbz);
+    if (vx && vy && vz) {
+      const int bx = bin_of(I, pkx, I.Lx, I.bsx, I.nbx), by = bin_of(I, pky, I.Ly, I.bsy, I.nby),
+                bz = bin_of(I, pkz, I.Lz, I.bsz, I.nbz);
+      b = (wrap_n(bz + oz, I.nbz) * I.nby + wrap_n(by + oy, I.nby)) * I.nbx + wrap_n(bx + ox, I.nbx);
+      nb = min(I.bin_count[b], kBinCap);
+    }
+  }
+  // pass 1: count this lane's candidates; pass 2: store them at the wave offset
+  for (int pass = 0; pass < 2; pass++) {
+    int32_t off = 0, tot = 0;
+    if (pass == 1) {
+      off = wave_excl_scan(found, tot);
+      if (active && lane == 0 && tot > kItlNbrCap) atomicOr(I.err, 2);
+    }
+    int32_t c = 0;
+    for (int q = 0; q < nb; q++) {
+      const int m = I.bin_list[b * kBinCap + q];
+      if (m == k) continue;
+      const double dx = minimg(pkx - I.px[m], I.Lx), dy = minimg(pky - I.py[m], I.Ly), dz = minimg(pkz - I.pz[m], I.Lz);
+      if (dx * dx + dy * dy + dz * dz >= I.rc2x4) continue;
+      if (pass == 1 && off + c < kItlNbrCap) cand[w][off + c] = m;
+      c++;
+    }
+    if (pass == 0) found = c;
+    if (pass == 1) found = min(tot, kItlNbrCap);
+  }
+  __syncthreads();
+  if (active && lane < found) {
+    const int32_t v = cand[w][lane];
+    int32_t rank = 0;
+    for (int q = 0; q < found; q++) rank += cand[w][q] < v ? 1 : 0;
+    I.nbr[k * kItlNbrCap + rank] = v;
+  }
+  if (active && lane == 0) I.nnb[k] = found;
 }
 
 // local lattice coordinates (x doubled) of the periodic source of ext site (xe, ye, ze)
@@ -225,14 +253,17 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
       const double r = sqrt(d2);
       double p;
       const int m = seg(r, tf.r0, tf.inv_dr, tf.n_r, p);
+      // this pair's term, reused when the owner adds interstitial k to the site
+      double v_k = 0.0, fp_k = 0.0;
       if (MODE == 0) {
-        acc += spl_value(tf.rho, ti * tf.n_r + m, p);
+        v_k = spl_value(tf.rho, ti * tf.n_r + m, p);
+        acc += v_k;
       } else {
         const double dfi = DF[e];
         const double psip = dfi * spl_deriv(tf.rho, tk * tf.n_r + m, p) + dfk * spl_deriv(tf.rho, ti * tf.n_r + m, p) +
                             spl_deriv(tf.phi, (ti * tf.nt + tk) * tf.n_r + m, p);
-        const double fp = -psip / r;  // force on the lattice atom along del = x_i - x_k
-        fxk -= dx * fp, fyk -= dy * fp, fzk -= dz * fp;
+        fp_k = -psip / r;  // force on the lattice atom along del = x_i - x_k
+        fxk -= dx * fp_k, fyk -= dy * fp_k, fzk -= dz * fp_k;
       }
       // owner: no lower-numbered neighbour interacts with this lattice site
       int sx, sy, sz;
@@ -259,6 +290,15 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
           placed_k = 1;
         } else {
           mm = nb[q - placed_k];
+        }
+        if (mm == k) {  // the pair evaluated above: same image, same distance
+          if (MODE == 0) {
+            rho_i += tk == ti ? v_k : spl_value(tf.rho, tk * tf.n_r + m, p);
+          } else {
+            gx += dx * fp_k, gy += dy * fp_k, gz += dz * fp_k;
+            if (WANT_E) epair += spl_value(tf.phi, (ti * tf.nt + tk) * tf.n_r + m, p);
+          }
+          continue;
         }
         int64_t e2;
         double d2b, ex, ey, ez;
@@ -664,7 +704,7 @@ static int itl_prepare(mmd_ctx *c) {
   const dim3 grid((c->n_itl + kBlock - 1) / kBlock);
   k_itl_prepare<<<grid, kBlock, 0, c->stream>>>(c->g, I);
   ITL_TRY(hipGetLastError());
-  k_itl_nbr<<<grid, kBlock, 0, c->stream>>>(I);
+  k_itl_nbr<<<dim3((c->n_itl + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, c->stream>>>(I);
   ITL_TRY(hipGetLastError());
   c->itl_stale = false;
   return MMD_OK;
