@@ -20,7 +20,7 @@ struct BatchDom {
   double4 *pos;
   double *vx, *vy, *vz, *fx, *fy, *fz, *rho, *df;
   int8_t *type;
-  const int32_t *off;  // [2][kMaxStencil] stencil LDS offsets (shared by all replicas)
+  const int32_t *off;  // stencil device image (kStencilImage words, shared by all replicas)
   int32_t nk_even, nk_odd;
   double *e_rho, *e_force;  // per pair-pass tile block
   double *e_ke;             // per row-mapped site block

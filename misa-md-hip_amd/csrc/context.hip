@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -72,6 +73,46 @@ Halo make_halo(const int32_t *st3[2], const int32_t nk[2]) {
 int32_t stencil_lds_offset(int p, int64_t dX, int64_t dY, int64_t dZ, const Halo &h) {
   const int64_t s1 = (p + dX) & 1;
   return (int32_t)(((dZ * h.ly + dY) * 2 + (s1 - p)) * h.lx + floor_div2(p + dX));
+}
+
+// Ideal displacement of stencil entry (dX, dY, dZ) from a parity-p centre, in
+// units of a/2: the neighbour has parity p ^ (dX & 1), and a parity-1 site
+// sits at (a/2)(1,1,1) from its cell's corner, so y and z also shift by the
+// parity difference.
+static void ideal_disp2(int p, int64_t dX, int64_t dY, int64_t dZ, int64_t out[3]) {
+  const int64_t ds = ((p + dX) & 1) - p;  // parity of the neighbour minus the centre's
+  out[0] = dX, out[1] = 2 * dY + ds, out[2] = 2 * dZ + ds;
+}
+
+std::vector<int32_t> stencil_image(const int32_t *st3[2], const int32_t nk[2], const Halo &h) {
+  std::vector<int32_t> img(kStencilImage, 0);
+  std::vector<std::array<int64_t, 3>> disp[2];
+  for (int p = 0; p < 2; p++)
+    for (int32_t k = 0; k < nk[p]; k++) {
+      const int32_t *e = &st3[p][3 * (size_t)k];
+      img[(size_t)p * kMaxStencil + k] = stencil_lds_offset(p, e[0], e[1], e[2], h);
+      std::array<int64_t, 3> d;
+      ideal_disp2(p, e[0], e[1], e[2], d.data());
+      disp[p].push_back(d);
+    }
+  // symmetric: the neighbour reached by every entry sees the centre through
+  // an entry of its own parity with the opposite displacement
+  bool sym = true;
+  for (int p = 0; p < 2 && sym; p++)
+    for (int32_t k = 0; k < nk[p] && sym; k++) {
+      const int q = (p + st3[p][3 * (size_t)k]) & 1;
+      const std::array<int64_t, 3> neg = {-disp[p][k][0], -disp[p][k][1], -disp[p][k][2]};
+      sym = std::find(disp[q].begin(), disp[q].end(), neg) != disp[q].end();
+    }
+  for (int p = 0; p < 2; p++)
+    for (int32_t k = 0; k < nk[p]; k++) {
+      const auto &d = disp[p][k];
+      // lexicographically positive in (z, y, x): the side that counts the pair
+      const bool first = d[2] > 0 || (d[2] == 0 && (d[1] > 0 || (d[1] == 0 && d[0] > 0)));
+      if (!sym || first) img[kStencilBits + (size_t)p * (kMaxStencil / 32) + k / 32] |= (int32_t)(1u << (k % 32));
+    }
+  img[kStencilHalfFlag] = sym ? 1 : 0;
+  return img;
 }
 
 int check_tile_lds(int device, const Halo &h) {
@@ -410,14 +451,10 @@ extern "C" int mmd_set_stencil(mmd_ctx *c, const int64_t *even, int32_t n_even, 
     const int32_t nk[2] = {n_even, n_odd};
     const Halo h = make_halo(st3, nk);
     if ((rc = check_tile_lds(c->device, h))) return rc;
-    std::vector<int32_t> host(2 * kMaxStencil, 0);
-    for (int p = 0; p < 2; p++)
-      for (int32_t k = 0; k < nk[p]; k++)
-        host[(size_t)p * kMaxStencil + k] =
-            stencil_lds_offset(p, st3[p][3 * (size_t)k], st3[p][3 * (size_t)k + 1], st3[p][3 * (size_t)k + 2], h);
+    const std::vector<int32_t> host = stencil_image(st3, nk, h);
     c->halo = h;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (!c->d_off && (rc = dalloc(&c->d_off, 2 * kMaxStencil))) return rc;
+    if (!c->d_off && (rc = dalloc(&c->d_off, kStencilImage))) return rc;
     HIP_TRY(hipMemcpy(c->d_off, host.data(), host.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     c->nk_even = n_even;
     c->nk_odd = n_odd;

@@ -93,6 +93,8 @@ int32_t stencil_dev_offset(int p, int64_t dX, int64_t dY, int64_t dZ, const mmd_
 Halo make_halo(const int32_t *st3[2], const int32_t nk[2]);
 // LDS offset of stencil entry (dX doubled, dY, dZ) for centre parity p within a tile of halo h
 int32_t stencil_lds_offset(int p, int64_t dX, int64_t dY, int64_t dZ, const Halo &h);
+// the stencil's device image for the pair passes (layout at kStencilImage)
+std::vector<int32_t> stencil_image(const int32_t *st3[2], const int32_t nk[2], const Halo &h);
 // fails with MMD_E_RANGE when the tile of h does not fit the device's LDS
 int check_tile_lds(int device, const Halo &h);
 

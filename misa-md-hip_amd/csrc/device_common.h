@@ -21,6 +21,14 @@ constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kMaxStencil = 512;  // entries per parity
+// Device image of a stencil for the pair passes (stencil_image): [2][kMaxStencil]
+// LDS offsets, rows zero-padded; then [2][kMaxStencil / 32] bit masks of the
+// entries that count a pair's energy (each unordered pair on exactly one side
+// when the stencil is symmetric); then one word, 1 when the masks hold that
+// half (pair energy weight 1) and 0 when every entry counts with weight 1/2.
+constexpr int kStencilBits = 2 * kMaxStencil;
+constexpr int kStencilHalfFlag = kStencilBits + 2 * (kMaxStencil / 32);
+constexpr int kStencilImage = kStencilHalfFlag + 1;
 constexpr int kMaxTypes = 8;
 constexpr int kXcds = 8;          // MI355X: 8 accelerator complex dies, blocks dispatched round-robin
 

@@ -19,6 +19,7 @@ namespace mmd {
 #define MMD_PAIR_UNROLL 4
 #endif
 constexpr int kUnroll = MMD_PAIR_UNROLL;  // stencil entries whose loads are in flight together
+static_assert(32 % kUnroll == 0, "a trip's entries share one word of the energy masks");
 // stencil rows are read up to index nk + kUnroll - 2 (rows are zero-padded to kMaxStencil)
 static_assert(kUnroll <= 8, "stencil padding too short for the unroll");
 
@@ -88,10 +89,11 @@ __device__ __forceinline__ double rho_f64(const TabF64 &tf, int tj, double d2) {
 }
 
 // fpair = -(df_i rho'_{tj} + df_j rho'_{ti} + phi'_{ti tj}) / r (the scalar that
-// multiplies del = x_i - x_j); phi receives phi_{ti tj}(r) when WANT_E.
+// multiplies del = x_i - x_j); phi receives phi_{ti tj}(r) when WANT_E and the
+// (wave-uniform) entry counts the pair's energy.
 template <bool WANT_E>
 __device__ __forceinline__ double fpair_f64(const TabF64 &tf, int ti, int tj, double d2, double dfi, double dfj,
-                                            double &phi) {
+                                            bool count_e, double &phi) {
   const double rinv = rsqrt64(d2);
   const Seg sg = seg_f64(d2 * rinv, tf.r0, tf.inv_dr, tf.n_r);
   const uint32_t rp = (uint32_t)((ti * tf.nt + tj) * tf.n_r + sg.m);
@@ -104,7 +106,7 @@ __device__ __forceinline__ double fpair_f64(const TabF64 &tf, int ti, int tj, do
   const double rhojp = (c0.x * p + c0.y) * p + c1.x;
   const double phip = (c1.y * p + c2.x) * p + c2.y;
   const double rhoip = unlike ? (ua.x * p + ua.y) * p + uc : rhojp;
-  if (WANT_E) phi = spl_value(tf.phi, rp, p);
+  if (WANT_E && count_e) phi = spl_value(tf.phi, rp, p);
   return -(dfi * rhojp + dfj * rhoip + phip) * rinv;
 }
 
@@ -112,7 +114,8 @@ __device__ __forceinline__ double fpair_f64(const TabF64 &tf, int ti, int tj, do
 // no square root or divide; ok[u] = pair inside the tables' checked ranges.
 template <bool WANT_E, int U>
 __device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, int ti, const int *tj, const double *d2,
-                                                 double dfi, const double *dfj, double *fp, double *phi, bool *ok) {
+                                                 double dfi, const double *dfj, const bool *count_e, double *fp,
+                                                 double *phi, bool *ok) {
   double q[U];
   uint32_t rp[U], ra[U];
 #pragma unroll
@@ -130,7 +133,7 @@ __device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, int ti, con
     b0[u] = ld(tm.fb, 2 * rp[u]);
     b1[u] = ld(tm.fb, 2 * rp[u] + 1);
     ua[u] = ld(tm.rho.dp, ra[u]);
-    if (WANT_E) {
+    if (WANT_E && count_e[u]) {
       vb[u] = ld(tm.phi.vb, rp[u]);
       vc[u] = ld(tm.phi.vc, rp[u]);
     }
@@ -140,7 +143,7 @@ __device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, int ti, con
     const double rhojp = mix_deriv_rec(b0[u], q[u]);
     const double phip = mix_deriv_rec(b1[u], q[u]);
     const double rhoip = mix_deriv_rec(ua[u], q[u]);
-    if (WANT_E) {
+    if (WANT_E && count_e[u]) {
       const float cc = fmaf((float)q[u], vc[u].y, vc[u].x);
       phi[u] = fma(q[u], fma(q[u], (double)cc, vb[u].y), vb[u].x);
     }
@@ -337,6 +340,9 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
     const int t0 = lt[t.c];
     if (t0 >= 0) {
       const int32_t *o = off + t.s * kMaxStencil;
+      // entries that count the pair energy (each pair once for a symmetric stencil)
+      const int32_t *ebits = off + kStencilBits + t.s * (kMaxStencil / 32);
+      const double ew = off[kStencilHalfFlag] ? 1.0 : 0.5;
       const int nk = t.s ? nk_odd : nk_even;
       const double dfi = czd.y;
       const double d2_safe = 0.25 * rc2;
@@ -364,6 +370,10 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
         }
         if (__ballot(any) == 0) continue;
         double fp[kUnroll], phi[kUnroll];
+        bool ce[kUnroll];  // wave-uniform: this entry counts the pair energy
+        const uint32_t hw = WANT_E ? (uint32_t)ebits[k >> 5] : 0u;
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) ce[u] = WANT_E && ((hw >> ((k & 31) + u)) & 1u);
         bool redo = false;
         if (PREC == 1) {
           int tq[kUnroll];
@@ -376,7 +386,7 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
             dq[u] = in ? d2[u] : d2_safe;
           }
           bool ok[kUnroll];
-          trip_force_mixed<WANT_E, kUnroll>(tm, t0, tq, dq, dfi, dfj, fp, phi, ok);
+          trip_force_mixed<WANT_E, kUnroll>(tm, t0, tq, dq, dfi, dfj, ce, fp, phi, ok);
 #pragma unroll
           for (int u = 0; u < kUnroll; u++) redo |= d2[u] >= 0.0 && !ok[u];
         } else {
@@ -386,7 +396,7 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           for (int u = 0; u < kUnroll; u++) {
             const bool in = d2[u] >= 0.0;
             phi[u] = 0.0;
-            fp[u] = fpair_f64<WANT_E>(tf, t0, in ? tj[u] : t0, in ? d2[u] : d2_safe, dfi, dfj[u], phi[u]);
+            fp[u] = fpair_f64<WANT_E>(tf, t0, in ? tj[u] : t0, in ? d2[u] : d2_safe, dfi, dfj[u], ce[u], phi[u]);
           }
         }
         if (PREC == 1 && __ballot(redo) != 0) {
@@ -394,7 +404,7 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           for (int u = 0; u < kUnroll; u++) {
             const bool in = d2[u] >= 0.0;
             if (in && !mix_in(tm, d2[u]))
-              fp[u] = fpair_f64<WANT_E>(tf, t0, tj[u], d2[u], dfi, dfj[u], phi[u]);
+              fp[u] = fpair_f64<WANT_E>(tf, t0, tj[u], d2[u], dfi, dfj[u], ce[u], phi[u]);
           }
         }
 #pragma unroll
@@ -411,7 +421,7 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
       FX[i] = ax;
       FY[i] = ay;
       FZ[i] = az;
-      if (WANT_E) e_loc = 0.5 * ep;
+      if (WANT_E) e_loc = ew * ep;
     }
   }
   if (WANT_E) {

@@ -12,7 +12,7 @@ namespace mmd {
 int64_t site_blocks(const Geom &g);
 int64_t tile_blocks(const Geom &g);
 
-// EAM passes; off = [2][kMaxStencil] LDS offsets of the stencil within the tile (stencil_lds_offsets)
+// EAM passes; off = the stencil's device image (stencil_image): LDS offsets within the tile, energy masks
 hipError_t launch_rho_df(hipStream_t st, const Geom &g, const Halo &h, const int32_t *off, int32_t nk_even,
                          int32_t nk_odd, const double4 *P, double *RHO, double *DF, const TabF64 &tf,
                          const TabMixed &tm, double rc2, int prec, double *e_part);

@@ -274,7 +274,7 @@ struct mmd_server {
   std::vector<int32_t> st3[2];  // stencil (dx doubled, dy, dz) per centre parity
   int32_t nk[2] = {0, 0};
   Halo halo{};                  // pair-pass LDS tile of the stencil
-  int32_t *d_off_lds = nullptr; // [2][kMaxStencil] stencil LDS offsets, shared by all replicas
+  int32_t *d_off_lds = nullptr; // stencil device image (LDS offsets, energy masks), shared by all replicas
   BatchDom *h_dom = nullptr, *d_dom = nullptr;  // [max_replicas], indexed by slot
   ListBuf step, prime;
   std::vector<int> step_set;  // active replicas the step lists were built for
@@ -743,12 +743,7 @@ extern "C" int mmd_server_create(const mmd_server_opts *opts, const mmd_eam_tabl
       const size_t need = tile_lds_bytes(S->halo, true);
       if (need > 65536 && batch_kernels_allow_lds(need) != hipSuccess)
         return bail(fail(MMD_E_RANGE, "mmd_server_create: LDS tile of %zu bytes refused", need));
-      std::vector<int32_t> off(2 * kMaxStencil, 0);
-      for (int p = 0; p < 2; p++)
-        for (int32_t k = 0; k < S->nk[p]; k++) {
-          const int32_t *e = &S->st3[p][3 * (size_t)k];
-          off[(size_t)p * kMaxStencil + k] = stencil_lds_offset(p, e[0], e[1], e[2], S->halo);
-        }
+      const std::vector<int32_t> off = stencil_image(st3, S->nk, S->halo);
       if (hipMalloc((void **)&S->d_off_lds, off.size() * sizeof(int32_t)) != hipSuccess)
         return bail(fail(MMD_E_NOMEM, "mmd_server_create: stencil offsets"));
       if (hipMemcpy(S->d_off_lds, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
