@@ -263,6 +263,18 @@ static int run_kick(mmd_ctx *c, double h, bool drift, bool want_ke) {
   return MMD_OK;
 }
 
+static int run_kick_fused(mmd_ctx *c, double h, bool want_ke) {
+  hipEvent_t ev;
+  int rc = timed_begin(c, MMD_K_INTEGRATE, &ev);
+  if (rc) return rc;
+  HIP_TRY(launch_kick_fused(c->stream, c->g, c->masses, h, c->md.dt, c->pos, c->vx, c->vy, c->vz, c->fx, c->fy, c->fz,
+                            c->type, want_ke ? c->e_ke : nullptr));
+  rc = timed_end(c, MMD_K_INTEGRATE, ev);
+  if (rc) return rc;
+  if (c->n_itl > 0) return itl_kick_fused(c, h, want_ke);
+  return MMD_OK;
+}
+
 static int run_finalize(mmd_ctx *c, double *out, int32_t *counter) {
   const int64_t nt = tile_blocks(c->g), nb = site_blocks(c->g);
   const bool itl = c->n_itl > 0;
@@ -272,14 +284,30 @@ static int run_finalize(mmd_ctx *c, double *out, int32_t *counter) {
   return MMD_OK;
 }
 
-// one velocity-Verlet step; energy recorded into d_energy[(*d_counter)++]
-static int md_step(mmd_ctx *c, bool want_e) {
+// One velocity-Verlet step up to its force pass: the opening half kick and
+// drift (fused with the previous step's closing half kick when `fused`, which
+// then also records that step's energy when prev_e), ghost x, rho + df, ghost
+// df, force (with the energy partials when want_e).  The step's closing half
+// kick comes with the next step, or from md_close at the end of a run.
+// Energies go to d_energy[(*d_counter)++].
+static int md_step(mmd_ctx *c, bool fused, bool prev_e, bool want_e) {
   int rc;
-  if ((rc = run_kick(c, 0.5 * c->md.dt, true, false))) return rc;
+  const double h = 0.5 * c->md.dt;
+  if (fused) {
+    if ((rc = run_kick_fused(c, h, prev_e))) return rc;
+    if (prev_e && (rc = run_finalize(c, c->d_energy, c->d_counter))) return rc;
+  } else if ((rc = run_kick(c, h, true, false))) {
+    return rc;
+  }
   if ((rc = run_ghost(c, MMD_F_X))) return rc;
   if ((rc = run_rho_df(c, c->md.cutoff, want_e))) return rc;
   if ((rc = run_ghost(c, MMD_F_DF))) return rc;
-  if ((rc = run_force(c, c->md.cutoff, want_e))) return rc;
+  return run_force(c, c->md.cutoff, want_e);
+}
+
+// the last step's closing half kick (and its energy record)
+static int md_close(mmd_ctx *c, bool want_e) {
+  int rc;
   if ((rc = run_kick(c, 0.5 * c->md.dt, false, want_e))) return rc;
   if (want_e && (rc = run_finalize(c, c->d_energy, c->d_counter))) return rc;
   return MMD_OK;
@@ -1081,15 +1109,16 @@ extern "C" int mmd_md_run(mmd_ctx *c, int32_t nsteps, double *energies, int32_t 
     const int64_t n_rec = every > 0 ? nsteps / every : 0;
     if ((rc = ensure_energy_cap(c, std::max<int64_t>(n_rec, 1)))) return rc;
     HIP_TRY(launch_fill_i32(c->stream, c->d_counter, 1, 0));
+    auto records = [&](int32_t s) { return every > 0 && (s + 1) % every == 0; };
     for (int32_t s = 0; s < nsteps; s++) {
-      const bool want_e = every > 0 && (s + 1) % every == 0;
+      const bool want_e = records(s), fused = s > 0, prev_e = fused && records(s - 1);
       if (use_graph) {
-        hipGraphExec_t &ge = c->graph[want_e ? 1 : 0];
+        hipGraphExec_t &ge = c->graph[(fused ? 4 : 0) + (prev_e ? 2 : 0) + (want_e ? 1 : 0)];
         if (!ge) {
           hipGraph_t gr;
           HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
           c->capturing = true;
-          rc = md_step(c, want_e);
+          rc = md_step(c, fused, prev_e, want_e);
           c->capturing = false;
           hipError_t ee = hipStreamEndCapture(c->stream, &gr);
           if (rc) {
@@ -1103,9 +1132,10 @@ extern "C" int mmd_md_run(mmd_ctx *c, int32_t nsteps, double *energies, int32_t 
         }
         HIP_TRY(hipGraphLaunch(ge, c->stream));
       } else {
-        if ((rc = md_step(c, want_e))) return rc;
+        if ((rc = md_step(c, fused, prev_e, want_e))) return rc;
       }
     }
+    if (nsteps > 0 && (rc = md_close(c, records(nsteps - 1)))) return rc;
     if ((rc = itl_check(c))) return rc;
     if (n_rec > 0 && energies) {
       std::vector<double> buf((size_t)(2 * n_rec));

@@ -50,7 +50,9 @@ struct mmd_ctx {
 
   // MD
   mmd_md_params md{};
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // captured steps, by (opening kick fused with the previous step's closing
+  // one, that step's energy recorded, this step's energy recorded)
+  hipGraphExec_t graph[8] = {};
   bool capturing = false;
 
   // drop-in AoS staging
@@ -111,6 +113,7 @@ int itl_rho_df(mmd_ctx *c, double cutoff, bool want_e);
 int itl_df(mmd_ctx *c);
 int itl_force(mmd_ctx *c, double cutoff, bool want_e);
 int itl_kick(mmd_ctx *c, double h, bool drift, bool want_ke);
+int itl_kick_fused(mmd_ctx *c, double h, bool want_ke);
 void itl_free(mmd_ctx *c);
 int itl_check(mmd_ctx *c);
 

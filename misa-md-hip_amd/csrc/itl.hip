@@ -605,6 +605,31 @@ __global__ __launch_bounds__(kBlock) void k_mig_compact(MigDev M) {
   if (threadIdx.x == 0) M.cnt[1] = base;
 }
 
+// closing half kick of a step + opening kick and drift of the next (the same
+// operations, in the same order, as k_itl_kick<false, K> then k_itl_kick<true, false>)
+template <bool WANT_KE>
+__global__ __launch_bounds__(kBlock) void k_itl_kick_fused(ItlDev I, double *vx, double *vy, double *vz, double *x,
+                                                             double *y, double *z, Masses M, double h, double dt,
+                                                             double *ke_part) {
+  __shared__ double lds[kWavesPerBlock];
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  double ke = 0.0;
+  if (k < I.n) {
+    const int t = I.type[k];
+    const double sc = h * kFtm2v * M.inv_m[t];
+    const double fx = I.fx[k], fy = I.fy[k], fz = I.fz[k];
+    const double a = vx[k] + sc * fx, b = vy[k] + sc * fy, c = vz[k] + sc * fz;
+    if (WANT_KE) ke = 0.5 * M.m[t] * (a * a + b * b + c * c) * kMvv2e;
+    const double a2 = a + sc * fx, b2 = b + sc * fy, c2 = c + sc * fz;
+    vx[k] = a2, vy[k] = b2, vz[k] = c2;
+    x[k] += dt * a2, y[k] += dt * b2, z[k] += dt * c2;
+  }
+  if (WANT_KE) {
+    const double s = block_sum(ke, lds);
+    if (threadIdx.x == 0) ke_part[blockIdx.x] = s;
+  }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------- host
@@ -781,6 +806,21 @@ int itl_kick(mmd_ctx *c, double h, bool drift, bool want_ke) {
   }
   ITL_TRY(hipGetLastError());
   if (drift) c->itl_stale = true;
+  return MMD_OK;
+}
+
+int itl_kick_fused(mmd_ctx *c, double h, bool want_ke) {
+  ItlDev I = make_dev(c);
+  const dim3 grid((c->n_itl + kBlock - 1) / kBlock);
+  double *kp = c->e_itl + c->itl_part_n;
+  if (want_ke)
+    k_itl_kick_fused<true><<<grid, kBlock, 0, c->stream>>>(I, c->ivx, c->ivy, c->ivz, c->ix, c->iy, c->iz, c->masses, h,
+                                                           c->md.dt, kp);
+  else
+    k_itl_kick_fused<false><<<grid, kBlock, 0, c->stream>>>(I, c->ivx, c->ivy, c->ivz, c->ix, c->iy, c->iz, c->masses,
+                                                            h, c->md.dt, kp);
+  ITL_TRY(hipGetLastError());
+  c->itl_stale = true;
   return MMD_OK;
 }
 

@@ -54,6 +54,12 @@ hipError_t launch_kick(hipStream_t st, const Geom &g, const Masses &M, double h,
                        double *VX, double *VY, double *VZ, const double *FX, const double *FY, const double *FZ,
                        const int8_t *T, double *ke_part);
 
+// the closing half kick of a step and the opening kick + drift of the next in
+// one pass; ke_part (optional) gets the completed step's kinetic energy partials
+hipError_t launch_kick_fused(hipStream_t st, const Geom &g, const Masses &M, double h, double dt, double4 *P,
+                             double *VX, double *VY, double *VZ, const double *FX, const double *FY, const double *FZ,
+                             const int8_t *T, double *ke_part);
+
 // energies: out[slot] = {sum(ep_a) + sum(ep_b), sum(ke)}, slot = (*counter)++
 hipError_t launch_energy_finalize(hipStream_t st, const double *ep_a, int64_t na, const double *ep_b, int64_t nb,
                                   const double *ep_c, int64_t nc, const double *ke, int64_t nk, const double *ke_b,

@@ -119,6 +119,16 @@ __global__ __launch_bounds__(kBlock) void k_kick(Geom g, Masses M, double h, dou
   kick_block<DRIFT, WANT_KE>(g, xcd_block(), M, h, dt, P, VX, VY, VZ, FX, FY, FZ, T, ke_part, lds);
 }
 
+template <bool WANT_KE>
+__global__ __launch_bounds__(kBlock) void k_kick_fused(Geom g, Masses M, double h, double dt, double4 *__restrict__ P,
+                                                         double *__restrict__ VX, double *__restrict__ VY,
+                                                         double *__restrict__ VZ, const double *__restrict__ FX,
+                                                         const double *__restrict__ FY, const double *__restrict__ FZ,
+                                                         const int8_t *__restrict__ T, double *__restrict__ ke_part) {
+  __shared__ double lds[kWavesPerBlock];
+  kick_fused_block<WANT_KE>(g, xcd_block(), M, h, dt, P, VX, VY, VZ, FX, FY, FZ, T, ke_part, lds);
+}
+
 __device__ double ordered_sum(const double *__restrict__ p, int64_t n) {
   double s = 0.0;
   for (int64_t k = threadIdx.x; k < n; k += kBlock) s += p[k];
@@ -223,6 +233,17 @@ hipError_t launch_kick(hipStream_t st, const Geom &g, const Masses &M, double h,
     else
       k_kick<false, false><<<grid, kBlock, 0, st>>>(g, M, h, dt, P, VX, VY, VZ, FX, FY, FZ, T, ke_part);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_kick_fused(hipStream_t st, const Geom &g, const Masses &M, double h, double dt, double4 *P,
+                             double *VX, double *VY, double *VZ, const double *FX, const double *FY, const double *FZ,
+                             const int8_t *T, double *ke_part) {
+  const dim3 grid((unsigned)site_blocks(g));
+  if (ke_part)
+    k_kick_fused<true><<<grid, kBlock, 0, st>>>(g, M, h, dt, P, VX, VY, VZ, FX, FY, FZ, T, ke_part);
+  else
+    k_kick_fused<false><<<grid, kBlock, 0, st>>>(g, M, h, dt, P, VX, VY, VZ, FX, FY, FZ, T, ke_part);
   return hipGetLastError();
 }
 

@@ -102,6 +102,44 @@ __device__ __forceinline__ void kick_block(const Geom &g, int64_t lb, const Mass
   }
 }
 
+// The closing half kick of one step fused with the opening half kick and
+// drift of the next: v += h a (the completed step's kinetic energy from this
+// v), then v += h a and x += dt v.  The same operations in the same order as
+// the two separate passes, so the trajectory is bit-identical.
+template <bool WANT_KE>
+__device__ __forceinline__ void kick_fused_block(const Geom &g, int64_t lb, const Masses &M, double h, double dt,
+                                                 double4 *__restrict__ P, double *__restrict__ VX,
+                                                 double *__restrict__ VY, double *__restrict__ VZ,
+                                                 const double *__restrict__ FX, const double *__restrict__ FY,
+                                                 const double *__restrict__ FZ, const int8_t *__restrict__ T,
+                                                 double *__restrict__ ke_part, double *lds) {
+  double ke = 0.0;
+  Site st;
+  if (map_site_lb(g, lb, st) && st.active) {
+    const int64_t i = st.i;
+    const int t = T[i];
+    if (t >= 0) {
+      const double sc = h * kFtm2v * M.inv_m[t];
+      const double fx = FX[i], fy = FY[i], fz = FZ[i];
+      const double ux = VX[i] + sc * fx, uy = VY[i] + sc * fy, uz = VZ[i] + sc * fz;
+      if (WANT_KE) ke = 0.5 * M.m[t] * (ux * ux + uy * uy + uz * uz) * kMvv2e;
+      const double vx = ux + sc * fx, vy = uy + sc * fy, vz = uz + sc * fz;
+      VX[i] = vx;
+      VY[i] = vy;
+      VZ[i] = vz;
+      double4 q = P[i];
+      q.x += dt * vx;
+      q.y += dt * vy;
+      q.z += dt * vz;
+      P[i] = q;
+    }
+  }
+  if (WANT_KE) {
+    const double b = block_sum(ke, lds);
+    if (threadIdx.x == 0) ke_part[lb] = b;
+  }
+}
+
 }  // namespace mmd
 
 #endif
