@@ -100,8 +100,16 @@ __device__ __forceinline__ double fpair_f64(const TabF64 &tf, int ti, int tj, do
   const bool unlike = ti != tj;
   const uint32_t ra = unlike ? (uint32_t)(ti * tf.n_r + sg.m) : 0u;
   const double2 c0 = ld(tf.fc, 3 * rp), c1 = ld(tf.fc, 3 * rp + 1), c2 = ld(tf.fc, 3 * rp + 2);
-  const double2 ua = ld(tf.rho.d01, ra);
-  const double uc = ld(tf.rho.d2, ra);
+  // rho'_{ti} only for unlike pairs: the load runs on those lanes alone (the
+  // load unit's time follows the active lanes; out-of-cutoff lanes are too
+  // few to pay for the same treatment), into registers set first so that no
+  // copy, and no wait, is needed where the lanes rejoin
+  double2 ua = make_double2(0.0, 0.0);
+  double uc = 0.0;
+  if (unlike) {
+    ua = ld(tf.rho.d01, ra);
+    uc = ld(tf.rho.d2, ra);
+  }
   const double p = sg.p;
   const double rhojp = (c0.x * p + c0.y) * p + c1.x;
   const double phip = (c1.y * p + c2.x) * p + c2.y;
@@ -132,7 +140,9 @@ __device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, int ti, con
   for (int u = 0; u < U; u++) {
     b0[u] = ld(tm.fb, 2 * rp[u]);
     b1[u] = ld(tm.fb, 2 * rp[u] + 1);
-    ua[u] = ld(tm.rho.dp, ra[u]);
+    // rho'_{ti} only on unlike lanes (like pairs reuse rho'_{tj}); see fpair_f64
+    ua[u] = make_double2(0.0, 0.0);
+    if (ti != tj[u]) ua[u] = ld(tm.rho.dp, ra[u]);
     if (WANT_E && count_e[u]) {
       vb[u] = ld(tm.phi.vb, rp[u]);
       vc[u] = ld(tm.phi.vc, rp[u]);
@@ -142,7 +152,7 @@ __device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, int ti, con
   for (int u = 0; u < U; u++) {
     const double rhojp = mix_deriv_rec(b0[u], q[u]);
     const double phip = mix_deriv_rec(b1[u], q[u]);
-    const double rhoip = mix_deriv_rec(ua[u], q[u]);
+    const double rhoip = ti != tj[u] ? mix_deriv_rec(ua[u], q[u]) : rhojp;
     if (WANT_E && count_e[u]) {
       const float cc = fmaf((float)q[u], vc[u].y, vc[u].x);
       phi[u] = fma(q[u], fma(q[u], (double)cc, vb[u].y), vb[u].x);
