@@ -47,6 +47,9 @@ inline int batch_blocks(int64_t items) { return (int)((std::max<int64_t>(items, 
 
 hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, int blocks, const Masses &M, double h, double dt,
                              bool drift, bool want_ke);
+// a step's closing half kick fused with the next step's opening kick and drift
+hipError_t launch_batch_kick_fused(hipStream_t st, const BatchLists &L, int blocks, const Masses &M, double h,
+                                   double dt, bool want_ke);
 hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, int blocks, uint32_t fields);
 hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, int blocks, const Halo &h, const TabF64 &tf,
                                const TabMixed &tm, double rc2, int prec, bool want_e);

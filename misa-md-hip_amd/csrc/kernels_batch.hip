@@ -38,6 +38,17 @@ __global__ __launch_bounds__(kBlock) void kb_kick(BatchLists L, Masses M, double
   }
 }
 
+template <bool WANT_KE>
+__global__ __launch_bounds__(kBlock) void kb_kick_fused(BatchLists L, Masses M, double h, double dt) {
+  __shared__ double lds[kWavesPerBlock];
+  const Range r = xcd_range(*L.n_site_items);
+  for (int it = r.begin; it < r.end; it += r.stride) {
+    const int2 w = L.site_items[it];
+    const BatchDom &d = L.dom[w.x];
+    kick_fused_block<WANT_KE>(d.g, w.y, M, h, dt, d.pos, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.type, d.e_ke, lds);
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void kb_ghost(BatchLists L, uint32_t fields) {
   const Range r = xcd_range(*L.n_ext_items);
   for (int it = r.begin; it < r.end; it += r.stride) {
@@ -99,6 +110,16 @@ __global__ __launch_bounds__(kBlock) void kb_energy(BatchLists L, int which) {
 }
 
 }  // namespace
+
+hipError_t launch_batch_kick_fused(hipStream_t st, const BatchLists &L, int blocks, const Masses &M, double h,
+                                   double dt, bool want_ke) {
+  const dim3 grid(blocks), block(kBlock);
+  if (want_ke)
+    kb_kick_fused<true><<<grid, block, 0, st>>>(L, M, h, dt);
+  else
+    kb_kick_fused<false><<<grid, block, 0, st>>>(L, M, h, dt);
+  return hipGetLastError();
+}
 
 hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, int blocks, const Masses &M, double h, double dt,
                              bool drift, bool want_ke) {

@@ -278,7 +278,7 @@ struct mmd_server {
   BatchDom *h_dom = nullptr, *d_dom = nullptr;  // [max_replicas], indexed by slot
   ListBuf step, prime;
   std::vector<int> step_set;  // active replicas the step lists were built for
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graph[4] = {};  // captured steps by (fused opening kick, energy partials)
   double *h_e = nullptr;  // pinned [max_replicas][4]
 
   // worker pool
@@ -433,17 +433,29 @@ int fill_lists(mmd_server *S, ListBuf &b, const std::vector<int> &reps, bool is_
   return MMD_OK;
 }
 
-int step_passes(mmd_server *S, const BatchLists &L, const ListBuf &B, bool want_e) {
+// One batched step up to its force pass: the opening half kick and drift,
+// fused with the previous step's closing half kick when `fused` (the same
+// arithmetic as the two separate passes, as on the single-domain path).  The
+// chunk's last step closes with step_close, which also records the energies.
+int step_passes(mmd_server *S, const BatchLists &L, const ListBuf &B, bool fused, bool want_e) {
   const double dt = S->o.dt, h = 0.5 * S->o.dt, rc2 = S->o.cutoff * S->o.cutoff;
   const Masses &M = S->tab->masses;
   const int prec = S->o.precision;
-  HIP_TRY(launch_batch_kick(S->cs, L, B.g_site, M, h, dt, true, false));
+  if (fused)
+    HIP_TRY(launch_batch_kick_fused(S->cs, L, B.g_site, M, h, dt, false));
+  else
+    HIP_TRY(launch_batch_kick(S->cs, L, B.g_site, M, h, dt, true, false));
   HIP_TRY(launch_batch_ghost(S->cs, L, B.g_ext, MMD_F_X));
   HIP_TRY(launch_batch_rho_df(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e));
   HIP_TRY(launch_batch_ghost(S->cs, L, B.g_ext, MMD_F_DF));
   HIP_TRY(launch_batch_force(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e));
-  HIP_TRY(launch_batch_kick(S->cs, L, B.g_site, M, h, dt, false, want_e));
-  if (want_e) HIP_TRY(launch_batch_energy(S->cs, L, 0));
+  return MMD_OK;
+}
+
+// the chunk's closing half kick and the energies of its last step
+int step_close(mmd_server *S, const BatchLists &L, const ListBuf &B) {
+  HIP_TRY(launch_batch_kick(S->cs, L, B.g_site, S->tab->masses, 0.5 * S->o.dt, S->o.dt, false, true));
+  HIP_TRY(launch_batch_energy(S->cs, L, 0));
   return MMD_OK;
 }
 
@@ -540,13 +552,13 @@ struct GpuExec final : Executor {
     }
     const BatchLists L = S->step.lists(S->d_dom);
     for (int s = 0; s < steps; s++) {
-      const bool want_e = s == steps - 1;
+      const bool want_e = s == steps - 1, fused = s > 0;
       if (S->o.use_graph) {
-        hipGraphExec_t &ge = S->graph[want_e ? 1 : 0];
+        hipGraphExec_t &ge = S->graph[(fused ? 2 : 0) + (want_e ? 1 : 0)];
         if (!ge) {
           hipGraph_t gr;
           HIP_TRY(hipStreamBeginCapture(S->cs, hipStreamCaptureModeThreadLocal));
-          rc = step_passes(S, L, S->step, want_e);
+          rc = step_passes(S, L, S->step, fused, want_e);
           hipError_t ee = hipStreamEndCapture(S->cs, &gr);
           if (rc) {
             if (ee == hipSuccess) (void)hipGraphDestroy(gr);
@@ -558,10 +570,11 @@ struct GpuExec final : Executor {
           HIP_TRY(ie);
         }
         HIP_TRY(hipGraphLaunch(ge, S->cs));
-      } else if ((rc = step_passes(S, L, S->step, want_e))) {
+      } else if ((rc = step_passes(S, L, S->step, fused, want_e))) {
         return rc;
       }
     }
+    if (steps > 0 && (rc = step_close(S, L, S->step))) return rc;
     HIP_TRY(hipStreamSynchronize(S->cs));
     return MMD_OK;
   }
