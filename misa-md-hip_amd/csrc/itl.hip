@@ -221,6 +221,31 @@ __device__ __forceinline__ void src_local(const Geom &g, int xe, int ye, int ze,
   sz = wrap_n(ze - g.gz, g.bz);
 }
 
+// ext device index of wide-stencil entry w around anchor (aX, aY, aZ)
+__device__ __forceinline__ int64_t wide_site(const Geom &g, const ItlDev &I, int aX, int aY, int aZ, int sgn, int w) {
+  const int xe = aX + sgn * I.wide[3 * w] + 2 * g.gxh, ye = aY + sgn * I.wide[3 * w + 1] + g.gy,
+            ze = aZ + sgn * I.wide[3 * w + 2] + g.gz;
+  return dev_index(g, xe & 1, xe >> 1, ye, ze);
+}
+
+// displacement site - interstitial and its square (one expression for both phases)
+__device__ __forceinline__ double itl_d2(const double4 &q, double px, double py, double pz, double &dx, double &dy,
+                                         double &dz) {
+  dx = q.x - px, dy = q.y - py, dz = q.z - pz;
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// LDS written by some lanes of a wave and read by others: order the wave's
+// LDS accesses (all lanes of the wave run the same path here)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kItlPer = 4;                 // wide-stencil entries per lane in one chunk
+constexpr int kItlChunk = kItlPer * kWave;  // entries tested per chunk
+
 // One wave per interstitial.  MODE 0: rho (+ owner-side lattice rho/df update),
 // MODE 1: force (+ owner-side lattice force update).
 template <int MODE, bool WANT_E>
@@ -230,6 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
                                                        double *__restrict__ FY, double *__restrict__ FZ, TabF64 tf,
                                                        double *__restrict__ e_part) {
   __shared__ double lds[kWavesPerBlock];
+  __shared__ int32_t wlist[kWavesPerBlock][kItlChunk];  // in-cutoff entries of the current chunk, per wave
   const int lane = threadIdx.x & 63;
   const int k = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   double acc = 0.0, fxk = 0.0, fyk = 0.0, fzk = 0.0, e_loc = 0.0;
@@ -240,98 +266,129 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
     const int nn = I.nnb[k];
     const int *nb = I.nbr + k * kItlNbrCap;
     const double dfk = MODE == 1 ? I.df[k] : 0.0;
-    for (int w = lane; w < I.n_wide; w += kWave) {
-      const int xe = aX + sgn * I.wide[3 * w] + 2 * g.gxh, ye = aY + sgn * I.wide[3 * w + 1] + g.gy,
-                ze = aZ + sgn * I.wide[3 * w + 2] + g.gz;
-      const int64_t e = dev_index(g, xe & 1, xe >> 1, ye, ze);
-      const int ti = T[e];
-      if (ti < 0) continue;
-      const double4 q = P[e];
-      const double dx = q.x - I.px[k], dy = q.y - I.py[k], dz = q.z - I.pz[k];
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      if (d2 >= I.rc2) continue;
-      const double r = sqrt(d2);
-      double p;
-      const int m = seg(r, tf.r0, tf.inv_dr, tf.n_r, p);
-      // this pair's term, reused when the owner adds interstitial k to the site
-      double v_k = 0.0, fp_k = 0.0;
-      if (MODE == 0) {
-        v_k = spl_value(tf.rho, ti * tf.n_r + m, p);
-        acc += v_k;
-      } else {
-        const double dfi = DF[e];
-        const double psip = dfi * spl_deriv(tf.rho, tk * tf.n_r + m, p) + dfk * spl_deriv(tf.rho, ti * tf.n_r + m, p) +
-                            spl_deriv(tf.phi, (ti * tf.nt + tk) * tf.n_r + m, p);
-        fp_k = -psip / r;  // force on the lattice atom along del = x_i - x_k
-        fxk -= dx * fp_k, fyk -= dy * fp_k, fzk -= dz * fp_k;
-      }
-      // owner: no lower-numbered neighbour interacts with this lattice site
-      int sx, sy, sz;
-      src_local(g, xe, ye, ze, sx, sy, sz);
-      bool owner = true;
-      for (int q = 0; q < nn && owner; q++) {
-        const int mm = nb[q];
-        if (mm > k) break;
-        int64_t e2;
-        double d2b, ex, ey, ez;
-        if (li_pair(g, I, P, mm, sx, sy, sz, e2, d2b, ex, ey, ez)) owner = false;
-      }
-      if (!owner) continue;
-      const int64_t src = dev_index(g, sx & 1, (sx >> 1) + g.gxh, sy + g.gy, sz + g.gz);
-      double rho_i = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, epair = 0.0;
-      if (MODE == 0) rho_i = RHO[src];
-      const double dfi = MODE == 1 ? DF[src] : 0.0;
-      // every interacting interstitial in increasing order (k merged into its neighbours)
-      int placed_k = 0;
-      for (int q = 0; q < nn + 1; q++) {
-        int mm;
-        if (!placed_k && (q - placed_k >= nn || nb[q - placed_k] > k)) {
-          mm = k;
-          placed_k = 1;
-        } else {
-          mm = nb[q - placed_k];
+    const double pxk = I.px[k], pyk = I.py[k], pzk = I.pz[k];
+    int32_t *cw = wlist[threadIdx.x >> 6];
+    for (int w0 = 0; w0 < I.n_wide; w0 += kItlChunk) {
+      // phase 1: every lane tests its entries of the chunk, the site-type and
+      // position loads of all of them in flight together; the entries inside
+      // the cutoff are compacted in stencil order
+      int32_t found = 0;
+      {
+        int64_t ee[kItlPer];
+        int tt[kItlPer];
+        double4 qq[kItlPer];
+#pragma unroll
+        for (int j = 0; j < kItlPer; j++) {
+          const int w = w0 + j * kWave + lane;
+          ee[j] = w < I.n_wide ? wide_site(g, I, aX, aY, aZ, sgn, w) : 0;
+          tt[j] = w < I.n_wide ? T[ee[j]] : -1;
+          qq[j] = P[ee[j]];
         }
-        if (mm == k) {  // the pair evaluated above: same image, same distance
-          if (MODE == 0) {
-            rho_i += tk == ti ? v_k : spl_value(tf.rho, tk * tf.n_r + m, p);
-          } else {
-            gx += dx * fp_k, gy += dy * fp_k, gz += dz * fp_k;
-            if (WANT_E) epair += spl_value(tf.phi, (ti * tf.nt + tk) * tf.n_r + m, p);
-          }
-          continue;
+#pragma unroll
+        for (int j = 0; j < kItlPer; j++) {
+          double dx, dy, dz;
+          const bool in = tt[j] >= 0 && itl_d2(qq[j], pxk, pyk, pzk, dx, dy, dz) < I.rc2;
+          const uint64_t m = __ballot(in);
+          if (in) cw[found + __popcll(m & ((1ull << lane) - 1))] = w0 + j * kWave + lane;
+          found += __popcll(m);
         }
-        int64_t e2;
-        double d2b, ex, ey, ez;
-        if (!li_pair(g, I, P, mm, sx, sy, sz, e2, d2b, ex, ey, ez)) continue;
-        const int tm = I.type[mm];
-        const double rb = sqrt(d2b);
-        double pb;
-        const int mb = seg(rb, tf.r0, tf.inv_dr, tf.n_r, pb);
+      }
+      wave_lds_sync();
+      // phase 2: one in-cutoff pair per lane (rarely two)
+      for (int c = lane; c < found; c += kWave) {
+        const int w = cw[c];
+        const int xe = aX + sgn * I.wide[3 * w] + 2 * g.gxh, ye = aY + sgn * I.wide[3 * w + 1] + g.gy,
+                  ze = aZ + sgn * I.wide[3 * w + 2] + g.gz;
+        const int64_t e = dev_index(g, xe & 1, xe >> 1, ye, ze);
+        const int ti = T[e];
+        const double4 q = P[e];
+        double dx, dy, dz;
+        const double d2 = itl_d2(q, pxk, pyk, pzk, dx, dy, dz);
+        if (ti < 0 || d2 >= I.rc2) continue;
+        const double r = sqrt(d2);
+        double p;
+        const int m = seg(r, tf.r0, tf.inv_dr, tf.n_r, p);
+        // this pair's term, reused when the owner adds interstitial k to the site
+        double v_k = 0.0, fp_k = 0.0;
         if (MODE == 0) {
-          rho_i += spl_value(tf.rho, tm * tf.n_r + mb, pb);
+          v_k = spl_value(tf.rho, ti * tf.n_r + m, p);
+          acc += v_k;
         } else {
-          const double psip = dfi * spl_deriv(tf.rho, tm * tf.n_r + mb, pb) +
-                              I.df[mm] * spl_deriv(tf.rho, ti * tf.n_r + mb, pb) +
-                              spl_deriv(tf.phi, (ti * tf.nt + tm) * tf.n_r + mb, pb);
-          const double fp = -psip / rb;
-          gx += ex * fp, gy += ey * fp, gz += ez * fp;
-          if (WANT_E) epair += spl_value(tf.phi, (ti * tf.nt + tm) * tf.n_r + mb, pb);
+          const double dfi = DF[e];
+          const double psip = dfi * spl_deriv(tf.rho, tk * tf.n_r + m, p) + dfk * spl_deriv(tf.rho, ti * tf.n_r + m, p) +
+                              spl_deriv(tf.phi, (ti * tf.nt + tk) * tf.n_r + m, p);
+          fp_k = -psip / r;  // force on the lattice atom along del = x_i - x_k
+          fxk -= dx * fp_k, fyk -= dy * fp_k, fzk -= dz * fp_k;
+        }
+        // owner: no lower-numbered neighbour interacts with this lattice site
+        int sx, sy, sz;
+        src_local(g, xe, ye, ze, sx, sy, sz);
+        bool owner = true;
+        for (int q = 0; q < nn && owner; q++) {
+          const int mm = nb[q];
+          if (mm > k) break;
+          int64_t e2;
+          double d2b, ex, ey, ez;
+          if (li_pair(g, I, P, mm, sx, sy, sz, e2, d2b, ex, ey, ez)) owner = false;
+        }
+        if (!owner) continue;
+        const int64_t src = dev_index(g, sx & 1, (sx >> 1) + g.gxh, sy + g.gy, sz + g.gz);
+        double rho_i = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, epair = 0.0;
+        if (MODE == 0) rho_i = RHO[src];
+        const double dfi = MODE == 1 ? DF[src] : 0.0;
+        // every interacting interstitial in increasing order (k merged into its neighbours)
+        int placed_k = 0;
+        for (int q = 0; q < nn + 1; q++) {
+          int mm;
+          if (!placed_k && (q - placed_k >= nn || nb[q - placed_k] > k)) {
+            mm = k;
+            placed_k = 1;
+          } else {
+            mm = nb[q - placed_k];
+          }
+          if (mm == k) {  // the pair evaluated above: same image, same distance
+            if (MODE == 0) {
+              rho_i += tk == ti ? v_k : spl_value(tf.rho, tk * tf.n_r + m, p);
+            } else {
+              gx += dx * fp_k, gy += dy * fp_k, gz += dz * fp_k;
+              if (WANT_E) epair += spl_value(tf.phi, (ti * tf.nt + tk) * tf.n_r + m, p);
+            }
+            continue;
+          }
+          int64_t e2;
+          double d2b, ex, ey, ez;
+          if (!li_pair(g, I, P, mm, sx, sy, sz, e2, d2b, ex, ey, ez)) continue;
+          const int tm = I.type[mm];
+          const double rb = sqrt(d2b);
+          double pb;
+          const int mb = seg(rb, tf.r0, tf.inv_dr, tf.n_r, pb);
+          if (MODE == 0) {
+            rho_i += spl_value(tf.rho, tm * tf.n_r + mb, pb);
+          } else {
+            const double psip = dfi * spl_deriv(tf.rho, tm * tf.n_r + mb, pb) +
+                                I.df[mm] * spl_deriv(tf.rho, ti * tf.n_r + mb, pb) +
+                                spl_deriv(tf.phi, (ti * tf.nt + tm) * tf.n_r + mb, pb);
+            const double fp = -psip / rb;
+            gx += ex * fp, gy += ey * fp, gz += ez * fp;
+            if (WANT_E) epair += spl_value(tf.phi, (ti * tf.nt + tm) * tf.n_r + mb, pb);
+          }
+        }
+        if (MODE == 0) {
+          double pe;
+          const int me = seg(rho_i, tf.rho0, tf.inv_drho, tf.n_rho, pe);
+          if (WANT_E) {
+            double po;
+            const int mo = seg(RHO[src], tf.rho0, tf.inv_drho, tf.n_rho, po);
+            e_loc += spl_value(tf.emb, ti * tf.n_rho + me, pe) - spl_value(tf.emb, ti * tf.n_rho + mo, po);
+          }
+          RHO[src] = rho_i;
+          DF[src] = spl_deriv(tf.emb, ti * tf.n_rho + me, pe);
+        } else {
+          FX[src] += gx, FY[src] += gy, FZ[src] += gz;
+          if (WANT_E) e_loc += epair;
         }
       }
-      if (MODE == 0) {
-        double pe;
-        const int me = seg(rho_i, tf.rho0, tf.inv_drho, tf.n_rho, pe);
-        if (WANT_E) {
-          double po;
-          const int mo = seg(RHO[src], tf.rho0, tf.inv_drho, tf.n_rho, po);
-          e_loc += spl_value(tf.emb, ti * tf.n_rho + me, pe) - spl_value(tf.emb, ti * tf.n_rho + mo, po);
-        }
-        RHO[src] = rho_i;
-        DF[src] = spl_deriv(tf.emb, ti * tf.n_rho + me, pe);
-      } else {
-        FX[src] += gx, FY[src] += gy, FZ[src] += gz;
-        if (WANT_E) e_loc += epair;
-      }
+      wave_lds_sync();  // the list is rewritten by the next chunk
     }
     // interstitial <-> interstitial
     for (int q = lane; q < nn; q += kWave) {
