@@ -2,27 +2,35 @@
 """MISA-MD EAM step throughput on one MI355X.
 
 Headline workload (the FP64 metric): 128x128x128 BCC cells (4,194,304 atoms),
-a = 2.855 A, 3 atom types drawn 90/5/5 from the seed, tables of the
-Finnis-Sinclair BCC Fe potential (embedding strength per type and pair term
-per type pair scaled by +-10%, phi symmetric), 5000-point splines on
-r in [0.5, 5.6] A (cutoff 5.6 A for every pass), Gaussian displacement 0.05 A, Maxwell velocities at 800 K,
-velocity Verlet with dt = 1 fs.  Everything is device resident: ghost
-refresh, rho + df, ghost df, force and integration run on the GPU, energies
-are recorded every 10 steps.  One step = one full velocity-Verlet step over
-every atom; `value` is atom-steps/s of the FP64 path.
+a = 2.855 A, 3 atom types drawn 90/5/5 from the seed, 9 synthetic pair tables
+of the named functional forms (phi = A (rc-r)^2 (1 + B/r), rho = (rc-r)^2
+exp(-r), F = -sqrt(rho) + C rho^2; A, B, C jittered +-10%, phi symmetric),
+5000-point splines on r in [0.5, 5.6] A, Gaussian displacement 0.05 A,
+Maxwell velocities at 800 K, velocity Verlet with dt = 1 fs.  Everything is
+device resident: ghost refresh, rho + df, ghost df, force and integration run
+on the GPU, energies are recorded every 10 steps.  One step = one full
+velocity-Verlet step over every atom; `value` is atom-steps/s of the FP64 path.
 
 The same run also measures
   * mixed precision against FP64 on the defect box (same 128^3 3-type alloy
-    plus 0.5% vacancies and 0.2% interstitials at tetrahedral sites, 1200 K):
-    atom-steps/s of both, force error of mixed vs FP64 on identical states
-    (start and end of the run) and the per-step energy difference per atom;
+    plus 0.5% vacancies and 0.2% interstitials at tetrahedral sites, 1200 K,
+    200 steps): atom-steps/s of both, force error of mixed vs FP64 on
+    identical states (start and end of the run), the per-step energy
+    difference per atom;
+  * the same FP64 step and the same mixed defect-box comparison with
+    Finnis-Sinclair Fe tables (a mechanically stable lattice; the synthetic
+    forms are far from equilibrium and disorder within ~100 steps), plus
+    interstitial migration on that defect box's final state, reported apart
+    as `stable_potential`;
   * the serving loop: 2,000 independent replicas (box edge log-uniform in
     10..64 BCC cells, 3-type alloy, 300-1500 K, 200 steps each, mixed
     precision) arriving as a Poisson stream at 75% of the capacity measured
     by a closed-loop burst just before; throughput over the whole trace and
     p50/p99 per-replica latency (arrival to results);
   * the CPU oracle (FP64 C restatement of the reference's EAM algorithm,
-    OpenMP on the host cores) on the headline box.
+    OpenMP on the host cores) on the headline box, one thread on the 20^3-cell
+    single-element box, and the per-atom check of the GPU's final forces
+    against the oracle on the headline box.
 
 Inputs are resident in HBM before the timed region starts.  Per-kernel
 durations come from HIP events recorded on the context's stream around every
@@ -53,13 +61,22 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_workload(cells, seed=2024, temperature=800.0, vacancy_frac=0.0, interstitial_frac=0.0):
-    spec = mmd.lattice_spec((cells, cells, cells), ghost_cells=3, a=A, n_types=3, sigma=0.05,
+def make_workload(cells, seed=2024, temperature=800.0, vacancy_frac=0.0, interstitial_frac=0.0, potential="synthetic",
+                  n_types=3):
+    """The benchmark box.  potential "synthetic": the functional forms the
+    benchmark configurations name (phi = A (rc-r)^2 (1 + B/r), rho = (rc-r)^2
+    exp(-r), F = -sqrt(rho) + C rho^2, A, B, C jittered +-10% per type / pair);
+    "fs": Finnis-Sinclair Fe, a mechanically stable lattice (reported apart)."""
+    spec = mmd.lattice_spec((cells, cells, cells), ghost_cells=3, a=A, n_types=n_types, sigma=0.05,
                             temperature=temperature, vacancy_frac=vacancy_frac,
                             interstitial_frac=interstitial_frac, seed=seed)
     cap = int(interstitial_frac * 2 * cells ** 3 * 1.5) + 16
     dom, box, st, itl = mmd.lattice_init(spec, itl_cap=cap)
-    tab = mmd.fs_tables(n_types=3, rc=RC, jitter=0.1, seed=seed)
+    jit = 0.1 if n_types > 1 else 0.0
+    if potential == "synthetic":
+        tab = mmd.synth_tables(n_types=n_types, rc=RC, jitter=jit, seed=seed)
+    else:
+        tab = mmd.fs_tables(n_types=n_types, rc=RC, jitter=jit, seed=seed)
     st3 = mmd.stencil_build(A, RC + SKIN, 0)
     return dict(dom=dom, box=box, st=st, itl=itl, tab=tab, st3=st3, even=mmd.stencil_linear(dom, st3),
                 odd=mmd.stencil_linear(dom, -st3), wide3=mmd.wide_stencil(A, RC, SKIN))
@@ -134,7 +151,7 @@ def run_gpu(w, prec, steps, warmup, keep_state=False):
     out = dict(seconds=t1 - t0, kernel_ms=ms, launches=n, energies=en, f0=f0, e0=e0)
     if keep_state:
         st = w["st"].copy()
-        ctx.download(st, mmd.F_X | mmd.F_V | mmd.F_TYPE)
+        ctx.download(st, mmd.F_X | mmd.F_V | mmd.F_TYPE | mmd.F_F)
         out["state"] = st
         out["itl"] = ctx.itl_get(w["itl"].n) if w["itl"].n else w["itl"]
         out["f_end"] = forces(ctx, w)
@@ -148,11 +165,39 @@ def force_err(F, Fr):
     return float(np.linalg.norm(F - Fr) / np.linalg.norm(Fr)), float(d.max() / rms)
 
 
-def cpu_baseline(w, steps):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(w, steps, gpu_end=None):
+    """The oracle on the host cores: the headline box for `steps` steps on all
+    threads, the small single-element box on one thread, and (gpu_end) a
+    per-atom check of the GPU run's final forces against the oracle on the
+    same positions."""
     import oracle as O
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    st = w["st"].copy()
     T = O.Tables(w["tab"], mmd.DEFAULT_MASSES)
+    out = {}
+    if gpu_end is not None:
+        st = gpu_end.copy()
+        O.rho_df(w["dom"], w["even"], w["odd"], T, RC, st, nthreads=nthreads)
+        O.ghost_df(w["dom"], st)
+        O.force(w["dom"], w["even"], w["odd"], T, RC, st, nthreads=nthreads)
+        li = st.local_index()
+        occ = li[st.type[li] >= 0]
+        Fo = np.stack([st.fx[occ], st.fy[occ], st.fz[occ]], 1)
+        Fg = np.stack([gpu_end.fx[occ], gpu_end.fy[occ], gpu_end.fz[occ]], 1)
+        out["gpu_check"] = {"what": "FP64 GPU forces after the timed steps vs the oracle on the same positions, "
+                                    "every atom", "atoms": int(occ.size),
+                            "max_abs_err_over_max_force": float(np.abs(Fg - Fo).max() / np.abs(Fo).max())}
+    st = w["st"].copy()
     O.rho_df(w["dom"], w["even"], w["odd"], T, RC, st, nthreads=nthreads)
     O.ghost_df(w["dom"], st)
     O.force(w["dom"], w["even"], w["odd"], T, RC, st, nthreads=nthreads)
@@ -160,9 +205,23 @@ def cpu_baseline(w, steps):
     t0 = time.perf_counter()
     O.md_run(w["dom"], w["even"], w["odd"], T, RC, w["box"], DT, steps, st, nthreads=nthreads, energies=False)
     dt = time.perf_counter() - t0
-    return dict(value=round(n_atoms * steps / dt, 1), unit="atom-steps/s", cores=nthreads, kind="port",
-                sample=f"CPU oracle (FP64 C restatement of the reference EAM step, OpenMP, {nthreads} threads) on "
-                       f"the same {n_atoms}-atom box for {steps} velocity-Verlet steps ({dt:.2f} s wall)")
+    # one thread on the small single-element box (20^3 cells, 16,000 atoms, 10 steps)
+    w1 = make_workload(20, seed=7, n_types=1)
+    s1 = w1["st"].copy()
+    T1 = O.Tables(w1["tab"], mmd.DEFAULT_MASSES)
+    O.rho_df(w1["dom"], w1["even"], w1["odd"], T1, RC, s1, nthreads=1)
+    O.ghost_df(w1["dom"], s1)
+    O.force(w1["dom"], w1["even"], w1["odd"], T1, RC, s1, nthreads=1)
+    t1 = time.perf_counter()
+    O.md_run(w1["dom"], w1["even"], w1["odd"], T1, RC, w1["box"], DT, 10, s1, nthreads=1, energies=False)
+    d1 = time.perf_counter() - t1
+    out.update(value=round(n_atoms * steps / dt, 1), unit="atom-steps/s", cores=nthreads, kind="port",
+               sample=f"CPU oracle (FP64 C restatement of the reference EAM step, OpenMP, {nthreads} threads) on "
+                      f"the same {n_atoms}-atom box for {steps} velocity-Verlet steps ({dt:.2f} s wall)",
+               one_thread={"value": round(n_atoms_of(w1) * 10 / d1, 1), "unit": "atom-steps/s",
+                           "sample": f"20^3 cells single element ({n_atoms_of(w1)} atoms), 10 steps, 1 thread"},
+               nproc=os.cpu_count(), cpu_model=cpu_model())
+    return out
 
 
 def load_traffic():
@@ -175,7 +234,7 @@ def load_traffic():
     return None
 
 
-def mixed_vs_fp64(w, steps, warmup):
+def mixed_vs_fp64(w, steps, warmup, migrate=False):
     n_atoms = n_atoms_of(w)
     r64 = run_gpu(w, mmd.PREC_FP64, steps, warmup, keep_state=True)
     rmx = run_gpu(w, mmd.PREC_MIXED, steps, warmup, keep_state=True)
@@ -187,7 +246,7 @@ def mixed_vs_fp64(w, steps, warmup):
     f_ref_end = forces(ctx, w)
     ctx.close()
     nw1, pa1 = force_err(rmx["f_end"], f_ref_end)
-    mig = migration(w, rmx["state"], rmx["itl"])
+    mig = migration(w, rmx["state"], rmx["itl"]) if migrate else None
     e64, emx = r64["energies"].sum(1), rmx["energies"].sum(1)
     rec = np.arange(ENERGY_EVERY - 1, steps, ENERGY_EVERY)
     d64 = e64[rec] - (r64["e0"][0] + r64["e0"][1])
@@ -203,7 +262,7 @@ def mixed_vs_fp64(w, steps, warmup):
             "force_kernel_ms": round(mk[mmd.K_FORCE] / max(mn[mmd.K_FORCE], 1), 4),
             "rho_df_kernel_ms": round(mk[mmd.K_RHO] / max(mn[mmd.K_RHO], 1), 4),
             "interstitial_ms_per_step": round(mk[mmd.K_ITL] / steps, 4),
-            "migration": mig}
+            **({"migration": mig} if mig else {})}
 
 
 def migration(w, state, itl, reps=5):
@@ -270,7 +329,7 @@ def replica_trace(n, seed, rate, t0=0.0, steps=200):
 
 
 def serving(n_replicas, load, seed=77, burst=512, steps=200, use_graph=True):
-    tab = mmd.fs_tables(n_types=3, rc=RC, jitter=0.1, seed=seed)
+    tab = mmd.synth_tables(n_types=3, rc=RC, jitter=0.1, seed=seed)
     with mmd.Server(tab, precision=mmd.PREC_MIXED, a=A, cutoff=RC, skin=SKIN, dt=DT, sigma=0.05, max_replicas=64,
                     hbm_fraction=0.5, chunk_steps=10, workers=16, use_graph=use_graph) as srv:
         # capacity: a burst of the same replica mix, all present at t = 0, measured while the admission queue
@@ -318,6 +377,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-fs", action="store_true", help="skip the stable-potential (Finnis-Sinclair) FP64 line")
+    ap.add_argument("--mixed-steps", type=int, default=200)
     ap.add_argument("--serve-replicas", type=int, default=2000)
     ap.add_argument("--serve-load", type=float, default=0.75)
     ap.add_argument("--serve-no-graph", action="store_true", help="launch the batched step directly (profiling)")
@@ -343,7 +404,7 @@ def main():
     Nc = pairs_in_cutoff(w)
 
     log(f"bench: FP64 {args.steps} steps (+{args.warmup} warmup), {n_atoms} atoms")
-    r64 = run_gpu(w, mmd.PREC_FP64, args.steps, args.warmup)
+    r64 = run_gpu(w, mmd.PREC_FP64, args.steps, args.warmup, keep_state=not args.no_cpu)
     value = n_atoms * args.steps / r64["seconds"]
     ms, n = r64["kernel_ms"], r64["launches"]
     t_force = ms[mmd.K_FORCE] / max(n[mmd.K_FORCE], 1) * 1e-3
@@ -367,7 +428,8 @@ def main():
            "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(r64["seconds"] / args.steps * 1e3, 4), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": f"BCC {args.cells}^3 cells, 3-type alloy 90/5/5, 800 K, dt 1 fs, FP64",
+           "config": {"workload": f"BCC {args.cells}^3 cells, 3-type alloy 90/5/5, synthetic EAM tables (A, B, C "
+                                  f"+-10%), 800 K, dt 1 fs, FP64",
                       "atoms": n_atoms, "stencil_entries": S, "pairs_in_cutoff": round(Nc, 2),
                       "energy_every": ENERGY_EVERY},
            "roofline": roofline,
@@ -378,9 +440,31 @@ def main():
     out["dropin_pcie"] = dropin_hooks(w)
 
     if not args.no_mixed:
-        log("bench: mixed vs FP64 on the defect box (0.5% vacancies, 0.2% interstitials, 1200 K)")
+        log(f"bench: mixed vs FP64 on the defect box (0.5% vacancies, 0.2% interstitials, 1200 K), "
+            f"{args.mixed_steps} steps")
         wd = make_workload(args.cells, seed=2025, temperature=1200.0, vacancy_frac=0.005, interstitial_frac=0.002)
-        out["mixed"] = mixed_vs_fp64(wd, args.steps, args.warmup)
+        out["mixed"] = mixed_vs_fp64(wd, args.mixed_steps, args.warmup)
+        del wd
+
+    if not args.no_fs:
+        log("bench: the same steps on a mechanically stable potential (Finnis-Sinclair Fe tables)")
+        wf = make_workload(args.cells, potential="fs")
+        rf = run_gpu(wf, mmd.PREC_FP64, args.steps, args.warmup)
+        mf, nf = rf["kernel_ms"], rf["launches"]
+        stable = {
+            "workload": f"BCC {args.cells}^3 cells, 3-type alloy, Finnis-Sinclair Fe tables (+-10% per type / pair), "
+                        "800 K, dt 1 fs, FP64; the synthetic forms leave the lattice far from equilibrium (it "
+                        "disorders within ~100 steps), this potential keeps a thermal BCC crystal",
+            "value": round(n_atoms_of(wf) * args.steps / rf["seconds"], 1), "unit": "atom-steps/s",
+            "force_kernel_ms": round(mf[mmd.K_FORCE] / max(nf[mmd.K_FORCE], 1), 4),
+            "rho_df_kernel_ms": round(mf[mmd.K_RHO] / max(nf[mmd.K_RHO], 1), 4)}
+        del wf
+        if not args.no_mixed:
+            wfd = make_workload(args.cells, seed=2025, temperature=1200.0, vacancy_frac=0.005,
+                                interstitial_frac=0.002, potential="fs")
+            stable["mixed_defect_box"] = mixed_vs_fp64(wfd, args.mixed_steps, args.warmup, migrate=True)
+            del wfd
+        out["stable_potential"] = stable
 
     if not args.no_serve:
         out["serving"] = serving(args.serve_replicas, args.serve_load, use_graph=not args.serve_no_graph)
@@ -388,7 +472,7 @@ def main():
 
     if not args.no_cpu:
         log(f"bench: CPU oracle baseline, {args.cpu_steps} steps")
-        out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, gpu_end=r64.get("state"))
     print(json.dumps(out), flush=True)
 
 
