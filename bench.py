@@ -253,6 +253,17 @@ def mixed_vs_fp64(w, steps, warmup, migrate=False):
     dmx = emx[rec] - (rmx["e0"][0] + rmx["e0"][1])
     per_step = np.abs(dmx - d64) / n_atoms / (rec + 1 + warmup)
     mk, mn = rmx["kernel_ms"], rmx["launches"]
+    # roofline of the mixed force kernel: the algorithmic flop count of the
+    # pass; the mixed path stores FP32 spline corrections but evaluates in
+    # FP64 (FP32 arithmetic only in the phi value of energy steps), so the
+    # FP64 vector peak is the roof
+    S, Nc = len(w["st3"]), pairs_in_cutoff(w)
+    t_mf = mk[mmd.K_FORCE] / max(mn[mmd.K_FORCE], 1) * 1e-3
+    ach = (8 * S + 40 * Nc) * n_atoms / t_mf / 1e12
+    mixed_roof = {"kernel": "k_force<mixed>", "achieved": round(ach, 3), "peak": PEAK_FP64_TFLOPS,
+                  "unit": "TFLOP/s", "frac": round(ach / PEAK_FP64_TFLOPS, 4),
+                  "flops_per_atom": {"fp64": round(8 * S + 40 * Nc, 1), "fp32": round(2 * Nc / ENERGY_EVERY, 1)},
+                  "hbm_frac": round(57 * n_atoms / t_mf / 1e9 / PEAK_HBM_GBPS, 4)}
     return {"workload": f"BCC {w['dom'].box[1]}^3 cells, 3-type alloy, 0.5% vacancies, "
                         f"{w['itl'].n} interstitials (0.2%), 1200 K, dt 1 fs",
             "atoms": n_atoms, "steps": steps, "fp64_value": round(v64, 1), "value": round(vmx, 1),
@@ -260,6 +271,7 @@ def mixed_vs_fp64(w, steps, warmup, migrate=False):
             "force_rel_err_normwise": max(nw0, nw1), "force_err_max_over_rms": max(pa0, pa1),
             "energy_diff_per_atom_per_step_max": float(per_step.max()) if len(per_step) else None,
             "force_kernel_ms": round(mk[mmd.K_FORCE] / max(mn[mmd.K_FORCE], 1), 4),
+            "roofline": mixed_roof,
             "rho_df_kernel_ms": round(mk[mmd.K_RHO] / max(mn[mmd.K_RHO], 1), 4),
             "interstitial_ms_per_step": round(mk[mmd.K_ITL] / steps, 4),
             **({"migration": mig} if mig else {})}
