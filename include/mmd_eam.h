@@ -227,8 +227,9 @@ int mmd_itl_get(mmd_ctx *ctx, int32_t *n, double *x, double *y, double *z, doubl
  * in the reference plugin, which runs one sub-domain per call).  Replicas
  * arrive over time; each is admitted when a slot and device memory (an
  * arena carved from free HBM at creation) are available, its lattice stays
- * resident, and all active replicas advance together through persistent
- * batched kernels replayed from a captured HIP graph.  A replica's results
+ * resident, and all active replicas advance together through batched
+ * launches over (replica, block) work items, replayed from a captured HIP
+ * graph.  A replica's results
  * are identical whether it runs alone or in a mix. */
 typedef struct mmd_server mmd_server;
 

@@ -222,7 +222,7 @@ __device__ __forceinline__ void trip_rho_f64(const TabF64 &tf, const int *tj, co
 // together; pairs are masked without per-pair branches and a trip with no
 // in-cutoff pair in the whole wave is skipped.  Records outside the
 // ghost-extended lattice are staged as vacancies.  The body starts with a
-// barrier so a persistent block can restage for its next tile.
+// barrier so a block that walks several work items can restage for its next tile.
 
 // LDS tile layout (n records): lxy = (x, y) [n], then lzd = (z, df) [n] for
 // the force pass or lz = z [n] for the rho pass, then the int8 types [n].

@@ -9,8 +9,8 @@
 // Resident replicas are primed (forces and admission energies), then advance
 // together: one batched step is the velocity-Verlet sequence of the
 // single-domain path (kick + drift, ghost x, rho + df, ghost df, force, kick),
-// each pass one persistent launch over a device work list of
-// (replica, block) items, replayed from a captured HIP graph.  A replica's
+// each pass one launch over a device work list of (replica, block) items, one
+// block per item, replayed from a captured HIP graph.  A replica's
 // blocks run the same per-block code with the same block numbering as the
 // single-domain kernels and its energies are reduced in the same fixed order,
 // so its trajectory is bit-identical alone, in any mix and on the

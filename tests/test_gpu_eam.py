@@ -407,3 +407,23 @@ def test_golden_fs_lattices_on_device(prec):
         assert np.allclose(out.rho[li], case["rho"], rtol=tol, atol=0), case["name"]
         assert abs(ep / n - case["energy"]) < tol * abs(case["energy"]), (case["name"], ep / n)
         assert np.abs(np.stack([out.fx[li], out.fy[li], out.fz[li]])).max() < 1e3 * tol, case["name"]
+
+
+def test_energy_with_asymmetric_stencil():
+    """A stencil whose parity-1 half lacks an entry (not symmetric): the pair
+    energy falls back to half of every visited pair, as the oracle sums it."""
+    c = W.case(**CASES["bcc12_3type_vac"])
+    odd = c["odd"][:-1].copy()  # drop the last parity-1 entry
+    T = O.Tables(c["tab"], c["masses"])
+    ref = c["st"].copy()
+    e1 = O.rho_df(c["dom"], c["even"], odd, T, RC, ref, want_e=True)
+    O.ghost_df(c["dom"], ref)
+    e2 = O.force(c["dom"], c["even"], odd, T, RC, ref, want_e=True)
+    with mmd.Context(0) as ctx:
+        ctx.set_domain(c["dom"])
+        ctx.set_stencil(c["even"], odd)
+        ctx.set_tables(c["tab"], c["masses"])
+        ctx.upload(c["st"], mmd.F_X | mmd.F_TYPE | mmd.F_V)
+        ctx.md_setup(c["box"], A, 1.0, RC)
+        ep, _ = ctx.energy()
+    assert abs(ep - (e1 + e2)) < 1e-12 * abs(e1 + e2)
