@@ -340,10 +340,13 @@ def replica_trace(n, seed, rate, t0=0.0, steps=200):
                  arrival_s=float(arr[i])) for i in range(n)]
 
 
-def serving(n_replicas, load, seed=77, burst=512, steps=200, use_graph=True):
+def serving(n_replicas, load, seed=77, burst=512, steps=200, use_graph=True, ghost_cells=2):
+    """Replicas carry no interstitials, so their ghost shell only needs the
+    stencil's reach (2 cells at rc + skin = 5.9 A)."""
     tab = mmd.synth_tables(n_types=3, rc=RC, jitter=0.1, seed=seed)
     with mmd.Server(tab, precision=mmd.PREC_MIXED, a=A, cutoff=RC, skin=SKIN, dt=DT, sigma=0.05, max_replicas=64,
-                    hbm_fraction=0.5, chunk_steps=10, workers=16, use_graph=use_graph) as srv:
+                    hbm_fraction=0.5, chunk_steps=10, workers=16, use_graph=use_graph,
+                    ghost_cells=ghost_cells) as srv:
         # capacity: a burst of the same replica mix, all present at t = 0, measured while the admission queue
         # is non-empty (from the first replica resident to the last admission), each replica's atom-steps
         # spread evenly over its residency
@@ -394,6 +397,7 @@ def main():
     ap.add_argument("--serve-replicas", type=int, default=2000)
     ap.add_argument("--serve-load", type=float, default=0.75)
     ap.add_argument("--serve-no-graph", action="store_true", help="launch the batched step directly (profiling)")
+    ap.add_argument("--serve-ghost", type=int, default=2, help="ghost shell of the serving replicas (cells)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only time a device-to-device streaming copy of BYTES (HBM bandwidth / counter calibration)")
     args = ap.parse_args()
@@ -479,7 +483,8 @@ def main():
         out["stable_potential"] = stable
 
     if not args.no_serve:
-        out["serving"] = serving(args.serve_replicas, args.serve_load, use_graph=not args.serve_no_graph)
+        out["serving"] = serving(args.serve_replicas, args.serve_load, use_graph=not args.serve_no_graph,
+                                 ghost_cells=args.serve_ghost)
         out["serving"]["capacity_vs_fp64_path"] = round(out["serving"]["capacity_atom_steps_per_s"] / value, 3)
 
     if not args.no_cpu:
