@@ -7,6 +7,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <new>
 #include <utility>
 #include <vector>
@@ -528,17 +529,20 @@ static SplHost pack_f64(Arena &A, const double *knots, int nfun, int n, double d
   return h;
 }
 
-struct MixHost {
-  size_t vb, vc, dp;
-  std::vector<char> ok;  // per segment: every function of the family meets the accuracy bounds
-};
-
 // 16-byte derivative record: FP64 base, FP32 linear and quadratic terms
 struct DerivRec {
   double base;
   float b1, b2;
 };
 static_assert(sizeof(DerivRec) == 16, "derivative record must be one 16-byte load");
+
+struct MixHost {
+  std::vector<double> vb;  // (base, a1) per record
+  std::vector<float> vc;   // (a2, a3) per record
+  std::vector<DerivRec> dp;
+  std::vector<char> ok;  // per segment: every function of the family meets the accuracy bounds
+};
+
 
 // Accuracy bounds of the r^2-keyed tables, relative error against the FP64
 // r-spline they resample, per pair term: values 1e-9; the pair-force term
@@ -557,10 +561,12 @@ constexpr double kMixedFloor = 3e-2;
 // Where the mixed tables are used: the segments from the first to the last
 // good one, minus at most kMixedHoles interior runs of failing segments (runs
 // separated by the fewest good segments are merged until that many remain;
-// the merged good segments then use the FP64 spline too).
-static void mixed_ranges(const std::vector<char> &ok_a, const std::vector<char> &ok_b, int n_s, double s0, double ds,
-                         TabMixed &tm) {
-  auto good = [&](int m) { return ok_a[(size_t)m] && ok_b[(size_t)m]; };
+// the merged good segments then use the FP64 spline too).  The first and the
+// last segment never count as good: lookups below the table, beyond it or
+// clamped to its ends land on one of them.  Returns the per-segment verdict.
+static std::vector<char> mixed_ranges(const std::vector<char> &ok_a, const std::vector<char> &ok_b, int n_s, double s0,
+                                      double ds, TabMixed &tm) {
+  auto good = [&](int m) { return m > 0 && m < n_s - 2 && ok_a[(size_t)m] && ok_b[(size_t)m]; };
   int lo = 0, hi = n_s - 1;
   while (lo < hi && !good(lo)) lo++;
   while (hi > lo && !good(hi - 1)) hi--;
@@ -581,18 +587,38 @@ static void mixed_ranges(const std::vector<char> &ok_a, const std::vector<char> 
   }
   tm.s_head = s0 + ds * (double)lo;
   tm.s_tail = s0 + ds * (double)hi;
+  std::vector<char> use((size_t)n_s, 0);
+  for (int m = lo; m < hi; m++) use[(size_t)m] = 1;
   for (int k = 0; k < kMixedHoles; k++) {
     const bool used = k < (int)runs.size();
     tm.hole_lo[k] = used ? s0 + ds * (double)runs[(size_t)k].first : 0.0;
     tm.hole_hi[k] = used ? s0 + ds * (double)runs[(size_t)k].second : 0.0;
+    if (used)
+      for (int m = runs[(size_t)k].first; m < runs[(size_t)k].second; m++) use[(size_t)m] = 0;
   }
+  return use;
+}
+
+// Marks the segments the mixed tables do not serve: the FP32 cubic term of
+// the value record and the quadratic term of the derivative record become
+// NaN, so any evaluation there yields NaN and the pair passes take the FP64
+// spline for it (a NaN test instead of comparing s against the ranges).
+static void mixed_flag(MixHost &h, int nfun, int n_s, const std::vector<char> &use) {
+  const float nan = std::numeric_limits<float>::quiet_NaN();
+  for (int f = 0; f < nfun; f++)
+    for (int m = 0; m < n_s; m++)
+      if (!use[(size_t)m]) {
+        const size_t r = (size_t)f * n_s + m;
+        h.vc[2 * r + 1] = nan;
+        h.dp[r].b2 = nan;
+      }
 }
 
 // Mixed-precision r^2-keyed family resampled from the FP64 r-keyed spline:
 // knot values and exact derivatives of g(sqrt(s)) at uniform s, cubic Hermite
 // in s; FP64 base and linear terms, FP32 quadratic and cubic terms.
-static MixHost pack_mixed(Arena &A, const double *knots, int nfun, int n_r, double r0, double dr, int n_s, double s0,
-                          double ds, double tol_deriv, std::vector<DerivRec> *dp_out = nullptr) {
+static MixHost pack_mixed(const double *knots, int nfun, int n_r, double r0, double dr, int n_s, double s0, double ds,
+                          double tol_deriv) {
   std::vector<double> c7((size_t)n_r * 7), h(n_s), hp(n_s), vb((size_t)nfun * n_s * 2);
   std::vector<float> vc((size_t)nfun * n_s * 2);
   std::vector<DerivRec> dp((size_t)nfun * n_s);
@@ -645,19 +671,19 @@ static MixHost pack_mixed(Arena &A, const double *knots, int nfun, int n_r, doub
         const double g_ex = spline7_deriv(c7.data(), n_r, r0, inv_dr, r) / (2.0 * r);
         const float c = std::fma((float)q, vc[2 * rec + 1], vc[2 * rec]);
         const double h = std::fma(q, std::fma(q, (double)c, vb[2 * rec + 1]), vb[2 * rec]);
-        const double g = std::fma(q, std::fma(q, (double)dp[rec].b2, (double)dp[rec].b1), dp[rec].base);
+        const float gc = std::fma((float)q, dp[rec].b2, dp[rec].b1);
+        const double g = std::fma(q, (double)gc, dp[rec].base);
         if (std::fabs(h - h_ex) > kMixedTolValue * (std::fabs(h_ex) + kMixedFloor * hmax) ||
             std::fabs(g - g_ex) > tol_deriv * (std::fabs(g_ex) + kMixedFloor * gmax))
           ok[(size_t)m] = 0;
       }
     }
   }
-  if (dp_out) *dp_out = dp;
   MixHost o;
   o.ok = std::move(ok);
-  o.vb = A.put(vb.data(), vb.size() * 8);
-  o.vc = A.put(vc.data(), vc.size() * 4);
-  o.dp = A.put(dp.data(), dp.size() * sizeof(DerivRec));
+  o.vb = std::move(vb);
+  o.vc = std::move(vc);
+  o.dp = std::move(dp);
   return o;
 }
 
@@ -704,7 +730,6 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
     const int n_s = c->mixed_knots > 0 ? c->mixed_knots : std::max(256, (nr + 3) / 4);
     const double rmax = t->r0 + t->dr * (double)(nr - 1);
     const double s0 = t->r0 * t->r0, ds = (rmax * rmax - s0) / (double)(n_s - 1);
-    std::vector<DerivRec> rdp, pdp;
     // largest |F'| over the central 80% of the embedding tables
     double fpmax = 0.02;
     {
@@ -714,8 +739,24 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
         for (int m = nrho / 10; m < nrho - nrho / 10; m++) fpmax = std::max(fpmax, std::fabs(ce[(size_t)m * 7 + 2]));
       }
     }
-    const MixHost m_rho = pack_mixed(A, t->rho_r, nt, nr, t->r0, t->dr, n_s, s0, ds, kMixedTolForce / fpmax, &rdp);
-    const MixHost m_phi = pack_mixed(A, t->phi_r, nt * nt, nr, t->r0, t->dr, n_s, s0, ds, kMixedTolForce, &pdp);
+    MixHost m_rho = pack_mixed(t->rho_r, nt, nr, t->r0, t->dr, n_s, s0, ds, kMixedTolForce / fpmax);
+    MixHost m_phi = pack_mixed(t->phi_r, nt * nt, nr, t->r0, t->dr, n_s, s0, ds, kMixedTolForce);
+    TabMixed tm_new = {};
+    const std::vector<char> use = mixed_ranges(m_rho.ok, m_phi.ok, n_s, s0, ds, tm_new);
+    mixed_flag(m_rho, nt, n_s, use);
+    mixed_flag(m_phi, nt * nt, n_s, use);
+    // derivative records as the force pass uses them: -2 d/ds (exact scaling;
+    // NaN flags stay NaN)
+    for (MixHost *h : {&m_rho, &m_phi})
+      for (DerivRec &d : h->dp) d.base *= -2.0, d.b1 *= -2.0f, d.b2 *= -2.0f;
+    const std::vector<DerivRec> &rdp = m_rho.dp, &pdp = m_phi.dp;
+    auto put_mixed = [&](const MixHost &h) {
+      const size_t vb = A.put(h.vb.data(), h.vb.size() * 8);
+      const size_t vc = A.put(h.vc.data(), h.vc.size() * 4);
+      const size_t dp = A.put(h.dp.data(), h.dp.size() * sizeof(DerivRec));
+      return std::array<size_t, 3>{vb, vc, dp};
+    };
+    const std::array<size_t, 3> o_rho = put_mixed(m_rho), o_phi = put_mixed(m_phi);
     const size_t h_fc = pack_force_f64(A, t->rho_r, t->phi_r, nt, nr, t->dr);
     // mixed force records per type pair (a, b): rho_b and phi_ab derivative records side by side
     std::vector<DerivRec> fb((size_t)nt * nt * n_s * 2);
@@ -742,11 +783,11 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
       s.d2 = reinterpret_cast<const double *>(dev + h.d2);
       return s;
     };
-    auto M = [&](const MixHost &h) {
+    auto M = [&](const std::array<size_t, 3> &h) {
       SplMixed s;
-      s.vb = reinterpret_cast<const double2 *>(dev + h.vb);
-      s.vc = reinterpret_cast<const float2 *>(dev + h.vc);
-      s.dp = reinterpret_cast<const double2 *>(dev + h.dp);
+      s.vb = reinterpret_cast<const double2 *>(dev + h[0]);
+      s.vc = reinterpret_cast<const float2 *>(dev + h[1]);
+      s.dp = reinterpret_cast<const double2 *>(dev + h[2]);
       return s;
     };
     TabF64 &tf = c->tf;
@@ -755,10 +796,10 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
     tf.nt = nt, tf.n_r = nr, tf.n_rho = nrho;
     tf.r0 = t->r0, tf.inv_dr = 1.0 / t->dr, tf.rho0 = t->rho0, tf.inv_drho = 1.0 / t->drho;
     TabMixed &tm = c->tm;
-    tm.rho = M(m_rho), tm.phi = M(m_phi);
+    tm = tm_new;
+    tm.rho = M(o_rho), tm.phi = M(o_phi);
     tm.fb = reinterpret_cast<const double2 *>(dev + h_fb);
     tm.nt = nt, tm.n_s = n_s, tm.s0 = s0, tm.inv_ds = 1.0 / ds;
-    mixed_ranges(m_rho.ok, m_phi.ok, n_s, s0, ds, tm);
     for (int a = 0; a < kMaxTypes; a++) {
       c->masses.m[a] = a < nt ? t->mass[a] : 1.0;
       c->masses.inv_m[a] = 1.0 / c->masses.m[a];

@@ -106,7 +106,7 @@ __device__ __forceinline__ double spl_deriv(const SplF64 &s, uint32_t rec, doubl
 // Mixed-precision spline keyed by s = r^2 (no square root or division per
 // pair).  Per knot, with q the fraction of the segment:
 //   value = base + q * (a1 + q * c),   c = a2 + q * a3   (c in FP32)
-//   d/ds  = dbase + q * (b1 + q * b2)                     (b1, b2 in FP32)
+//   d/ds  = dbase + q * e,             e = b1 + q * b2   (e in FP32)
 // The FP64 base (and the value's linear term) carry the leading digits; the
 // FP32 terms are corrections of relative size ds * f''/f' (a few per cent)
 // or smaller, so their rounding stays far below the table's interpolation
@@ -114,14 +114,16 @@ __device__ __forceinline__ double spl_deriv(const SplF64 &s, uint32_t rec, doubl
 struct SplMixed {
   const double2 *vb;  // (base, a1)
   const float2 *vc;   // (a2, a3)
-  const double2 *dp;  // (dbase, (b1, b2) as two floats)
+  const double2 *dp;  // -2 x (dbase, (b1, b2) as two floats): the force pass's form
 };
 
 struct TabMixed {
   SplMixed rho;  // [nt][n_s]
   SplMixed phi;  // [nt*nt][n_s]
   // force records per type pair (a, b) and knot: fb[2r] = rho_b derivative
-  // record, fb[2r+1] = phi_ab derivative record (two 16-byte loads per pair)
+  // record, fb[2r+1] = phi_ab derivative record (two 16-byte loads per pair).
+  // The force pass reads derivatives only from fb and rho.dp, both stored
+  // times -2 (the pair force is -2 dpsi/ds along the displacement).
   const double2 *fb;  // [nt*nt][n_s][2]
   int32_t nt, n_s;
   double s0, inv_ds;
@@ -130,29 +132,19 @@ struct TabMixed {
   // slopes leave kinks a uniform s grid cannot follow) and those in up to
   // kMixedHoles interior holes [hole_lo, hole_hi) (a tabulated function that
   // is only C1 somewhere, e.g. where a potential term reaches its own cutoff
-  // inside the table).  Unused holes are empty ([0, 0)).
+  // inside the table).  Unused holes are empty ([0, 0)).  The first and last
+  // segments are never served.  The kernels do not compare s against these
+  // bounds: every record of a segment outside them carries a NaN FP32 term,
+  // so its evaluation is NaN (host-side bookkeeping only).
   double s_head, s_tail;
   double hole_lo[2], hole_hi[2];
 };
 constexpr int kMixedHoles = 2;
 
-// Is a pair at squared distance s evaluated from the mixed tables?
-__device__ __forceinline__ bool mix_in(const TabMixed &tm, double s) {
-  return s >= tm.s_head && s < tm.s_tail && !(s >= tm.hole_lo[0] && s < tm.hole_hi[0]) &&
-         !(s >= tm.hole_lo[1] && s < tm.hole_hi[1]);
-}
-
-__device__ __forceinline__ double mix_value(const SplMixed &s, uint32_t rec, double q) {
-  const double2 b = ld(s.vb, rec);
-  const float2 c = ld(s.vc, rec);
-  const float cc = fmaf((float)q, c.y, c.x);
-  return fma(q, fma(q, (double)cc, b.y), b.x);
-}
-
-// derivative from one 16-byte record
-__device__ __forceinline__ double mix_deriv_rec(double2 d, double q) {
+// derivative from one 16-byte record; qf = (float)q
+__device__ __forceinline__ double mix_deriv_rec(double2 d, double q, float qf) {
   const float2 b = __builtin_bit_cast(float2, d.y);
-  return fma(q, fma(q, (double)b.y, (double)b.x), d.x);
+  return fma(q, (double)fmaf(qf, b.y, b.x), d.x);
 }
 
 struct Masses {

@@ -53,17 +53,20 @@ __device__ __forceinline__ Seg seg_f64(double x, double x0, double inv_dx, int n
 }
 
 struct SegS {
-  int m;
+  uint32_t m;
   double q;
 };
 
+// Segment of s = r^2 >= 0 in the mixed tables.  p is capped below n_s - 1 so
+// the conversion stays in range; p >= -s0 / ds > INT_MIN.  A p below 0 maps to
+// segment 0 (p > -1) or, through the unsigned clamp, to segment n_s - 2, like
+// every p past the table: neither segment is served by the mixed tables (their
+// records are NaN-flagged), so no clamping of q is needed.
 __device__ __forceinline__ SegS seg_s(double s2, const TabMixed &t) {
-  double p = fma(s2, t.inv_ds, -t.s0 * t.inv_ds);
-  p = fmax(p, 0.0);
-  int m = min((int)p, t.n_s - 2);
+  const double p = fmin(fma(s2, t.inv_ds, -t.s0 * t.inv_ds), (double)t.n_s - 1.5);
   SegS r;
-  r.m = m;
-  r.q = fmin(p - (double)m, 1.0);
+  r.m = min((uint32_t)(int32_t)p, (uint32_t)(t.n_s - 2));
+  r.q = p - (double)r.m;
   return r;
 }
 
@@ -73,13 +76,11 @@ __device__ __forceinline__ SegS seg_s(double s2, const TabMixed &t) {
 // every table load, then the arithmetic — with no control flow around a
 // pair's loads, so all of a trip's loads are in flight together (a branch or
 // an early use makes the compiler wait for each pair's loads before issuing
-// the next pair's).  rho'_{ti} is always loaded from the rho table at the
-// centre type: for like pairs it holds the same coefficients as the rho'_{tj}
-// part of the pair record, so no select is needed.  The mixed path evaluates
-// every lane with the r^2-keyed tables (index clamped into the table) and
-// reports whether the pair lies in the range where they meet the accuracy
-// bound; the few pairs outside it are redone with the FP64 r-spline in a
-// separate, wave-uniformly guarded pass.
+// the next pair's).  The mixed path evaluates every lane with the r^2-keyed
+// tables (index clamped into the table); segments outside the range where
+// they meet the accuracy bound hold NaN-flagged records, so a NaN result marks
+// the few pairs that are redone with the FP64 r-spline in a separate,
+// wave-uniformly guarded pass.
 
 // FP64 r-keyed spline: rho_{tj}(r), r^2 = d2
 __device__ __forceinline__ double rho_f64(const TabF64 &tf, int tj, double d2) {
@@ -118,21 +119,22 @@ __device__ __forceinline__ double fpair_f64(const TabF64 &tf, int ti, int tj, do
   return -(dfi * rhojp + dfj * rhoip + phip) * rinv;
 }
 
-// Phased evaluation of a trip (mixed): d/ds tables, -(dpsi/dr)/r = -2 dpsi/ds,
-// no square root or divide; ok[u] = pair inside the tables' checked ranges.
+// Phased evaluation of a trip (mixed): d/ds tables, -(dpsi/dr)/r = -2 dpsi/ds
+// (the force-side derivative records hold -2 dpsi/ds), no square root or
+// divide; NaN where the pair needs the FP64 spline.
+// row_i = ti * nt * n_s and ra_i = ti * n_s (the centre's rows).
 template <bool WANT_E, int U>
-__device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, int ti, const int *tj, const double *d2,
-                                                 double dfi, const double *dfj, const bool *count_e, double *fp,
-                                                 double *phi, bool *ok) {
+__device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, uint32_t ti, uint32_t row_i, uint32_t ra_i,
+                                                 const uint32_t *tj, const double *d2, double dfi, const double *dfj,
+                                                 const bool *count_e, double *fp, double *phi) {
   double q[U];
   uint32_t rp[U], ra[U];
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    ok[u] = mix_in(tm, d2[u]);
     const SegS sg = seg_s(d2[u], tm);
     q[u] = sg.q;
-    rp[u] = (uint32_t)((ti * tm.nt + tj[u]) * tm.n_s + sg.m);
-    ra[u] = (uint32_t)(ti * tm.n_s + sg.m);
+    rp[u] = __umul24(tj[u], (uint32_t)tm.n_s) + row_i + sg.m;
+    ra[u] = ra_i + sg.m;
   }
   double2 b0[U], b1[U], ua[U], vb[U];
   float2 vc[U];
@@ -150,29 +152,29 @@ __device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, int ti, con
   }
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    const double rhojp = mix_deriv_rec(b0[u], q[u]);
-    const double phip = mix_deriv_rec(b1[u], q[u]);
-    const double rhoip = ti != tj[u] ? mix_deriv_rec(ua[u], q[u]) : rhojp;
+    const float qf = (float)q[u];
+    const double rhojp = mix_deriv_rec(b0[u], q[u], qf);
+    const double phip = mix_deriv_rec(b1[u], q[u], qf);
+    const double rhoip = ti != tj[u] ? mix_deriv_rec(ua[u], q[u], qf) : rhojp;
     if (WANT_E && count_e[u]) {
-      const float cc = fmaf((float)q[u], vc[u].y, vc[u].x);
+      const float cc = fmaf(qf, vc[u].y, vc[u].x);
       phi[u] = fma(q[u], fma(q[u], (double)cc, vb[u].y), vb[u].x);
     }
-    fp[u] = -2.0 * (dfi * rhojp + dfj[u] * rhoip + phip);
+    fp[u] = fma(dfi, rhojp, fma(dfj[u], rhoip, phip));
   }
 }
 
-// Phased rho of a trip (mixed): rho_{tj}(s) with the checked-range flags.
+// Phased rho of a trip (mixed): rho_{tj}(s), NaN where the pair needs the
+// FP64 spline.
 template <int U>
-__device__ __forceinline__ void trip_rho_mixed(const TabMixed &tm, const int *tj, const double *d2, double *v,
-                                               bool *ok) {
+__device__ __forceinline__ void trip_rho_mixed(const TabMixed &tm, const uint32_t *tj, const double *d2, double *v) {
   double q[U];
   uint32_t rec[U];
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    ok[u] = mix_in(tm, d2[u]);
     const SegS sg = seg_s(d2[u], tm);
     q[u] = sg.q;
-    rec[u] = (uint32_t)(tj[u] * tm.n_s + sg.m);
+    rec[u] = __umul24(tj[u], (uint32_t)tm.n_s) + sg.m;
   }
   double2 vb[U];
   float2 vc[U];
@@ -221,11 +223,17 @@ __device__ __forceinline__ void trip_rho_f64(const TabF64 &tf, const int *tj, co
 // holds one parity), kUnroll per trip with the tables' loads of a trip issued
 // together; pairs are masked without per-pair branches and a trip with no
 // in-cutoff pair in the whole wave is skipped.  Records outside the
-// ghost-extended lattice are staged as vacancies.  The body starts with a
-// barrier so a block that walks several work items can restage for its next tile.
+// ghost-extended lattice are staged as vacancies, and vacancies at a far
+// point (kFarPos on every axis), so the mixed path screens them with the
+// cutoff test alone.  The body starts with a barrier so a block that walks
+// several work items can restage for its next tile.
 
 // LDS tile layout (n records): lxy = (x, y) [n], then lzd = (z, df) [n] for
 // the force pass or lz = z [n] for the rho pass, then the int8 types [n].
+// A vacancy's staged coordinates: any occupied site lies beyond the cutoff
+// of it (coordinates are within 1e6 A of the origin).
+constexpr double kFarPos = 1e7;
+
 template <bool WITH_DF>
 __device__ __forceinline__ void stage_tile(const Geom &g, const Halo &h, const TileSite &t, int64_t lb,
                                            const double4 *__restrict__ P, const double *__restrict__ DF,
@@ -240,6 +248,7 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const Halo &h, const T
         q = ld(P, (uint32_t)src);
         if (WITH_DF) f = ld(DF, (uint32_t)src);
       }
+      if (q.w < 0.0) q.x = q.y = q.z = kFarPos, f = 0.0;
       lxy[idx] = make_double2(q.x, q.y);
       if (WITH_DF)
         static_cast<double2 *>(lz_or_lzd)[idx] = make_double2(q.z, f);
@@ -272,52 +281,75 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
       const int nk = t.s ? nk_odd : nk_even;
       const double d2_safe = 0.25 * rc2;
       double acc = 0.0;
-      for (int k = 0; k < nk; k += kUnroll) {
-        int tj[kUnroll];
-        double d2[kUnroll];
+      if constexpr (PREC == 1) {
+        // mixed: vacancies sit at the far point, so the cutoff test screens
+        // them; their type byte (0xff) is clamped to a valid table row
+        const uint8_t *ltu = reinterpret_cast<const uint8_t *>(lt);
+        const uint32_t tmax = (uint32_t)(tm.nt - 1);
+        auto trip = [&](int k, bool tail) {
+          uint32_t tq[kUnroll];
+          double d2[kUnroll];
+          bool in[kUnroll], any = false;
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-          const int32_t j = t.c + o[k + u];
-          const double2 pxy = lxy[j];
-          tj[u] = lt[j];
-          const double dx = cxy.x - pxy.x, dy = cxy.y - pxy.y, dz = cz - lz[j];
-          d2[u] = dx * dx + dy * dy + dz * dz;
-        }
-        bool any = false;
+          for (int u = 0; u < kUnroll; u++) {
+            const int32_t j = t.c + o[k + u];
+            const double2 pxy = lxy[j];
+            tq[u] = min((uint32_t)ltu[j], tmax);
+            const double dx = cxy.x - pxy.x, dy = cxy.y - pxy.y, dz = cz - lz[j];
+            d2[u] = dx * dx + dy * dy + dz * dz;
+            in[u] = d2[u] < rc2 && (!tail || k + u < nk);
+            any |= in[u];
+          }
+          if (__ballot(any) == 0) return;
+          double v[kUnroll];
+          trip_rho_mixed<kUnroll>(tm, tq, d2, v);
+          uint64_t redo = 0;  // lanes with a pair to redo (scalar masks)
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-          const bool in = (k + u < nk) && tj[u] >= 0 && d2[u] < rc2;
-          if (!in) d2[u] = -1.0;
-          any |= in;
-        }
-        if (__ballot(any) == 0) continue;
-        double v[kUnroll];
-        int tq[kUnroll];
-        double dq[kUnroll];
-        bool redo = false;
+          for (int u = 0; u < kUnroll; u++) redo |= __ballot(in[u] && __builtin_isnan(v[u]));
+          if (redo != 0) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-          const bool in = d2[u] >= 0.0;
-          tq[u] = in ? tj[u] : 0;
-          dq[u] = in ? d2[u] : d2_safe;
-        }
-        if (PREC == 1) {
-          bool ok[kUnroll];
-          trip_rho_mixed<kUnroll>(tm, tq, dq, v, ok);
+            for (int u = 0; u < kUnroll; u++)
+              if (in[u] && __builtin_isnan(v[u])) v[u] = rho_f64(tf, (int)tq[u], d2[u]);
+          }
 #pragma unroll
-          for (int u = 0; u < kUnroll; u++) redo |= d2[u] >= 0.0 && !ok[u];
-        } else {
-          trip_rho_f64<kUnroll>(tf, tq, dq, v);
-        }
-        if (PREC == 1 && __ballot(redo) != 0) {
+          for (int u = 0; u < kUnroll; u++) acc += in[u] ? v[u] : 0.0;
+        };
+        const int nk4 = nk & ~(kUnroll - 1);
+        for (int k = 0; k < nk4; k += kUnroll) trip(k, false);
+        if (nk4 < nk) trip(nk4, true);
+      } else {
+        for (int k = 0; k < nk; k += kUnroll) {
+          int tj[kUnroll];
+          double d2[kUnroll];
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) {
+            const int32_t j = t.c + o[k + u];
+            const double2 pxy = lxy[j];
+            tj[u] = lt[j];
+            const double dx = cxy.x - pxy.x, dy = cxy.y - pxy.y, dz = cz - lz[j];
+            d2[u] = dx * dx + dy * dy + dz * dz;
+          }
+          bool any = false;
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) {
+            const bool in = (k + u < nk) && tj[u] >= 0 && d2[u] < rc2;
+            if (!in) d2[u] = -1.0;
+            any |= in;
+          }
+          if (__ballot(any) == 0) continue;
+          double v[kUnroll];
+          int tq[kUnroll];
+          double dq[kUnroll];
 #pragma unroll
           for (int u = 0; u < kUnroll; u++) {
             const bool in = d2[u] >= 0.0;
-            if (in && !mix_in(tm, d2[u])) v[u] = rho_f64(tf, tj[u], d2[u]);
+            tq[u] = in ? tj[u] : 0;
+            dq[u] = in ? d2[u] : d2_safe;
           }
-        }
+          trip_rho_f64<kUnroll>(tf, tq, dq, v);
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) acc += d2[u] >= 0.0 ? v[u] : 0.0;
+          for (int u = 0; u < kUnroll; u++) acc += d2[u] >= 0.0 ? v[u] : 0.0;
+        }
       }
       const uint32_t i = (uint32_t)t.i;
       RHO[i] = acc;
@@ -357,49 +389,87 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
       const double dfi = czd.y;
       const double d2_safe = 0.25 * rc2;
       double ax = 0.0, ay = 0.0, az = 0.0, ep = 0.0;
-      for (int k = 0; k < nk; k += kUnroll) {
-        int tj[kUnroll];
-        double dx[kUnroll], dy[kUnroll], dz[kUnroll], d2[kUnroll], dfj[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-          const int32_t j = t.c + o[k + u];
-          const double2 pxy = lxy[j], pzd = lzd[j];
-          tj[u] = lt[j];
-          dx[u] = cxy.x - pxy.x;
-          dy[u] = cxy.y - pxy.y;
-          dz[u] = czd.x - pzd.x;
-          dfj[u] = pzd.y;
-          d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-        }
-        bool any = false;
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-          const bool in = (k + u < nk) && tj[u] >= 0 && d2[u] < rc2;
-          if (!in) d2[u] = -1.0;
-          any |= in;
-        }
-        if (__ballot(any) == 0) continue;
-        double fp[kUnroll], phi[kUnroll];
-        bool ce[kUnroll];  // wave-uniform: this entry counts the pair energy
-        const uint32_t hw = WANT_E ? (uint32_t)ebits[k >> 5] : 0u;
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) ce[u] = WANT_E && ((hw >> ((k & 31) + u)) & 1u);
-        bool redo = false;
-        if (PREC == 1) {
-          int tq[kUnroll];
-          double dq[kUnroll];
+      if constexpr (PREC == 1) {
+        // mixed: vacancies sit at the far point (see rho_df_block)
+        const uint8_t *ltu = reinterpret_cast<const uint8_t *>(lt);
+        const uint32_t tmax = (uint32_t)(tm.nt - 1), ti = (uint32_t)t0;
+        const uint32_t ra_i = ti * (uint32_t)tm.n_s, row_i = ra_i * (uint32_t)tm.nt;
+        auto trip = [&](int k, bool tail) {
+          uint32_t tq[kUnroll];
+          double dx[kUnroll], dy[kUnroll], dz[kUnroll], d2[kUnroll], dfj[kUnroll];
+          bool in[kUnroll], any = false;
 #pragma unroll
           for (int u = 0; u < kUnroll; u++) {
-            const bool in = d2[u] >= 0.0;
-            phi[u] = 0.0;
-            tq[u] = in ? tj[u] : t0;
-            dq[u] = in ? d2[u] : d2_safe;
+            const int32_t j = t.c + o[k + u];
+            const double2 pxy = lxy[j], pzd = lzd[j];
+            tq[u] = min((uint32_t)ltu[j], tmax);
+            dx[u] = cxy.x - pxy.x;
+            dy[u] = cxy.y - pxy.y;
+            dz[u] = czd.x - pzd.x;
+            dfj[u] = pzd.y;
+            d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+            in[u] = d2[u] < rc2 && (!tail || k + u < nk);
+            any |= in[u];
           }
-          bool ok[kUnroll];
-          trip_force_mixed<WANT_E, kUnroll>(tm, t0, tq, dq, dfi, dfj, ce, fp, phi, ok);
+          if (__ballot(any) == 0) return;
+          double fp[kUnroll], phi[kUnroll];
+          bool ce[kUnroll];  // wave-uniform: this entry counts the pair energy
+          const uint32_t hw = WANT_E ? (uint32_t)ebits[k >> 5] : 0u;
 #pragma unroll
-          for (int u = 0; u < kUnroll; u++) redo |= d2[u] >= 0.0 && !ok[u];
-        } else {
+          for (int u = 0; u < kUnroll; u++) {
+            ce[u] = WANT_E && ((hw >> ((k & 31) + u)) & 1u);
+            phi[u] = 0.0;
+          }
+          trip_force_mixed<WANT_E, kUnroll>(tm, ti, row_i, ra_i, tq, d2, dfi, dfj, ce, fp, phi);
+          uint64_t redo = 0;  // lanes with a pair to redo (scalar masks)
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) redo |= __ballot(in[u] && __builtin_isnan(fp[u]));
+          if (redo != 0) {
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+              if (in[u] && __builtin_isnan(fp[u]))
+                fp[u] = fpair_f64<WANT_E>(tf, t0, (int)tq[u], d2[u], dfi, dfj[u], ce[u], phi[u]);
+          }
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) {
+            const double f = in[u] ? fp[u] : 0.0;
+            ax = fma(dx[u], f, ax);
+            ay = fma(dy[u], f, ay);
+            az = fma(dz[u], f, az);
+            if (WANT_E) ep += in[u] ? phi[u] : 0.0;
+          }
+        };
+        const int nk4 = nk & ~(kUnroll - 1);
+        for (int k = 0; k < nk4; k += kUnroll) trip(k, false);
+        if (nk4 < nk) trip(nk4, true);
+      } else {
+        for (int k = 0; k < nk; k += kUnroll) {
+          int tj[kUnroll];
+          double dx[kUnroll], dy[kUnroll], dz[kUnroll], d2[kUnroll], dfj[kUnroll];
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) {
+            const int32_t j = t.c + o[k + u];
+            const double2 pxy = lxy[j], pzd = lzd[j];
+            tj[u] = lt[j];
+            dx[u] = cxy.x - pxy.x;
+            dy[u] = cxy.y - pxy.y;
+            dz[u] = czd.x - pzd.x;
+            dfj[u] = pzd.y;
+            d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+          }
+          bool any = false;
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) {
+            const bool in = (k + u < nk) && tj[u] >= 0 && d2[u] < rc2;
+            if (!in) d2[u] = -1.0;
+            any |= in;
+          }
+          if (__ballot(any) == 0) continue;
+          double fp[kUnroll], phi[kUnroll];
+          bool ce[kUnroll];  // wave-uniform: this entry counts the pair energy
+          const uint32_t hw = WANT_E ? (uint32_t)ebits[k >> 5] : 0u;
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) ce[u] = WANT_E && ((hw >> ((k & 31) + u)) & 1u);
           // FP64: pair by pair (its large tables are gathered from L2; this
           // measured faster than phased loads)
 #pragma unroll
@@ -408,23 +478,15 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
             phi[u] = 0.0;
             fp[u] = fpair_f64<WANT_E>(tf, t0, in ? tj[u] : t0, in ? d2[u] : d2_safe, dfi, dfj[u], ce[u], phi[u]);
           }
-        }
-        if (PREC == 1 && __ballot(redo) != 0) {
 #pragma unroll
           for (int u = 0; u < kUnroll; u++) {
             const bool in = d2[u] >= 0.0;
-            if (in && !mix_in(tm, d2[u]))
-              fp[u] = fpair_f64<WANT_E>(tf, t0, tj[u], d2[u], dfi, dfj[u], ce[u], phi[u]);
+            const double f = in ? fp[u] : 0.0;
+            ax = fma(dx[u], f, ax);
+            ay = fma(dy[u], f, ay);
+            az = fma(dz[u], f, az);
+            if (WANT_E) ep += in ? phi[u] : 0.0;
           }
-        }
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-          const bool in = d2[u] >= 0.0;
-          const double f = in ? fp[u] : 0.0;
-          ax = fma(dx[u], f, ax);
-          ay = fma(dy[u], f, ay);
-          az = fma(dz[u], f, az);
-          if (WANT_E) ep += in ? phi[u] : 0.0;
         }
       }
       const uint32_t i = (uint32_t)t.i;
