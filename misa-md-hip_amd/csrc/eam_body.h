@@ -41,9 +41,13 @@ struct Seg {
   double p;
 };
 
+// Segment and fraction of x in a table of n knots, clamped to the first and
+// last segment.  p is capped at n - 1 before the conversion, which leaves the
+// result unchanged (the fraction is capped at 1 anyway) and keeps any x --
+// the far point of a vacancy included -- in the converter's range.
 __device__ __forceinline__ Seg seg_f64(double x, double x0, double inv_dx, int n) {
   double p = (x - x0) * inv_dx;
-  p = fmax(p, 0.0);
+  p = fmin(fmax(p, 0.0), (double)(n - 1));
   int m = (int)p;
   m = min(m, n - 2);
   Seg s;
@@ -97,9 +101,9 @@ __device__ __forceinline__ double fpair_f64(const TabF64 &tf, int ti, int tj, do
                                             bool count_e, double &phi) {
   const double rinv = rsqrt64(d2);
   const Seg sg = seg_f64(d2 * rinv, tf.r0, tf.inv_dr, tf.n_r);
-  const uint32_t rp = (uint32_t)((ti * tf.nt + tj) * tf.n_r + sg.m);
+  const uint32_t rp = __umul24((uint32_t)(ti * tf.nt + tj), (uint32_t)tf.n_r) + (uint32_t)sg.m;
   const bool unlike = ti != tj;
-  const uint32_t ra = unlike ? (uint32_t)(ti * tf.n_r + sg.m) : 0u;
+  const uint32_t ra = unlike ? __umul24((uint32_t)ti, (uint32_t)tf.n_r) + (uint32_t)sg.m : 0u;
   const double2 c0 = ld(tf.fc, 3 * rp), c1 = ld(tf.fc, 3 * rp + 1), c2 = ld(tf.fc, 3 * rp + 2);
   // rho'_{ti} only for unlike pairs: the load runs on those lanes alone (the
   // load unit's time follows the active lanes; out-of-cutoff lanes are too
@@ -192,7 +196,7 @@ __device__ __forceinline__ void trip_rho_mixed(const TabMixed &tm, const uint32_
 
 // Phased rho of a trip (FP64)
 template <int U>
-__device__ __forceinline__ void trip_rho_f64(const TabF64 &tf, const int *tj, const double *d2, double *v) {
+__device__ __forceinline__ void trip_rho_f64(const TabF64 &tf, const uint32_t *tj, const double *d2, double *v) {
   double p[U];
   uint32_t rec[U];
 #pragma unroll
@@ -200,7 +204,7 @@ __device__ __forceinline__ void trip_rho_f64(const TabF64 &tf, const int *tj, co
     const double r = d2[u] * rsqrt64(d2[u]);
     const Seg sg = seg_f64(r, tf.r0, tf.inv_dr, tf.n_r);
     p[u] = sg.p;
-    rec[u] = (uint32_t)(tj[u] * tf.n_r + sg.m);
+    rec[u] = __umul24(tj[u], (uint32_t)tf.n_r) + (uint32_t)sg.m;
   }
   double2 a[U], b[U];
 #pragma unroll
@@ -279,29 +283,28 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
     if (t0 >= 0) {
       const int32_t *o = off + t.s * kMaxStencil;
       const int nk = t.s ? nk_odd : nk_even;
-      const double d2_safe = 0.25 * rc2;
       double acc = 0.0;
-      if constexpr (PREC == 1) {
-        // mixed: vacancies sit at the far point, so the cutoff test screens
-        // them; their type byte (0xff) is clamped to a valid table row
-        const uint8_t *ltu = reinterpret_cast<const uint8_t *>(lt);
-        const uint32_t tmax = (uint32_t)(tm.nt - 1);
-        auto trip = [&](int k, bool tail) {
-          uint32_t tq[kUnroll];
-          double d2[kUnroll];
-          bool in[kUnroll], any = false;
+      // vacancies sit at the far point, so the cutoff test screens them;
+      // their type byte (0xff) is clamped to a valid table row
+      const uint8_t *ltu = reinterpret_cast<const uint8_t *>(lt);
+      const uint32_t tmax = (uint32_t)(tf.nt - 1);
+      auto trip = [&](int k, bool tail) {
+        uint32_t tq[kUnroll];
+        double d2[kUnroll];
+        bool in[kUnroll], any = false;
 #pragma unroll
-          for (int u = 0; u < kUnroll; u++) {
-            const int32_t j = t.c + o[k + u];
-            const double2 pxy = lxy[j];
-            tq[u] = min((uint32_t)ltu[j], tmax);
-            const double dx = cxy.x - pxy.x, dy = cxy.y - pxy.y, dz = cz - lz[j];
-            d2[u] = dx * dx + dy * dy + dz * dz;
-            in[u] = d2[u] < rc2 && (!tail || k + u < nk);
-            any |= in[u];
-          }
-          if (__ballot(any) == 0) return;
-          double v[kUnroll];
+        for (int u = 0; u < kUnroll; u++) {
+          const int32_t j = t.c + o[k + u];
+          const double2 pxy = lxy[j];
+          tq[u] = min((uint32_t)ltu[j], tmax);
+          const double dx = cxy.x - pxy.x, dy = cxy.y - pxy.y, dz = cz - lz[j];
+          d2[u] = dx * dx + dy * dy + dz * dz;
+          in[u] = d2[u] < rc2 && (!tail || k + u < nk);
+          any |= in[u];
+        }
+        if (__ballot(any) == 0) return;
+        double v[kUnroll];
+        if constexpr (PREC == 1) {
           trip_rho_mixed<kUnroll>(tm, tq, d2, v);
           uint64_t redo = 0;  // lanes with a pair to redo (scalar masks)
 #pragma unroll
@@ -311,46 +314,15 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
             for (int u = 0; u < kUnroll; u++)
               if (in[u] && __builtin_isnan(v[u])) v[u] = rho_f64(tf, (int)tq[u], d2[u]);
           }
-#pragma unroll
-          for (int u = 0; u < kUnroll; u++) acc += in[u] ? v[u] : 0.0;
-        };
-        const int nk4 = nk & ~(kUnroll - 1);
-        for (int k = 0; k < nk4; k += kUnroll) trip(k, false);
-        if (nk4 < nk) trip(nk4, true);
-      } else {
-        for (int k = 0; k < nk; k += kUnroll) {
-          int tj[kUnroll];
-          double d2[kUnroll];
-#pragma unroll
-          for (int u = 0; u < kUnroll; u++) {
-            const int32_t j = t.c + o[k + u];
-            const double2 pxy = lxy[j];
-            tj[u] = lt[j];
-            const double dx = cxy.x - pxy.x, dy = cxy.y - pxy.y, dz = cz - lz[j];
-            d2[u] = dx * dx + dy * dy + dz * dz;
-          }
-          bool any = false;
-#pragma unroll
-          for (int u = 0; u < kUnroll; u++) {
-            const bool in = (k + u < nk) && tj[u] >= 0 && d2[u] < rc2;
-            if (!in) d2[u] = -1.0;
-            any |= in;
-          }
-          if (__ballot(any) == 0) continue;
-          double v[kUnroll];
-          int tq[kUnroll];
-          double dq[kUnroll];
-#pragma unroll
-          for (int u = 0; u < kUnroll; u++) {
-            const bool in = d2[u] >= 0.0;
-            tq[u] = in ? tj[u] : 0;
-            dq[u] = in ? d2[u] : d2_safe;
-          }
-          trip_rho_f64<kUnroll>(tf, tq, dq, v);
-#pragma unroll
-          for (int u = 0; u < kUnroll; u++) acc += d2[u] >= 0.0 ? v[u] : 0.0;
+        } else {
+          trip_rho_f64<kUnroll>(tf, tq, d2, v);
         }
-      }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) acc += in[u] ? v[u] : 0.0;
+      };
+      const int nk4 = nk & ~(kUnroll - 1);
+      for (int k = 0; k < nk4; k += kUnroll) trip(k, false);
+      if (nk4 < nk) trip(nk4, true);
       const uint32_t i = (uint32_t)t.i;
       RHO[i] = acc;
       const Seg se = seg_f64(acc, tf.rho0, tf.inv_drho, tf.n_rho);
@@ -387,39 +359,38 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
       const double ew = off[kStencilHalfFlag] ? 1.0 : 0.5;
       const int nk = t.s ? nk_odd : nk_even;
       const double dfi = czd.y;
-      const double d2_safe = 0.25 * rc2;
       double ax = 0.0, ay = 0.0, az = 0.0, ep = 0.0;
-      if constexpr (PREC == 1) {
-        // mixed: vacancies sit at the far point (see rho_df_block)
-        const uint8_t *ltu = reinterpret_cast<const uint8_t *>(lt);
-        const uint32_t tmax = (uint32_t)(tm.nt - 1), ti = (uint32_t)t0;
-        const uint32_t ra_i = ti * (uint32_t)tm.n_s, row_i = ra_i * (uint32_t)tm.nt;
-        auto trip = [&](int k, bool tail) {
-          uint32_t tq[kUnroll];
-          double dx[kUnroll], dy[kUnroll], dz[kUnroll], d2[kUnroll], dfj[kUnroll];
-          bool in[kUnroll], any = false;
+      // vacancies sit at the far point (see rho_df_block)
+      const uint8_t *ltu = reinterpret_cast<const uint8_t *>(lt);
+      const uint32_t tmax = (uint32_t)(tf.nt - 1), ti = (uint32_t)t0;
+      const uint32_t ra_i = ti * (uint32_t)tm.n_s, row_i = ra_i * (uint32_t)tm.nt;
+      auto trip = [&](int k, bool tail) {
+        uint32_t tq[kUnroll];
+        double dx[kUnroll], dy[kUnroll], dz[kUnroll], d2[kUnroll], dfj[kUnroll];
+        bool in[kUnroll], any = false;
 #pragma unroll
-          for (int u = 0; u < kUnroll; u++) {
-            const int32_t j = t.c + o[k + u];
-            const double2 pxy = lxy[j], pzd = lzd[j];
-            tq[u] = min((uint32_t)ltu[j], tmax);
-            dx[u] = cxy.x - pxy.x;
-            dy[u] = cxy.y - pxy.y;
-            dz[u] = czd.x - pzd.x;
-            dfj[u] = pzd.y;
-            d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-            in[u] = d2[u] < rc2 && (!tail || k + u < nk);
-            any |= in[u];
-          }
-          if (__ballot(any) == 0) return;
-          double fp[kUnroll], phi[kUnroll];
-          bool ce[kUnroll];  // wave-uniform: this entry counts the pair energy
-          const uint32_t hw = WANT_E ? (uint32_t)ebits[k >> 5] : 0u;
+        for (int u = 0; u < kUnroll; u++) {
+          const int32_t j = t.c + o[k + u];
+          const double2 pxy = lxy[j], pzd = lzd[j];
+          tq[u] = min((uint32_t)ltu[j], tmax);
+          dx[u] = cxy.x - pxy.x;
+          dy[u] = cxy.y - pxy.y;
+          dz[u] = czd.x - pzd.x;
+          dfj[u] = pzd.y;
+          d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
+          in[u] = d2[u] < rc2 && (!tail || k + u < nk);
+          any |= in[u];
+        }
+        if (__ballot(any) == 0) return;
+        double fp[kUnroll], phi[kUnroll];
+        bool ce[kUnroll];  // wave-uniform: this entry counts the pair energy
+        const uint32_t hw = WANT_E ? (uint32_t)ebits[k >> 5] : 0u;
 #pragma unroll
-          for (int u = 0; u < kUnroll; u++) {
-            ce[u] = WANT_E && ((hw >> ((k & 31) + u)) & 1u);
-            phi[u] = 0.0;
-          }
+        for (int u = 0; u < kUnroll; u++) {
+          ce[u] = WANT_E && ((hw >> ((k & 31) + u)) & 1u);
+          phi[u] = 0.0;
+        }
+        if constexpr (PREC == 1) {
           trip_force_mixed<WANT_E, kUnroll>(tm, ti, row_i, ra_i, tq, d2, dfi, dfj, ce, fp, phi);
           uint64_t redo = 0;  // lanes with a pair to redo (scalar masks)
 #pragma unroll
@@ -430,65 +401,25 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
               if (in[u] && __builtin_isnan(fp[u]))
                 fp[u] = fpair_f64<WANT_E>(tf, t0, (int)tq[u], d2[u], dfi, dfj[u], ce[u], phi[u]);
           }
-#pragma unroll
-          for (int u = 0; u < kUnroll; u++) {
-            const double f = in[u] ? fp[u] : 0.0;
-            ax = fma(dx[u], f, ax);
-            ay = fma(dy[u], f, ay);
-            az = fma(dz[u], f, az);
-            if (WANT_E) ep += in[u] ? phi[u] : 0.0;
-          }
-        };
-        const int nk4 = nk & ~(kUnroll - 1);
-        for (int k = 0; k < nk4; k += kUnroll) trip(k, false);
-        if (nk4 < nk) trip(nk4, true);
-      } else {
-        for (int k = 0; k < nk; k += kUnroll) {
-          int tj[kUnroll];
-          double dx[kUnroll], dy[kUnroll], dz[kUnroll], d2[kUnroll], dfj[kUnroll];
-#pragma unroll
-          for (int u = 0; u < kUnroll; u++) {
-            const int32_t j = t.c + o[k + u];
-            const double2 pxy = lxy[j], pzd = lzd[j];
-            tj[u] = lt[j];
-            dx[u] = cxy.x - pxy.x;
-            dy[u] = cxy.y - pxy.y;
-            dz[u] = czd.x - pzd.x;
-            dfj[u] = pzd.y;
-            d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-          }
-          bool any = false;
-#pragma unroll
-          for (int u = 0; u < kUnroll; u++) {
-            const bool in = (k + u < nk) && tj[u] >= 0 && d2[u] < rc2;
-            if (!in) d2[u] = -1.0;
-            any |= in;
-          }
-          if (__ballot(any) == 0) continue;
-          double fp[kUnroll], phi[kUnroll];
-          bool ce[kUnroll];  // wave-uniform: this entry counts the pair energy
-          const uint32_t hw = WANT_E ? (uint32_t)ebits[k >> 5] : 0u;
-#pragma unroll
-          for (int u = 0; u < kUnroll; u++) ce[u] = WANT_E && ((hw >> ((k & 31) + u)) & 1u);
+        } else {
           // FP64: pair by pair (its large tables are gathered from L2; this
           // measured faster than phased loads)
 #pragma unroll
-          for (int u = 0; u < kUnroll; u++) {
-            const bool in = d2[u] >= 0.0;
-            phi[u] = 0.0;
-            fp[u] = fpair_f64<WANT_E>(tf, t0, in ? tj[u] : t0, in ? d2[u] : d2_safe, dfi, dfj[u], ce[u], phi[u]);
-          }
-#pragma unroll
-          for (int u = 0; u < kUnroll; u++) {
-            const bool in = d2[u] >= 0.0;
-            const double f = in ? fp[u] : 0.0;
-            ax = fma(dx[u], f, ax);
-            ay = fma(dy[u], f, ay);
-            az = fma(dz[u], f, az);
-            if (WANT_E) ep += in ? phi[u] : 0.0;
-          }
+          for (int u = 0; u < kUnroll; u++)
+            fp[u] = fpair_f64<WANT_E>(tf, t0, (int)tq[u], d2[u], dfi, dfj[u], ce[u], phi[u]);
         }
-      }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+          const double f = in[u] ? fp[u] : 0.0;
+          ax = fma(dx[u], f, ax);
+          ay = fma(dy[u], f, ay);
+          az = fma(dz[u], f, az);
+          if (WANT_E) ep += in[u] ? phi[u] : 0.0;
+        }
+      };
+      const int nk4 = nk & ~(kUnroll - 1);
+      for (int k = 0; k < nk4; k += kUnroll) trip(k, false);
+      if (nk4 < nk) trip(nk4, true);
       const uint32_t i = (uint32_t)t.i;
       FX[i] = ax;
       FY[i] = ay;
