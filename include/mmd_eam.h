@@ -122,7 +122,8 @@ int mmd_set_tables(mmd_ctx *ctx, const mmd_eam_tables *tab);
 int mmd_set_precision(mmd_ctx *ctx, int32_t mode);
 /* Where MMD_PREC_MIXED evaluates pairs from its r^2-keyed tables, as squared
  * distances: out[0..1] = [head, tail), out[2..5] = two interior holes
- * [lo, hi) (empty as [0, 0)); pairs elsewhere use the FP64 spline.  No
+ * [lo, hi) (empty as [0, 0)), all on table knots (whole segments); pairs
+ * elsewhere use the FP64 spline.  No
  * counterpart in the reference (FP64 only); for callers that check how much
  * of their workload the fast path covers. */
 int mmd_mixed_ranges(mmd_ctx *ctx, double out[6]);
@@ -130,7 +131,9 @@ int mmd_mixed_ranges(mmd_ctx *ctx, double out[6]);
 /* ------------------------------------------------------------ data movement */
 
 /* Copies the masked fields of all ext sites host -> device (reference layout
- * on the host).  Synchronous. */
+ * on the host).  Synchronous.  Coordinates must lie within 1e6 A of the
+ * origin: the pair passes stage vacant sites at a far point (1e7 A on every
+ * axis) so that the cutoff test alone screens them. */
 int mmd_upload(mmd_ctx *ctx, uint32_t fields, const mmd_host_soa *h);
 /* Device -> host for all ext sites.  Synchronous. */
 int mmd_download(mmd_ctx *ctx, uint32_t fields, const mmd_host_soa *h);
