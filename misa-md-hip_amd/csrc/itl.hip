@@ -235,6 +235,18 @@ __device__ __forceinline__ double itl_d2(const double4 &q, double px, double py,
   return dx * dx + dy * dy + dz * dz;
 }
 
+// Radial derivatives of a pair (a, b) at segment m, fraction p: rho_b' and
+// phi_ab' from the combined force record (three 16-byte loads, the same
+// coefficients as the per-function tables), rho_a' read only when a != b.
+__device__ __forceinline__ void itl_derivs(const TabF64 &tf, int a, int b, int m, double p, double &rho_b,
+                                           double &phi_ab, double &rho_a) {
+  const uint32_t rp = (uint32_t)((a * tf.nt + b) * tf.n_r + m);
+  const double2 c0 = ld(tf.fc, 3 * rp), c1 = ld(tf.fc, 3 * rp + 1), c2 = ld(tf.fc, 3 * rp + 2);
+  rho_b = (c0.x * p + c0.y) * p + c1.x;
+  phi_ab = (c1.y * p + c2.x) * p + c2.y;
+  rho_a = a == b ? rho_b : spl_deriv(tf.rho, (uint32_t)(a * tf.n_r + m), p);
+}
+
 // LDS written by some lanes of a wave and read by others: order the wave's
 // LDS accesses (all lanes of the wave run the same path here)
 __device__ __forceinline__ void wave_lds_sync() {
@@ -250,12 +262,14 @@ constexpr int kItlChunk = kItlPer * kWave;  // entries tested per chunk
 // MODE 1: force (+ owner-side lattice force update).
 template <int MODE, bool WANT_E>
 __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const double4 *__restrict__ P,
-                                                       const int8_t *__restrict__ T, double *__restrict__ RHO,
+                                                       double *__restrict__ RHO,
                                                        double *__restrict__ DF, double *__restrict__ FX,
                                                        double *__restrict__ FY, double *__restrict__ FZ, TabF64 tf,
                                                        double *__restrict__ e_part) {
   __shared__ double lds[kWavesPerBlock];
-  __shared__ int32_t wlist[kWavesPerBlock][kItlChunk];  // in-cutoff entries of the current chunk, per wave
+  // in-cutoff entries of the current chunk, per wave: wide-stencil index | site type << 24, and the site's position
+  __shared__ int32_t wlist[kWavesPerBlock][kItlChunk];
+  __shared__ double3 wpos[kWavesPerBlock][kItlChunk];
   const int lane = threadIdx.x & 63;
   const int k = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
   double acc = 0.0, fxk = 0.0, fyk = 0.0, fzk = 0.0, e_loc = 0.0;
@@ -268,43 +282,46 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
     const double dfk = MODE == 1 ? I.df[k] : 0.0;
     const double pxk = I.px[k], pyk = I.py[k], pzk = I.pz[k];
     int32_t *cw = wlist[threadIdx.x >> 6];
+    double3 *cq = wpos[threadIdx.x >> 6];
     for (int w0 = 0; w0 < I.n_wide; w0 += kItlChunk) {
-      // phase 1: every lane tests its entries of the chunk, the site-type and
-      // position loads of all of them in flight together; the entries inside
-      // the cutoff are compacted in stencil order
+      // phase 1: every lane tests its entries of the chunk, the record loads
+      // (position and type) of all of them in flight together; the entries
+      // inside the cutoff are compacted in stencil order with their records
       int32_t found = 0;
       {
-        int64_t ee[kItlPer];
-        int tt[kItlPer];
         double4 qq[kItlPer];
 #pragma unroll
         for (int j = 0; j < kItlPer; j++) {
           const int w = w0 + j * kWave + lane;
-          ee[j] = w < I.n_wide ? wide_site(g, I, aX, aY, aZ, sgn, w) : 0;
-          tt[j] = w < I.n_wide ? T[ee[j]] : -1;
-          qq[j] = P[ee[j]];
+          qq[j] = P[w < I.n_wide ? wide_site(g, I, aX, aY, aZ, sgn, w) : 0];
         }
 #pragma unroll
         for (int j = 0; j < kItlPer; j++) {
+          const int w = w0 + j * kWave + lane;
+          const int tj = w < I.n_wide ? (int)qq[j].w : -1;
           double dx, dy, dz;
-          const bool in = tt[j] >= 0 && itl_d2(qq[j], pxk, pyk, pzk, dx, dy, dz) < I.rc2;
+          const bool in = tj >= 0 && itl_d2(qq[j], pxk, pyk, pzk, dx, dy, dz) < I.rc2;
           const uint64_t m = __ballot(in);
-          if (in) cw[found + __popcll(m & ((1ull << lane) - 1))] = w0 + j * kWave + lane;
+          if (in) {
+            const int slot = found + __popcll(m & ((1ull << lane) - 1));
+            cw[slot] = w | (tj << 24);
+            cq[slot] = make_double3(qq[j].x, qq[j].y, qq[j].z);
+          }
           found += __popcll(m);
         }
       }
       wave_lds_sync();
       // phase 2: one in-cutoff pair per lane (rarely two)
       for (int c = lane; c < found; c += kWave) {
-        const int w = cw[c];
+        const int32_t wt = cw[c];
+        const int w = wt & 0xffffff, ti = wt >> 24;
         const int xe = aX + sgn * I.wide[3 * w] + 2 * g.gxh, ye = aY + sgn * I.wide[3 * w + 1] + g.gy,
                   ze = aZ + sgn * I.wide[3 * w + 2] + g.gz;
         const int64_t e = dev_index(g, xe & 1, xe >> 1, ye, ze);
-        const int ti = T[e];
-        const double4 q = P[e];
+        const double3 qp = cq[c];
         double dx, dy, dz;
-        const double d2 = itl_d2(q, pxk, pyk, pzk, dx, dy, dz);
-        if (ti < 0 || d2 >= I.rc2) continue;
+        const double d2 = itl_d2(make_double4(qp.x, qp.y, qp.z, 0.0), pxk, pyk, pzk, dx, dy, dz);
+        if (d2 >= I.rc2) continue;  // the phase-1 test, repeated on the same values
         const double r = sqrt(d2);
         double p;
         const int m = seg(r, tf.r0, tf.inv_dr, tf.n_r, p);
@@ -315,8 +332,9 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
           acc += v_k;
         } else {
           const double dfi = DF[e];
-          const double psip = dfi * spl_deriv(tf.rho, tk * tf.n_r + m, p) + dfk * spl_deriv(tf.rho, ti * tf.n_r + m, p) +
-                              spl_deriv(tf.phi, (ti * tf.nt + tk) * tf.n_r + m, p);
+          double rho_k, phi_ik, rho_i;
+          itl_derivs(tf, ti, tk, m, p, rho_k, phi_ik, rho_i);
+          const double psip = dfi * rho_k + dfk * rho_i + phi_ik;
           fp_k = -psip / r;  // force on the lattice atom along del = x_i - x_k
           fxk -= dx * fp_k, fyk -= dy * fp_k, fzk -= dz * fp_k;
         }
@@ -365,9 +383,9 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
           if (MODE == 0) {
             rho_i += spl_value(tf.rho, tm * tf.n_r + mb, pb);
           } else {
-            const double psip = dfi * spl_deriv(tf.rho, tm * tf.n_r + mb, pb) +
-                                I.df[mm] * spl_deriv(tf.rho, ti * tf.n_r + mb, pb) +
-                                spl_deriv(tf.phi, (ti * tf.nt + tm) * tf.n_r + mb, pb);
+            double rho_m, phi_im, rho_i;
+            itl_derivs(tf, ti, tm, mb, pb, rho_m, phi_im, rho_i);
+            const double psip = dfi * rho_m + I.df[mm] * rho_i + phi_im;
             const double fp = -psip / rb;
             gx += ex * fp, gy += ey * fp, gz += ez * fp;
             if (WANT_E) epair += spl_value(tf.phi, (ti * tf.nt + tm) * tf.n_r + mb, pb);
@@ -404,8 +422,9 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
       if (MODE == 0) {
         acc += spl_value(tf.rho, tm * tf.n_r + m, p);
       } else {
-        const double psip = dfk * spl_deriv(tf.rho, tm * tf.n_r + m, p) + I.df[mm] * spl_deriv(tf.rho, tk * tf.n_r + m, p) +
-                            spl_deriv(tf.phi, (tk * tf.nt + tm) * tf.n_r + m, p);
+        double rho_m, phi_km, rho_k;
+        itl_derivs(tf, tk, tm, m, p, rho_m, phi_km, rho_k);
+        const double psip = dfk * rho_m + I.df[mm] * rho_k + phi_km;
         const double fp = -psip / r;
         fxk += dx * fp, fyk += dy * fp, fzk += dz * fp;
         if (WANT_E) e_loc += 0.5 * spl_value(tf.phi, (tk * tf.nt + tm) * tf.n_r + m, p);
@@ -803,10 +822,10 @@ int itl_rho_df(mmd_ctx *c, double cutoff, bool want_e) {
   ItlDev I = make_dev(c);
   const dim3 grid((c->n_itl + kWavesPerBlock - 1) / kWavesPerBlock);
   if (want_e)
-    k_itl_pass<0, true><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->type, c->rho, c->df, c->fx,
+    k_itl_pass<0, true><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx,
                                                           c->fy, c->fz, c->tf, c->e_itl);
   else
-    k_itl_pass<0, false><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->type, c->rho, c->df, c->fx,
+    k_itl_pass<0, false><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx,
                                                            c->fy, c->fz, c->tf, c->e_itl);
   ITL_TRY(hipGetLastError());
   return timed_end(c, MMD_K_ITL, ev);
@@ -833,10 +852,10 @@ int itl_force(mmd_ctx *c, double cutoff, bool want_e) {
   // the embedding-energy partials of the rho pass live in the first half of e_itl
   double *ep = c->e_itl + c->itl_part_n / 2;
   if (want_e)
-    k_itl_pass<1, true><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->type, c->rho, c->df, c->fx,
+    k_itl_pass<1, true><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx,
                                                           c->fy, c->fz, c->tf, ep);
   else
-    k_itl_pass<1, false><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->type, c->rho, c->df, c->fx,
+    k_itl_pass<1, false><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx,
                                                            c->fy, c->fz, c->tf, ep);
   ITL_TRY(hipGetLastError());
   return timed_end(c, MMD_K_ITL, ev);
