@@ -373,6 +373,7 @@ extern "C" int mmd_ctx_destroy(mmd_ctx *c) {
     free_state(c);
     itl_free(c);
     dfree(c->d_off), dfree(c->d_tab), dfree(c->d_mixed), dfree(c->d_energy), dfree(c->d_counter);
+    dfree(c->d_maxtype);
     for (auto &p : c->pending) c->event_pool.push_back(p.start), c->event_pool.push_back(p.stop);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
@@ -1008,23 +1009,7 @@ static int aos_pass(mmd_ctx *c, void *atoms, const mmd_aos_layout *lay, uint32_t
   int rc = aos_desc(lay, &d);
   if (rc) return rc;
   const size_t bytes = (size_t)c->g.n_ext * lay->stride;
-  if (in & MMD_F_TYPE) {  // validate types on the host before they index device tables
-    const char *base = static_cast<const char *>(atoms) + lay->off_type;
-    int32_t mx = -1;
-    for (int64_t i = 0; i < c->g.n_ext; i++) {
-      const char *p = base + (size_t)i * lay->stride;
-      int32_t t;
-      if (lay->type_bytes == 1)
-        t = *reinterpret_cast<const int8_t *>(p);
-      else if (lay->type_bytes == 2)
-        t = *reinterpret_cast<const int16_t *>(p);
-      else
-        t = *reinterpret_cast<const int32_t *>(p);
-      mx = std::max(mx, t);
-    }
-    if (mx >= c->n_types) return fail(MMD_E_INVALID, "atom type %d has no table (n_types = %d)", mx, c->n_types);
-    c->max_type = mx;
-  }
+  if (!c->d_maxtype && (rc = dalloc(&c->d_maxtype, 1))) return rc;
   if (bytes > c->aos_bytes) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     dfree(c->aos);
@@ -1033,8 +1018,17 @@ static int aos_pass(mmd_ctx *c, void *atoms, const mmd_aos_layout *lay, uint32_t
     c->aos_bytes = bytes;
   }
   HIP_TRY(hipMemcpyAsync(c->aos, atoms, bytes, hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(launch_aos_unpack(c->stream, c->g, c->aos, d, in, c->pos, c->type, c->df, c->rho));
+  if (in & MMD_F_TYPE) HIP_TRY(launch_fill_i32(c->stream, c->d_maxtype, 1, -1));
+  HIP_TRY(launch_aos_unpack(c->stream, c->g, c->aos, d, in, c->pos, c->type, c->df, c->rho, c->n_types,
+                            c->d_maxtype));
   if (in & MMD_F_X) c->itl_stale = true;
+  if (in & MMD_F_TYPE) {  // types are checked on the device (a type with no table was stored as a vacancy)
+    int32_t mx = -1;
+    HIP_TRY(hipMemcpyAsync(&mx, c->d_maxtype, sizeof(mx), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->max_type = std::min(mx, c->n_types - 1);
+    if (mx >= c->n_types) return fail(MMD_E_INVALID, "atom type %d has no table (n_types = %d)", mx, c->n_types);
+  }
   if (pass == 0) rc = run_rho_df(c, cutoff, false);
   if (pass == 1) {
     HIP_TRY(launch_df(c->stream, c->g, c->type, c->rho, c->df, c->tf));

@@ -39,7 +39,7 @@ struct AosDesc {
   int32_t type_bytes;
 };
 hipError_t launch_aos_unpack(hipStream_t st, const Geom &g, const char *aos, AosDesc d, uint32_t fields, double4 *P,
-                             int8_t *T, double *DF, double *RHO);
+                             int8_t *T, double *DF, double *RHO, int32_t n_types, int32_t *max_type);
 hipError_t launch_aos_pack(hipStream_t st, const Geom &g, char *aos, AosDesc d, uint32_t fields, const int8_t *T,
                            const double *RHO, const double *DF, const double *FX, const double *FY,
                            const double *FZ);

@@ -46,23 +46,36 @@ __global__ __launch_bounds__(kBlock) void k_dev2ref_pos(Geom g, const double4 *_
   rx[r] = p.x, ry[r] = p.y, rz[r] = p.z;
 }
 
+// Types are checked here: a type with no table (>= n_types) is stored as a
+// vacancy, so device state never indexes a table out of range, and the
+// largest type read goes to *max_type (one atomic per wave) for the caller to
+// reject the call.
 __global__ __launch_bounds__(kBlock) void k_aos_unpack(Geom g, const char *__restrict__ aos, AosDesc d,
                                                          uint32_t fields, double4 *__restrict__ P,
                                                          int8_t *__restrict__ T, double *__restrict__ DF,
-                                                         double *__restrict__ RHO) {
+                                                         double *__restrict__ RHO, int32_t n_types,
+                                                         int32_t *__restrict__ max_type) {
   const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
-  if (idx >= (uint32_t)g.n_ext) return;
-  const char *rec = aos + ref_of(g, ext_of(g, idx)) * d.stride;
-  double4 p = P[idx];
-  if (fields & 0x20u) {
-    int32_t t;
+  const bool live = idx < (uint32_t)g.n_ext;
+  const char *rec = aos + (live ? ref_of(g, ext_of(g, idx)) * d.stride : 0);
+  int32_t t = -1;
+  if (live && (fields & 0x20u)) {
     if (d.type_bytes == 1)
       t = *reinterpret_cast<const int8_t *>(rec + d.off_type);
     else if (d.type_bytes == 2)
       t = *reinterpret_cast<const int16_t *>(rec + d.off_type);
     else
       t = *reinterpret_cast<const int32_t *>(rec + d.off_type);
-    const int8_t t8 = (int8_t)(t < 0 ? -1 : t);
+  }
+  if (fields & 0x20u) {
+    int32_t m = t;
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, kWave));
+    if ((threadIdx.x & 63) == 0 && m >= 0) atomicMax(max_type, m);
+  }
+  if (!live) return;
+  double4 p = P[idx];
+  if (fields & 0x20u) {
+    const int8_t t8 = (int8_t)(t < 0 || t >= n_types ? -1 : t);
     T[idx] = t8;
     p.w = (double)t8;
   }
@@ -200,8 +213,8 @@ hipError_t launch_dev2ref_i8(hipStream_t st, const Geom &g, const int8_t *dev, i
 }
 
 hipError_t launch_aos_unpack(hipStream_t st, const Geom &g, const char *aos, AosDesc d, uint32_t fields, double4 *P,
-                             int8_t *T, double *DF, double *RHO) {
-  k_aos_unpack<<<grid_ext(g), kBlock, 0, st>>>(g, aos, d, fields, P, T, DF, RHO);
+                             int8_t *T, double *DF, double *RHO, int32_t n_types, int32_t *max_type) {
+  k_aos_unpack<<<grid_ext(g), kBlock, 0, st>>>(g, aos, d, fields, P, T, DF, RHO, n_types, max_type);
   return hipGetLastError();
 }
 

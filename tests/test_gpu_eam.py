@@ -187,6 +187,34 @@ def test_dropin_aos_hooks_match_oracle():
         assert np.allclose(recs["df"][occ], ref.df[occ], rtol=1e-13)
 
 
+def test_dropin_rejects_type_without_table():
+    """A record whose type has no table fails the hook with MMD_E_INVALID and
+    leaves the host records as they were; the next valid call computes as
+    before (the device never indexed a table with that type)."""
+    c = W.case(**CASES["ragged_7x9x5"])
+    st = c["st"]
+    n = st.x.size
+    recs = np.zeros(n, mmd.REF_AOS_DTYPE)
+    recs["type"] = st.type
+    recs["x"] = np.stack([st.x, st.y, st.z], 1)
+    li = st.local_index()
+    occ = li[st.type[li] >= 0]
+    n_types = int(np.asarray(c["tab"]["rho_r"]).shape[0])
+    with _ctx(c) as ctx:
+        ctx.rho_calc_aos(recs, RC)
+        good = recs["rho"].copy()
+        bad = recs.copy()
+        bad["type"][occ[len(occ) // 2]] = n_types
+        bad["rho"] = -3.0
+        with pytest.raises(mmd.MmdError) as e:
+            ctx.rho_calc_aos(bad, RC)
+        assert e.value.code == mmd.E_INVALID
+        assert np.all(bad["rho"] == -3.0)
+        recs["rho"] = 0.0
+        ctx.rho_calc_aos(recs, RC)
+        assert np.array_equal(recs["rho"][occ], good[occ])
+
+
 def test_ghost_refresh_matches_oracle():
     c = W.case(**CASES["ragged_7x9x5"])
     st = c["st"].copy()
