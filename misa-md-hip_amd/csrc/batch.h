@@ -23,16 +23,16 @@ struct BatchDom {
   const int32_t *off;  // stencil device image (kStencilImage words, shared by all replicas)
   int32_t nk_even, nk_odd;
   double *e_rho, *e_force;  // per pair-pass tile block
-  double *e_ke;             // per row-mapped site block
+  double *e_ke;             // per local-site block
   double *energy;           // {potential, kinetic} after the last step, then at admission
   double L[3];
-  int64_t n_blocks;  // row-mapped site blocks, as the single-domain launch numbers them (site_blocks)
+  int64_t n_blocks;  // local-site blocks, as the single-domain launch numbers them (site_blocks)
   int64_t n_tiles;   // pair-pass tile blocks (tile_blocks)
 };
 
 struct BatchLists {
   const BatchDom *dom;
-  const int2 *site_items;  // (replica, row-mapped site block): velocity Verlet
+  const int2 *site_items;  // (replica, local-site block): velocity Verlet
   const int *n_site_items;
   const int2 *tile_items;  // (replica, tile block): rho + df and force
   const int *n_tile_items;

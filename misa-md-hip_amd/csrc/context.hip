@@ -45,11 +45,8 @@ Geom make_geom(const mmd_domain &d) {
   g.exh = d.ext[0] / 2;
   g.gxh = d.ghost[0] / 2, g.gy = d.ghost[1], g.gz = d.ghost[2];
   g.bxh = d.box[0] / 2, g.by = d.box[1], g.bz = d.box[2];
-  g.chunks = (g.bxh + kWave - 1) / kWave;
-  g.ty = std::min<int32_t>(g.by, 16);
-  g.n_ytiles = (g.by + g.ty - 1) / g.ty;
   g.n_ext = (int64_t)d.ext[0] * d.ext[1] * d.ext[2];
-  g.n_waves = (int64_t)2 * g.n_ytiles * g.ty * g.bz * g.chunks;
+  g.n_local = (int64_t)2 * g.bxh * g.by * g.bz;
   g.ntx = (g.bxh + kTileX - 1) / kTileX;
   g.nty = (g.by + kTileY - 1) / kTileY;
   g.ntz = (g.bz + kTileZ - 1) / kTileZ;

@@ -41,10 +41,8 @@ struct Geom {
   int32_t exh;                  // ext_x / 2 (cells per row)
   int32_t gxh, gy, gz;          // ghost width (x in cells)
   int32_t bxh, by, bz;          // local box (x in cells)
-  int32_t chunks;               // 64-lane chunks per local row
-  int32_t ty, n_ytiles;         // y rows per tile, tiles covering the local box
   int64_t n_ext;                // ext sites
-  int64_t n_waves;              // waves covering the local box (tiles padded to ty rows)
+  int64_t n_local;              // local sites (2 bxh by bz)
   int32_t ntx, nty, ntz;        // pair-pass tiles along x, y, z
   int64_t n_tiles;
 };
@@ -182,14 +180,13 @@ __device__ inline double block_sum(double v, double *lds4) {
   return r;
 }
 
-// One thread per local site.  Waves walk the box in y-tiles of g.ty rows:
-// logical wave w -> (tile, z, y in tile, parity, 64-cell chunk), so the XCD
-// that owns a contiguous range of logical blocks sweeps z through a thin slab
-// and the +-2-plane neighbour window of that slab stays in its L2.
+// One thread per local site, in device order (z, y, parity, x): a wave takes
+// 64 consecutive local sites -- one half-row of a wide box, several short
+// rows of a small one -- so no lane idles on rows shorter than a wave.  Used
+// by the streaming passes (integration, standalone df, host record pack).
 
 struct Site {
   int64_t i;    // device index
-  int s;        // sub-lattice parity (wave-uniform)
   bool active;  // inside the local box
 };
 
@@ -197,22 +194,13 @@ struct Site {
 // serving kernels take it from their work list, so a replica's sites map to
 // blocks identically in both)
 __device__ __forceinline__ bool map_site_lb(const Geom &g, int64_t lb, Site &st) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int64_t w = lb * kWavesPerBlock + (threadIdx.x >> 6);
-  if (w >= g.n_waves) return false;
-  const int64_t chunk = w % g.chunks;
-  const int64_t rl = w / g.chunks;
-  const int s = (int)(rl & 1);
-  const int64_t r = rl >> 1;  // (tile, z, y in tile)
-  const int yin = (int)(r % g.ty);
-  const int64_t tz = r / g.ty;
-  const int zl = (int)(tz % g.bz);
-  const int yl = (int)(tz / g.bz) * g.ty + yin;
-  if (yl >= g.by) return false;
-  const int xh = (int)chunk * kWave + lane;
-  st.active = xh < g.bxh;
-  st.s = __builtin_amdgcn_readfirstlane(s);
-  st.i = dev_index(g, st.s, xh + g.gxh, yl + g.gy, zl + g.gz);
+  const int64_t q = lb * kBlock + threadIdx.x;
+  if (q >= g.n_local) return false;
+  const uint32_t u = (uint32_t)q, bxh = (uint32_t)g.bxh;
+  const uint32_t r = u / bxh, xh = u - r * bxh;  // r = (z * by + y) * 2 + parity
+  const uint32_t yz = r >> 1, zl = yz / (uint32_t)g.by, yl = yz - zl * (uint32_t)g.by;
+  st.active = true;
+  st.i = dev_index(g, (int32_t)(r & 1u), (int32_t)xh + g.gxh, (int32_t)yl + g.gy, (int32_t)zl + g.gz);
   return true;
 }
 

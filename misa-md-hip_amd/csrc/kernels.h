@@ -8,7 +8,7 @@
 
 namespace mmd {
 
-// row-mapped site blocks (streaming passes), pair-pass tile blocks; both multiples of the 8 XCDs
+// local-site blocks (streaming passes), pair-pass tile blocks; both multiples of the 8 XCDs
 int64_t site_blocks(const Geom &g);
 int64_t tile_blocks(const Geom &g);
 

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kBlock) void k_df(Geom g, const int8_t *__restrict_
 // ----------------------------------------------------------------- launch
 
 int64_t site_blocks(const Geom &g) {
-  int64_t nb = (g.n_waves + kWavesPerBlock - 1) / kWavesPerBlock;
+  int64_t nb = (g.n_local + kBlock - 1) / kBlock;
   return (nb + kXcds - 1) / kXcds * kXcds;
 }
 
