@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "eam_body.h"
 #include "host_helpers.h"
 #include "kernels.h"
 
@@ -247,6 +248,59 @@ __device__ __forceinline__ void itl_derivs(const TabF64 &tf, int a, int b, int m
   rho_a = a == b ? rho_b : spl_deriv(tf.rho, (uint32_t)(a * tf.n_r + m), p);
 }
 
+// Pair lookups at squared distance s (segment m, fraction p of r = sqrt(s) in
+// the FP64 tables).  PREC 1 reads the r^2-keyed mixed tables and falls back
+// to the FP64 spline where they do not serve s (a NaN result), as the
+// lattice passes do; the embedding F(rho) stays FP64 in both.
+template <int PREC>
+__device__ __forceinline__ double itl_rho(const TabF64 &tf, const TabMixed &tmx, int t, double s, int m, double p) {
+  if (PREC == 1) {
+    const SegS sg = seg_s(s, tmx);
+    const uint32_t rec = __umul24((uint32_t)t, (uint32_t)tmx.n_s) + sg.m;
+    const double2 b = ld(tmx.rho.vb, rec);
+    const float2 c = ld(tmx.rho.vc, rec);
+    const double v = fma(sg.q, fma(sg.q, (double)fmaf((float)sg.q, c.y, c.x), b.y), b.x);
+    if (!__builtin_isnan(v)) return v;
+  }
+  return spl_value(tf.rho, (uint32_t)(t * tf.n_r + m), p);
+}
+
+template <int PREC>
+__device__ __forceinline__ double itl_phi(const TabF64 &tf, const TabMixed &tmx, int a, int b, double s, int m,
+                                          double p) {
+  if (PREC == 1) {
+    const SegS sg = seg_s(s, tmx);
+    const uint32_t rec = __umul24((uint32_t)(a * tmx.nt + b), (uint32_t)tmx.n_s) + sg.m;
+    const double2 vb = ld(tmx.phi.vb, rec);
+    const float2 vc = ld(tmx.phi.vc, rec);
+    const double v = fma(sg.q, fma(sg.q, (double)fmaf((float)sg.q, vc.y, vc.x), vb.y), vb.x);
+    if (!__builtin_isnan(v)) return v;
+  }
+  return spl_value(tf.phi, (uint32_t)((a * tf.nt + b) * tf.n_r + m), p);
+}
+
+// -(dpsi/dr)/r of pair (a, b) = -(df_a rho_b' + df_b rho_a' + phi_ab') / r,
+// the scalar along the displacement from b to a
+template <int PREC>
+__device__ __forceinline__ double itl_fpair(const TabF64 &tf, const TabMixed &tmx, int a, int b, double s, double r,
+                                            int m, double p, double dfa, double dfb) {
+  if (PREC == 1) {  // records hold -2 d/ds = -(d/dr)/r
+    const SegS sg = seg_s(s, tmx);
+    const uint32_t rp = __umul24((uint32_t)(a * tmx.nt + b), (uint32_t)tmx.n_s) + sg.m;
+    const float qf = (float)sg.q;
+    const double rho_b = mix_deriv_rec(ld(tmx.fb, 2 * rp), sg.q, qf);
+    const double phi_ab = mix_deriv_rec(ld(tmx.fb, 2 * rp + 1), sg.q, qf);
+    const double rho_a =
+        a == b ? rho_b : mix_deriv_rec(ld(tmx.rho.dp, __umul24((uint32_t)a, (uint32_t)tmx.n_s) + sg.m), sg.q, qf);
+    const double f = fma(dfa, rho_b, fma(dfb, rho_a, phi_ab));
+    if (!__builtin_isnan(f)) return f;
+  }
+  double rho_b, phi_ab, rho_a;
+  itl_derivs(tf, a, b, m, p, rho_b, phi_ab, rho_a);
+  const double psip = dfa * rho_b + dfb * rho_a + phi_ab;
+  return -psip / r;
+}
+
 // LDS written by some lanes of a wave and read by others: order the wave's
 // LDS accesses (all lanes of the wave run the same path here)
 __device__ __forceinline__ void wave_lds_sync() {
@@ -260,12 +314,12 @@ constexpr int kItlChunk = kItlPer * kWave;  // entries tested per chunk
 
 // One wave per interstitial.  MODE 0: rho (+ owner-side lattice rho/df update),
 // MODE 1: force (+ owner-side lattice force update).
-template <int MODE, bool WANT_E>
+template <int MODE, bool WANT_E, int PREC>
 __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const double4 *__restrict__ P,
                                                        double *__restrict__ RHO,
                                                        double *__restrict__ DF, double *__restrict__ FX,
                                                        double *__restrict__ FY, double *__restrict__ FZ, TabF64 tf,
-                                                       double *__restrict__ e_part) {
+                                                       TabMixed tmx, double *__restrict__ e_part) {
   __shared__ double lds[kWavesPerBlock];
   // in-cutoff entries of the current chunk, per wave: wide-stencil index | site type << 24, and the site's position
   __shared__ int32_t wlist[kWavesPerBlock][kItlChunk];
@@ -328,14 +382,11 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
         // this pair's term, reused when the owner adds interstitial k to the site
         double v_k = 0.0, fp_k = 0.0;
         if (MODE == 0) {
-          v_k = spl_value(tf.rho, ti * tf.n_r + m, p);
+          v_k = itl_rho<PREC>(tf, tmx, ti, d2, m, p);
           acc += v_k;
         } else {
-          const double dfi = DF[e];
-          double rho_k, phi_ik, rho_i;
-          itl_derivs(tf, ti, tk, m, p, rho_k, phi_ik, rho_i);
-          const double psip = dfi * rho_k + dfk * rho_i + phi_ik;
-          fp_k = -psip / r;  // force on the lattice atom along del = x_i - x_k
+          // force on the lattice atom along del = x_i - x_k
+          fp_k = itl_fpair<PREC>(tf, tmx, ti, tk, d2, r, m, p, DF[e], dfk);
           fxk -= dx * fp_k, fyk -= dy * fp_k, fzk -= dz * fp_k;
         }
         // owner: no lower-numbered neighbour interacts with this lattice site
@@ -366,10 +417,10 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
           }
           if (mm == k) {  // the pair evaluated above: same image, same distance
             if (MODE == 0) {
-              rho_i += tk == ti ? v_k : spl_value(tf.rho, tk * tf.n_r + m, p);
+              rho_i += tk == ti ? v_k : itl_rho<PREC>(tf, tmx, tk, d2, m, p);
             } else {
               gx += dx * fp_k, gy += dy * fp_k, gz += dz * fp_k;
-              if (WANT_E) epair += spl_value(tf.phi, (ti * tf.nt + tk) * tf.n_r + m, p);
+              if (WANT_E) epair += itl_phi<PREC>(tf, tmx, ti, tk, d2, m, p);
             }
             continue;
           }
@@ -381,14 +432,11 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
           double pb;
           const int mb = seg(rb, tf.r0, tf.inv_dr, tf.n_r, pb);
           if (MODE == 0) {
-            rho_i += spl_value(tf.rho, tm * tf.n_r + mb, pb);
+            rho_i += itl_rho<PREC>(tf, tmx, tm, d2b, mb, pb);
           } else {
-            double rho_m, phi_im, rho_i;
-            itl_derivs(tf, ti, tm, mb, pb, rho_m, phi_im, rho_i);
-            const double psip = dfi * rho_m + I.df[mm] * rho_i + phi_im;
-            const double fp = -psip / rb;
+            const double fp = itl_fpair<PREC>(tf, tmx, ti, tm, d2b, rb, mb, pb, dfi, I.df[mm]);
             gx += ex * fp, gy += ey * fp, gz += ez * fp;
-            if (WANT_E) epair += spl_value(tf.phi, (ti * tf.nt + tm) * tf.n_r + mb, pb);
+            if (WANT_E) epair += itl_phi<PREC>(tf, tmx, ti, tm, d2b, mb, pb);
           }
         }
         if (MODE == 0) {
@@ -420,14 +468,11 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
       double p;
       const int m = seg(r, tf.r0, tf.inv_dr, tf.n_r, p);
       if (MODE == 0) {
-        acc += spl_value(tf.rho, tm * tf.n_r + m, p);
+        acc += itl_rho<PREC>(tf, tmx, tm, d2, m, p);
       } else {
-        double rho_m, phi_km, rho_k;
-        itl_derivs(tf, tk, tm, m, p, rho_m, phi_km, rho_k);
-        const double psip = dfk * rho_m + I.df[mm] * rho_k + phi_km;
-        const double fp = -psip / r;
+        const double fp = itl_fpair<PREC>(tf, tmx, tk, tm, d2, r, m, p, dfk, I.df[mm]);
         fxk += dx * fp, fyk += dy * fp, fzk += dz * fp;
-        if (WANT_E) e_loc += 0.5 * spl_value(tf.phi, (tk * tf.nt + tm) * tf.n_r + m, p);
+        if (WANT_E) e_loc += 0.5 * itl_phi<PREC>(tf, tmx, tk, tm, d2, m, p);
       }
     }
     if (MODE == 0) {
@@ -811,6 +856,26 @@ static int itl_prepare(mmd_ctx *c) {
   return MMD_OK;
 }
 
+// the interstitial pass in the context's precision
+template <int MODE>
+static void launch_itl_pass(mmd_ctx *c, const ItlDev &I, dim3 grid, double *ep, bool want_e) {
+  if (c->prec == MMD_PREC_MIXED) {
+    if (want_e)
+      k_itl_pass<MODE, true, 1><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx, c->fy, c->fz,
+                                                                c->tf, c->tm, ep);
+    else
+      k_itl_pass<MODE, false, 1><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx, c->fy, c->fz,
+                                                                 c->tf, c->tm, ep);
+  } else {
+    if (want_e)
+      k_itl_pass<MODE, true, 0><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx, c->fy, c->fz,
+                                                                c->tf, c->tm, ep);
+    else
+      k_itl_pass<MODE, false, 0><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx, c->fy, c->fz,
+                                                                 c->tf, c->tm, ep);
+  }
+}
+
 int itl_rho_df(mmd_ctx *c, double cutoff, bool want_e) {
   if (!c->md_ready) return fail(MMD_E_STATE, "interstitial passes need mmd_md_setup (box, lattice constant, cutoff)");
   if (std::fabs(cutoff - c->md.cutoff) > 1e-12 * cutoff)
@@ -821,12 +886,7 @@ int itl_rho_df(mmd_ctx *c, double cutoff, bool want_e) {
   if (rc || (rc = itl_prepare(c))) return rc;
   ItlDev I = make_dev(c);
   const dim3 grid((c->n_itl + kWavesPerBlock - 1) / kWavesPerBlock);
-  if (want_e)
-    k_itl_pass<0, true><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx,
-                                                          c->fy, c->fz, c->tf, c->e_itl);
-  else
-    k_itl_pass<0, false><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx,
-                                                           c->fy, c->fz, c->tf, c->e_itl);
+  launch_itl_pass<0>(c, I, grid, c->e_itl, want_e);
   ITL_TRY(hipGetLastError());
   return timed_end(c, MMD_K_ITL, ev);
 }
@@ -851,12 +911,7 @@ int itl_force(mmd_ctx *c, double cutoff, bool want_e) {
   const dim3 grid((c->n_itl + kWavesPerBlock - 1) / kWavesPerBlock);
   // the embedding-energy partials of the rho pass live in the first half of e_itl
   double *ep = c->e_itl + c->itl_part_n / 2;
-  if (want_e)
-    k_itl_pass<1, true><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx,
-                                                          c->fy, c->fz, c->tf, ep);
-  else
-    k_itl_pass<1, false><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx,
-                                                           c->fy, c->fz, c->tf, ep);
+  launch_itl_pass<1>(c, I, grid, ep, want_e);
   ITL_TRY(hipGetLastError());
   return timed_end(c, MMD_K_ITL, ev);
 }
