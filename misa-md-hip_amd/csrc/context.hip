@@ -1006,7 +1006,7 @@ static int aos_pass(mmd_ctx *c, void *atoms, const mmd_aos_layout *lay, uint32_t
   int rc = aos_desc(lay, &d);
   if (rc) return rc;
   const size_t bytes = (size_t)c->g.n_ext * lay->stride;
-  if (!c->d_maxtype && (rc = dalloc(&c->d_maxtype, 1))) return rc;
+  if (!c->d_maxtype && (rc = dalloc(&c->d_maxtype, (size_t)kMaxTypeSlots * kMaxTypeStride))) return rc;
   if (bytes > c->aos_bytes) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     dfree(c->aos);
@@ -1015,14 +1015,17 @@ static int aos_pass(mmd_ctx *c, void *atoms, const mmd_aos_layout *lay, uint32_t
     c->aos_bytes = bytes;
   }
   HIP_TRY(hipMemcpyAsync(c->aos, atoms, bytes, hipMemcpyHostToDevice, c->stream));
-  if (in & MMD_F_TYPE) HIP_TRY(launch_fill_i32(c->stream, c->d_maxtype, 1, -1));
+  if (in & MMD_F_TYPE) HIP_TRY(launch_fill_i32(c->stream, c->d_maxtype, (int64_t)kMaxTypeSlots * kMaxTypeStride, -1));
   HIP_TRY(launch_aos_unpack(c->stream, c->g, c->aos, d, in, c->pos, c->type, c->df, c->rho, c->n_types,
                             c->d_maxtype));
   if (in & MMD_F_X) c->itl_stale = true;
   if (in & MMD_F_TYPE) {  // types are checked on the device (a type with no table was stored as a vacancy)
-    int32_t mx = -1;
-    HIP_TRY(hipMemcpyAsync(&mx, c->d_maxtype, sizeof(mx), hipMemcpyDeviceToHost, c->stream));
+    std::vector<int32_t> slots((size_t)kMaxTypeSlots * kMaxTypeStride);
+    HIP_TRY(hipMemcpyAsync(slots.data(), c->d_maxtype, slots.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    int32_t mx = -1;
+    for (int k = 0; k < kMaxTypeSlots; k++) mx = std::max(mx, slots[(size_t)k * kMaxTypeStride]);
     c->max_type = std::min(mx, c->n_types - 1);
     if (mx >= c->n_types) return fail(MMD_E_INVALID, "atom type %d has no table (n_types = %d)", mx, c->n_types);
   }

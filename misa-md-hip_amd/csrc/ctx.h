@@ -58,7 +58,7 @@ struct mmd_ctx {
   // drop-in AoS staging
   char *aos = nullptr;
   size_t aos_bytes = 0;
-  int32_t *d_maxtype = nullptr;  // largest type the last drop-in unpack read
+  int32_t *d_maxtype = nullptr;  // largest type the last drop-in unpack read, in kMaxTypeSlots slots
 
   // timing
   struct Pending {

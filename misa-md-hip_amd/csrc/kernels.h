@@ -38,6 +38,8 @@ struct AosDesc {
   int64_t stride, off_type, off_x, off_v, off_f, off_rho, off_df;
   int32_t type_bytes;
 };
+// largest type unpacked: the maximum over kMaxTypeSlots ints kMaxTypeStride apart
+constexpr int kMaxTypeSlots = 64, kMaxTypeStride = 32;
 hipError_t launch_aos_unpack(hipStream_t st, const Geom &g, const char *aos, AosDesc d, uint32_t fields, double4 *P,
                              int8_t *T, double *DF, double *RHO, int32_t n_types, int32_t *max_type);
 hipError_t launch_aos_pack(hipStream_t st, const Geom &g, char *aos, AosDesc d, uint32_t fields, const int8_t *T,

@@ -48,8 +48,9 @@ __global__ __launch_bounds__(kBlock) void k_dev2ref_pos(Geom g, const double4 *_
 
 // Types are checked here: a type with no table (>= n_types) is stored as a
 // vacancy, so device state never indexes a table out of range, and the
-// largest type read goes to *max_type (one atomic per wave) for the caller to
-// reject the call.
+// largest type read goes to max_type for the caller to reject the call: one
+// atomic per block, spread over kMaxTypeSlots slots a cache line apart (one
+// address for every block serialises the atomics).
 __global__ __launch_bounds__(kBlock) void k_aos_unpack(Geom g, const char *__restrict__ aos, AosDesc d,
                                                          uint32_t fields, double4 *__restrict__ P,
                                                          int8_t *__restrict__ T, double *__restrict__ DF,
@@ -68,9 +69,15 @@ __global__ __launch_bounds__(kBlock) void k_aos_unpack(Geom g, const char *__res
       t = *reinterpret_cast<const int32_t *>(rec + d.off_type);
   }
   if (fields & 0x20u) {
+    __shared__ int32_t wmax[kWavesPerBlock];
     int32_t m = t;
     for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o, kWave));
-    if ((threadIdx.x & 63) == 0 && m >= 0) atomicMax(max_type, m);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      m = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+      if (m >= 0) atomicMax(max_type + (blockIdx.x % kMaxTypeSlots) * kMaxTypeStride, m);
+    }
   }
   if (!live) return;
   double4 p = P[idx];
