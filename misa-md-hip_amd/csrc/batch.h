@@ -36,7 +36,7 @@ struct BatchLists {
   const int *n_site_items;
   const int2 *tile_items;  // (replica, tile block): rho + df and force
   const int *n_tile_items;
-  const int2 *ext_items;   // (replica, 256-site ext block): ghost refresh
+  const int2 *ext_items;   // (replica, block of 256 ghost sites): ghost refresh
   const int *n_ext_items;
   const int *reps;         // replica indices for per-replica reductions
   const int *n_reps;

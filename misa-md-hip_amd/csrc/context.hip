@@ -47,6 +47,7 @@ Geom make_geom(const mmd_domain &d) {
   g.bxh = d.box[0] / 2, g.by = d.box[1], g.bz = d.box[2];
   g.n_ext = (int64_t)d.ext[0] * d.ext[1] * d.ext[2];
   g.n_local = (int64_t)2 * g.bxh * g.by * g.bz;
+  g.n_ghost = g.n_ext - g.n_local;
   g.ntx = (g.bxh + kTileX - 1) / kTileX;
   g.nty = (g.by + kTileY - 1) / kTileY;
   g.ntz = (g.bz + kTileZ - 1) / kTileZ;

@@ -43,6 +43,7 @@ struct Geom {
   int32_t bxh, by, bz;          // local box (x in cells)
   int64_t n_ext;                // ext sites
   int64_t n_local;              // local sites (2 bxh by bz)
+  int64_t n_ghost;              // ghost sites (n_ext - n_local)
   int32_t ntx, nty, ntz;        // pair-pass tiles along x, y, z
   int64_t n_tiles;
 };

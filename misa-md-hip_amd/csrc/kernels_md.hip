@@ -234,7 +234,8 @@ hipError_t launch_aos_pack(hipStream_t st, const Geom &g, char *aos, AosDesc d, 
 
 hipError_t launch_ghost(hipStream_t st, const Geom &g, double Lx, double Ly, double Lz, double4 *P, int8_t *T,
                         double *DF, uint32_t fields) {
-  k_ghost<<<grid_ext(g), kBlock, 0, st>>>(g, Lx, Ly, Lz, P, T, DF, fields);
+  if (g.n_ghost == 0) return hipSuccess;
+  k_ghost<<<dim3((unsigned)((g.n_ghost + kBlock - 1) / kBlock)), kBlock, 0, st>>>(g, Lx, Ly, Lz, P, T, DF, fields);
   return hipGetLastError();
 }
 

@@ -46,13 +46,41 @@ __device__ __forceinline__ int32_t wrap_img(int32_t v, int32_t n, int32_t &img) 
 }
 
 // periodic single-domain ghost refresh of ext site idx (no-op for local sites)
-__device__ __forceinline__ void ghost_site(const Geom &g, uint32_t idx, double Lx, double Ly, double Lz,
+// Device index of the q-th ghost site, or -1 past the last: the ghost
+// z-layers first (both sides), then the ghost y-rows of the local layers,
+// then the ghost x-columns of the local rows (the first two are contiguous
+// ranges of the device layout).
+__device__ __forceinline__ int64_t ghost_index(const Geom &g, uint32_t q) {
+  const uint32_t row = (uint32_t)g.ext_x;  // one y row: both parity rows, 2 exh sites
+  const uint32_t layer = (uint32_t)g.ext_y * row;
+  const uint32_t n_a = 2u * (uint32_t)g.gz * layer;
+  if (q < n_a) return q < (uint32_t)g.gz * layer ? q : q + (uint32_t)g.bz * layer;
+  q -= n_a;
+  const uint32_t per_layer = 2u * (uint32_t)g.gy * row, n_b = (uint32_t)g.bz * per_layer;
+  if (q < n_b) {
+    const uint32_t zl = q / per_layer, r = q - zl * per_layer;
+    const uint32_t off = r < (uint32_t)g.gy * row ? r : r + (uint32_t)g.by * row;
+    return (int64_t)(zl + (uint32_t)g.gz) * layer + off;
+  }
+  q -= n_b;
+  const uint32_t side = (uint32_t)g.gxh, per_row = 4u * side;  // 2 parities x 2 sides
+  if (q >= (uint32_t)g.bz * (uint32_t)g.by * per_row) return -1;
+  const uint32_t rl = q / per_row, r = q - rl * per_row;
+  const uint32_t zl = rl / (uint32_t)g.by, yl = rl - zl * (uint32_t)g.by;
+  const uint32_t s = r / (2u * side), c = r - s * 2u * side;
+  const uint32_t xh = c < side ? c : c + (uint32_t)g.bxh;
+  return dev_index(g, (int32_t)s, (int32_t)xh, (int32_t)yl + g.gy, (int32_t)zl + g.gz);
+}
+
+// periodic refresh of the k-th ghost site from its source in the local box
+__device__ __forceinline__ void ghost_site(const Geom &g, uint32_t k, double Lx, double Ly, double Lz,
                                            double4 *__restrict__ P, int8_t *__restrict__ T, double *__restrict__ DF,
                                            uint32_t fields) {
-  if (idx >= (uint32_t)g.n_ext) return;
+  const int64_t gi = ghost_index(g, k);
+  if (gi < 0) return;
+  const uint32_t idx = (uint32_t)gi;
   const ExtPos p = ext_of(g, idx);
   const int32_t xl = p.xh - g.gxh, yl = p.ye - g.gy, zl = p.ze - g.gz;
-  if (xl >= 0 && xl < g.bxh && yl >= 0 && yl < g.by && zl >= 0 && zl < g.bz) return;
   int32_t ix, iy, iz;
   const int32_t sx = wrap_img(xl, g.bxh, ix), sy = wrap_img(yl, g.by, iy), sz = wrap_img(zl, g.bz, iz);
   const int64_t src = dev_index(g, p.s, sx + g.gxh, sy + g.gy, sz + g.gz);

@@ -76,7 +76,7 @@ int rep_layout(int32_t cells, int32_t ghost, double a, RepLayout &R) {
   R.g = make_geom(R.d);
   R.nb = site_blocks(R.g);
   R.nt = tile_blocks(R.g);
-  R.n_ext_blocks = (R.g.n_ext + kBlock - 1) / kBlock;
+  R.n_ext_blocks = (R.g.n_ghost + kBlock - 1) / kBlock;  // ghost refresh blocks
   const size_t n = (size_t)R.g.n_ext;
   size_t o = 0;
   auto put = [&](size_t bytes) {
