@@ -65,12 +65,13 @@ struct SegS {
 // the conversion stays in range; p >= -s0 / ds > INT_MIN.  A p below 0 maps to
 // segment 0 (p > -1) or, through the unsigned clamp, to segment n_s - 2, like
 // every p past the table: neither segment is served by the mixed tables (their
-// records are NaN-flagged), so no clamping of q is needed.
+// records are NaN-flagged), so q needs no clamping and is taken as the
+// fractional part of p (exactly p - m wherever the segment is served).
 __device__ __forceinline__ SegS seg_s(double s2, const TabMixed &t) {
   const double p = fmin(fma(s2, t.inv_ds, -t.s0 * t.inv_ds), (double)t.n_s - 1.5);
   SegS r;
   r.m = min((uint32_t)(int32_t)p, (uint32_t)(t.n_s - 2));
-  r.q = p - (double)r.m;
+  r.q = __builtin_amdgcn_fract(p);
   return r;
 }
 
