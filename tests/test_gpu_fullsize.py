@@ -1,0 +1,87 @@
+"""The headline configuration at its full size: 128^3 BCC cells (4,194,304
+atoms), 3-type alloy 90/5/5 with the named synthetic tables (A, B, C jittered
++-10%), Gaussian displacement 0.05 A, 800 K.
+
+* FP64: every atom's rho, df and force against the oracle on the same state
+  (the oracle runs on all host cores; same tolerances as the small cases).
+* Size-independent properties: the forces on a periodic box sum to zero (each
+  pair enters twice with opposite displacement; only rounding remains), in
+  both precisions.
+* Mixed precision against FP64 on the same state, to the stated bound (1e-6
+  of the RMS force per atom, and normwise).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import mmd
+import oracle as O
+import workload as W
+
+pytestmark = pytest.mark.gpu
+
+RC = W.RC
+
+
+@pytest.fixture(scope="module")
+def headline():
+    print("full-size case: 128^3 cells, 3 types, synthetic tables, 800 K", flush=True)
+    c = W.case(cells=(128, 128, 128), n_types=3, sigma=0.05, temperature=800.0, seed=2024)
+    c["tab"] = mmd.synth_tables(n_types=3, rc=RC, jitter=0.1, seed=2024)
+    st = c["st"]
+    li = st.local_index()
+    c["occ"] = li[st.type[li] >= 0]
+    return c
+
+
+def _device_pass(c, prec):
+    with mmd.Context(0) as ctx:
+        ctx.set_domain(c["dom"])
+        ctx.set_stencil(c["even"], c["odd"])
+        ctx.set_tables(c["tab"], c["masses"])
+        ctx.set_precision(prec)
+        ctx.upload(c["st"], mmd.F_X | mmd.F_TYPE)
+        ctx.md_setup(c["box"], W.A, 1.0, RC)
+        ctx.rho_df(RC)
+        ctx.ghost_refresh(mmd.F_DF)
+        ctx.force(RC)
+        out = c["st"].copy()
+        ctx.download(out, mmd.F_RHO | mmd.F_DF | mmd.F_F)
+    return out
+
+
+def _forces(st, idx):
+    return np.stack([st.fx[idx], st.fy[idx], st.fz[idx]], 1)
+
+
+def test_fullsize_fp64_every_atom_matches_oracle(headline):
+    c = headline
+    occ = c["occ"]
+    assert len(occ) == 4194304
+    out = _device_pass(c, mmd.PREC_FP64)
+    ref = c["st"].copy()
+    T = O.Tables(c["tab"], c["masses"])
+    n_thr = os.cpu_count() or 1
+    O.rho_df(c["dom"], c["even"], c["odd"], T, RC, ref, nthreads=n_thr)
+    O.ghost_df(c["dom"], ref)
+    O.force(c["dom"], c["even"], c["odd"], T, RC, ref, nthreads=n_thr)
+    assert np.allclose(out.rho[occ], ref.rho[occ], rtol=1e-12, atol=0)
+    assert np.allclose(out.df[occ], ref.df[occ], rtol=1e-12, atol=1e-15)
+    F, Fr = _forces(out, occ), _forces(ref, occ)
+    assert np.abs(F - Fr).max() <= 1e-11 * np.abs(Fr).max()
+    # the periodic box: forces sum to zero up to rounding
+    assert np.abs(F.sum(0)).max() <= 1e-15 * np.abs(F).sum()
+
+
+def test_fullsize_mixed_within_target_and_momentum(headline):
+    c = headline
+    occ = c["occ"]
+    a = _device_pass(c, mmd.PREC_FP64)
+    b = _device_pass(c, mmd.PREC_MIXED)
+    Fa, Fb = _forces(a, occ), _forces(b, occ)
+    rms = np.sqrt((np.linalg.norm(Fa, axis=1) ** 2).mean())
+    assert np.linalg.norm(Fb - Fa) / np.linalg.norm(Fa) <= 1e-6
+    assert np.linalg.norm(Fb - Fa, axis=1).max() / rms <= 1e-6
+    assert np.allclose(b.rho[occ], a.rho[occ], rtol=1e-8, atol=0)
+    assert np.abs(Fb.sum(0)).max() <= 1e-15 * np.abs(Fb).sum()
