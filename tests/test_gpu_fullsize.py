@@ -9,6 +9,10 @@ atoms), 3-type alloy 90/5/5 with the named synthetic tables (A, B, C jittered
   both precisions.
 * Mixed precision against FP64 on the same state, to the stated bound (1e-6
   of the RMS force per atom, and normwise).
+* The large box (256^3 cells, 33.4 M sites, 1 % vacancies, 0.5 % interstitials
+  at random tetrahedral sites, 1800 K) by properties only: the net force over
+  lattice and interstitial atoms is zero in both precisions, and mixed agrees
+  with FP64 within the same bound.
 """
 import os
 
@@ -85,3 +89,41 @@ def test_fullsize_mixed_within_target_and_momentum(headline):
     assert np.linalg.norm(Fb - Fa, axis=1).max() / rms <= 1e-6
     assert np.allclose(b.rho[occ], a.rho[occ], rtol=1e-8, atol=0)
     assert np.abs(Fb.sum(0)).max() <= 1e-15 * np.abs(Fb).sum()
+
+
+def _device_pass_itl(c, prec):
+    with mmd.Context(0) as ctx:
+        ctx.set_domain(c["dom"])
+        ctx.set_stencil(c["even"], c["odd"])
+        ctx.set_tables(c["tab"], c["masses"])
+        ctx.set_precision(prec)
+        ctx.upload(c["st"], mmd.F_X | mmd.F_TYPE)
+        ctx.md_setup(c["box"], W.A, 1.0, RC)
+        ctx.itl_set(c["itl"], c["wide3"])
+        ctx.rho_df(RC)
+        ctx.ghost_refresh(mmd.F_DF)
+        ctx.force(RC)
+        out = c["st"].copy()
+        ctx.download(out, mmd.F_F)
+        itl = ctx.itl_get(c["itl"].n)
+    li = out.local_index()
+    occ = li[out.type[li] >= 0]
+    return np.concatenate([_forces(out, occ), np.stack([itl.fx, itl.fy, itl.fz], 1)])
+
+
+def test_large_box_momentum_and_mixed_within_target():
+    """The large box (256^3 cells, 1 % vacancies, 0.5 % interstitials at
+    random tetrahedral sites, 1800 K): lattice and interstitial forces sum to
+    zero in both precisions, and mixed agrees with FP64 within the bound."""
+    print("full-size case: 256^3 cells, 3 types, 1% vacancies, 0.5% interstitials, 1800 K", flush=True)
+    c = W.case(cells=(256, 256, 256), n_types=3, sigma=0.05, temperature=1800.0, vacancy_frac=0.01,
+               interstitial_frac=0.005, seed=2026)
+    c["tab"] = mmd.synth_tables(n_types=3, rc=RC, jitter=0.1, seed=2026)
+    assert c["itl"].n > 160000
+    Fa = _device_pass_itl(c, mmd.PREC_FP64)
+    Fb = _device_pass_itl(c, mmd.PREC_MIXED)
+    for F in (Fa, Fb):
+        assert np.abs(F.sum(0)).max() <= 1e-15 * np.abs(F).sum()
+    rms = np.sqrt((np.linalg.norm(Fa, axis=1) ** 2).mean())
+    assert np.linalg.norm(Fb - Fa) / np.linalg.norm(Fa) <= 1e-6
+    assert np.linalg.norm(Fb - Fa, axis=1).max() / rms <= 1e-6
