@@ -17,6 +17,8 @@ The same run also measures
     200 steps): atom-steps/s of both, force error of mixed vs FP64 on
     identical states (start and end of the run), the per-step energy
     difference per atom;
+  * the Newton's-third-law half stencil against the full stencil on the
+    headline box, FP64 and mixed (`half_stencil`);
   * the same comparison on the large box (256^3 cells, 33.4 M atoms, 1%
     vacancies, 0.5% interstitials at uniformly random tetrahedral sites,
     1800 K, 100 steps), reported as `large_box`;
@@ -135,8 +137,10 @@ def forces(ctx, w):
     return F
 
 
-def run_gpu(w, prec, steps, warmup, keep_state=False):
+def run_gpu(w, prec, steps, warmup, keep_state=False, half=False):
     ctx = open_ctx(w, prec)
+    if half:
+        ctx.set_half_stencil(True)
     f0 = forces(ctx, w)
     e0 = ctx.energy()
     if warmup:
@@ -235,6 +239,26 @@ def load_traffic():
         with open(p) as f:
             return json.load(f).get("bytes_per_launch")
     return None
+
+
+def half_stencil(w, steps, warmup):
+    """The Newton's-third-law half stencil (mmd_set_half_stencil) against the
+    full stencil on the same box, FP64 and mixed: ms per step and the force
+    pass (forward + backward launches for the half stencil)."""
+    out = {"what": "each lattice pair evaluated once (forward entries), reactions pulled from per-entry pair-force "
+                   "slots by a second pass; full stencil = every pair evaluated from both sides", "steps": steps}
+    n_atoms = n_atoms_of(w)
+    for prec, key in ((mmd.PREC_FP64, "fp64"), (mmd.PREC_MIXED, "mixed")):
+        row = {}
+        for half in (False, True):
+            r = run_gpu(w, prec, steps, warmup, half=half)
+            ms, n = r["kernel_ms"], r["launches"]
+            row["half" if half else "full"] = {
+                "value": round(n_atoms * steps / r["seconds"], 1), "unit": "atom-steps/s",
+                "ms_per_step": round(r["seconds"] / steps * 1e3, 4),
+                "force_ms": round(ms[mmd.K_FORCE] / max(n[mmd.K_FORCE], 1), 4)}
+        out[key] = row
+    return out
 
 
 def mixed_vs_fp64(w, steps, warmup, migrate=False, desc=None):
@@ -399,6 +423,8 @@ def main():
     ap.add_argument("--no-fs", action="store_true", help="skip the stable-potential (Finnis-Sinclair) FP64 line")
     ap.add_argument("--mixed-steps", type=int, default=200)
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3-cell large-box line")
+    ap.add_argument("--no-half", action="store_true", help="skip the half-stencil comparison")
+    ap.add_argument("--half-steps", type=int, default=50)
     ap.add_argument("--large-steps", type=int, default=100)
     ap.add_argument("--serve-replicas", type=int, default=2000)
     ap.add_argument("--serve-load", type=float, default=0.75)
@@ -457,6 +483,10 @@ def main():
            "roofline": roofline,
            "kernel_ms_per_step": {k: round(ms[i] / max(args.steps, 1), 4) for i, k in
                                   enumerate(("rho_df", "force", "integrate", "ghost", "interstitial"))}}
+
+    if not args.no_half:
+        log(f"bench: Newton's-third-law half stencil vs full stencil, FP64 and mixed, {args.half_steps} steps")
+        out["half_stencil"] = half_stencil(w, args.half_steps, args.warmup)
 
     log("bench: drop-in hook path on host records (PCIe included)")
     out["dropin_pcie"] = dropin_hooks(w)

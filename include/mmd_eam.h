@@ -117,7 +117,8 @@ int mmd_set_domain(mmd_ctx *ctx, const mmd_domain *dom);
  * the reference layout for even / odd global x parity (NeiOffset values). */
 int mmd_set_stencil(mmd_ctx *ctx, const int64_t *even, int32_t n_even, const int64_t *odd, int32_t n_odd);
 /* hip_pot_init (src/hip_arch_acc_imp.cpp:146-150): builds the FP64 spline
- * tables (and the mixed-precision r^2-keyed tables) on the device. */
+ * tables (and the mixed-precision r^2-keyed tables) on the device.
+ * MMD_E_RANGE when n_types^2 * n_r reaches 2^24 (pair-table record limit). */
 int mmd_set_tables(mmd_ctx *ctx, const mmd_eam_tables *tab);
 int mmd_set_precision(mmd_ctx *ctx, int32_t mode);
 /* Where MMD_PREC_MIXED evaluates pairs from its r^2-keyed tables, as squared
@@ -179,6 +180,16 @@ typedef struct {
 } mmd_md_params;
 
 int mmd_md_setup(mmd_ctx *ctx, const mmd_md_params *p);
+/* The reference's Newton's-third-law half-stencil mode (USE_NEWTONS_THIRD_LOW,
+ * src/hip_arch_acc_imp.cpp:118-130, src/kernels/kernel_pairs.hpp:28-31,
+ * :55-59), rebuilt deterministically for the periodic step (mmd_md_prime,
+ * mmd_md_run, mmd_energy): each lattice pair's force is evaluated once and
+ * its reaction pulled by the neighbour in a second pass, no atomics.  Needs a
+ * symmetric stencil (MMD_E_INVALID otherwise); takes max-forward-entries x
+ * local sites doubles of device memory; mmd_set_stencil and mmd_set_domain
+ * turn it off.  mmd_eam_force and the drop-in hooks keep the full stencil
+ * (their ghost shell belongs to other sub-domains). */
+int mmd_set_half_stencil(mmd_ctx *ctx, int32_t on);
 /* Periodic single-domain ghost refresh of MMD_F_X (positions + types) and/or
  * MMD_F_DF. */
 int mmd_ghost_refresh(mmd_ctx *ctx, uint32_t fields);

@@ -15,7 +15,8 @@ INC = os.path.join(ROOT, "include")
 OUT = os.path.join(HERE, "mmd", "libmmd_eam.so")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["kernels_eam.hip", "kernels_md.hip", "kernels_batch.hip", "context.hip", "itl.hip", "serve.hip"]
+HIP_SOURCES = ["kernels_eam.hip", "kernels_half.hip", "kernels_md.hip", "kernels_batch.hip", "context.hip", "itl.hip",
+               "serve.hip"]
 CXX_SOURCES = ["host_helpers.cpp", "setfl.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIPFLAGS = [

@@ -114,6 +114,45 @@ std::vector<int32_t> stencil_image(const int32_t *st3[2], const int32_t nk[2], c
   return img;
 }
 
+std::vector<int32_t> half_stencil_image(const int32_t *st3[2], const int32_t nk[2], const Halo &h, int32_t nf[2],
+                                        int32_t nb[2]) {
+  std::vector<std::array<int64_t, 3>> disp[2];
+  std::vector<int32_t> fidx[2];  // forward entry number, -1 for a backward entry
+  for (int p = 0; p < 2; p++) {
+    nf[p] = nb[p] = 0;
+    for (int32_t k = 0; k < nk[p]; k++) {
+      const int32_t *e = &st3[p][3 * (size_t)k];
+      std::array<int64_t, 3> d;
+      ideal_disp2(p, e[0], e[1], e[2], d.data());
+      disp[p].push_back(d);
+      const bool fwd = d[2] > 0 || (d[2] == 0 && (d[1] > 0 || (d[1] == 0 && d[0] > 0)));
+      fidx[p].push_back(fwd ? nf[p]++ : -1);
+    }
+  }
+  std::vector<int32_t> img(kHalfImage, 0);
+  for (int p = 0; p < 2; p++)
+    for (int32_t k = 0; k < nk[p]; k++) {
+      const int32_t *e = &st3[p][3 * (size_t)k];
+      const int32_t lds = stencil_lds_offset(p, e[0], e[1], e[2], h);
+      if (fidx[p][k] >= 0) {
+        img[kHalfFwd + (size_t)p * kMaxStencil + fidx[p][k]] = lds;
+        continue;
+      }
+      // the neighbour (parity q) reaches back through its entry of opposite
+      // displacement, which is forward
+      const int q = (p + e[0]) & 1;
+      const std::array<int64_t, 3> neg = {-disp[p][k][0], -disp[p][k][1], -disp[p][k][2]};
+      const auto it = std::find(disp[q].begin(), disp[q].end(), neg);
+      if (it == disp[q].end()) return {};
+      const int32_t back = fidx[q][(size_t)(it - disp[q].begin())];
+      if (back < 0) return {};
+      img[kHalfBack + (size_t)p * kMaxStencil + nb[p]] = lds;
+      img[kHalfSlot + (size_t)p * kMaxStencil + nb[p]] = back;
+      nb[p]++;
+    }
+  return img;
+}
+
 int check_tile_lds(int device, const Halo &h) {
   const size_t need = tile_lds_bytes(h, true);
   int lim = 0;
@@ -121,7 +160,10 @@ int check_tile_lds(int device, const Halo &h) {
   if (need > (size_t)lim)
     return fail(MMD_E_RANGE, "stencil reach (%d, %d, %d) needs %zu bytes of LDS per tile, the device allows %d",
                 h.rx, h.ry, h.rz, need, lim);
-  if (need > 65536) HIP_TRY(pair_kernels_allow_lds(need));
+  if (need > 65536) {
+    HIP_TRY(pair_kernels_allow_lds(need));
+    HIP_TRY(half_kernels_allow_lds(std::max(need, half_back_lds_bytes(h))));
+  }
   return MMD_OK;
 }
 
@@ -160,6 +202,7 @@ static void free_state(mmd_ctx *c) {
   dfree(c->fx), dfree(c->fy), dfree(c->fz), dfree(c->rho), dfree(c->df), dfree(c->type);
   dfree(c->stage), dfree(c->e_rho), dfree(c->e_force), dfree(c->e_ke), dfree(c->e_itl);
   dfree(c->aos), c->aos_bytes = 0;
+  dfree(c->slots), c->half = false;
 }
 
 void ctx_drop_graphs(mmd_ctx *c) {
@@ -229,12 +272,16 @@ int run_rho_df(mmd_ctx *c, double cutoff, bool want_e) {
   return MMD_OK;
 }
 
-int run_force(mmd_ctx *c, double cutoff, bool want_e) {
+int run_force(mmd_ctx *c, double cutoff, bool want_e, bool periodic) {
   hipEvent_t ev;
   int rc = timed_begin(c, MMD_K_FORCE, &ev);
   if (rc) return rc;
-  HIP_TRY(launch_force(c->stream, c->g, c->halo, c->d_off, c->nk_even, c->nk_odd, c->pos, c->df, c->fx, c->fy, c->fz, c->tf,
-                       c->tm, cutoff * cutoff, c->prec, want_e ? c->e_force : nullptr));
+  if (periodic && c->half)
+    HIP_TRY(launch_force_half(c->stream, c->g, c->halo, c->d_half, c->nf, c->nb, c->pos, c->df, c->fx, c->fy, c->fz,
+                              c->slots, c->tf, c->tm, cutoff * cutoff, c->prec, want_e ? c->e_force : nullptr));
+  else
+    HIP_TRY(launch_force(c->stream, c->g, c->halo, c->d_off, c->nk_even, c->nk_odd, c->pos, c->df, c->fx, c->fy, c->fz,
+                         c->tf, c->tm, cutoff * cutoff, c->prec, want_e ? c->e_force : nullptr));
   rc = timed_end(c, MMD_K_FORCE, ev);
   if (rc) return rc;
   if (c->n_itl > 0) return itl_force(c, cutoff, want_e);
@@ -301,7 +348,7 @@ static int md_step(mmd_ctx *c, bool fused, bool prev_e, bool want_e) {
   if ((rc = run_ghost(c, MMD_F_X))) return rc;
   if ((rc = run_rho_df(c, c->md.cutoff, want_e))) return rc;
   if ((rc = run_ghost(c, MMD_F_DF))) return rc;
-  return run_force(c, c->md.cutoff, want_e);
+  return run_force(c, c->md.cutoff, want_e, true);
 }
 
 // the last step's closing half kick (and its energy record)
@@ -370,7 +417,7 @@ extern "C" int mmd_ctx_destroy(mmd_ctx *c) {
     ctx_drop_graphs(c);
     free_state(c);
     itl_free(c);
-    dfree(c->d_off), dfree(c->d_tab), dfree(c->d_mixed), dfree(c->d_energy), dfree(c->d_counter);
+    dfree(c->d_off), dfree(c->d_half), dfree(c->d_tab), dfree(c->d_mixed), dfree(c->d_energy), dfree(c->d_counter);
     dfree(c->d_maxtype);
     for (auto &p : c->pending) c->event_pool.push_back(p.start), c->event_pool.push_back(p.stop);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
@@ -480,10 +527,22 @@ extern "C" int mmd_set_stencil(mmd_ctx *c, const int64_t *even, int32_t n_even, 
     const Halo h = make_halo(st3, nk);
     if ((rc = check_tile_lds(c->device, h))) return rc;
     const std::vector<int32_t> host = stencil_image(st3, nk, h);
+    int32_t nf[2], nb[2];
+    const std::vector<int32_t> half = half_stencil_image(st3, nk, h, nf, nb);
     c->halo = h;
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (!c->d_off && (rc = dalloc(&c->d_off, kStencilImage))) return rc;
     HIP_TRY(hipMemcpy(c->d_off, host.data(), host.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    // a new stencil turns the half stencil off (its slots follow the stencil)
+    dfree(c->slots), c->half = false;
+    // every entry needs its reverse at the neighbour's parity, as for the
+    // pair energy's half masks (a one-sided pair would be counted differently)
+    c->half_ok = !half.empty() && host[kStencilHalfFlag] == 1;
+    if (c->half_ok) {
+      if (!c->d_half && (rc = dalloc(&c->d_half, kHalfImage))) return rc;
+      HIP_TRY(hipMemcpy(c->d_half, half.data(), half.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+      c->nf[0] = nf[0], c->nf[1] = nf[1], c->nb[0] = nb[0], c->nb[1] = nb[1];
+    }
     c->nk_even = n_even;
     c->nk_odd = n_odd;
     c->has_stencil = true;
@@ -712,6 +771,9 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
         !(t->drho > 0.0) || !(t->r0 >= 0.0) || !t->rho_r || !t->emb || !t->phi_r || !t->mass)
       return fail(MMD_E_INVALID, "mmd_set_tables: bad table descriptor");
     const int nt = t->n_types, nr = t->n_r, nrho = t->n_rho;
+    // pair-table records are addressed with 24-bit multiply-adds
+    if ((int64_t)nt * nt * nr >= ((int64_t)1 << 24))
+      return fail(MMD_E_RANGE, "mmd_set_tables: %d types x %d r knots exceed 2^24 pair-table records", nt, nr);
     for (int a = 0; a < nt; a++) {
       if (!(t->mass[a] > 0.0)) return fail(MMD_E_INVALID, "mmd_set_tables: mass of type %d must be > 0", a);
       for (int b = 0; b < nt; b++)
@@ -726,7 +788,7 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
     const SplHost h_phi = pack_f64(A, t->phi_r, nt * nt, nr, t->dr);
     // mixed: r^2-keyed, n_s knots over [r0^2, rmax^2]; a quarter of the r
     // knots keeps the resampling error near 1e-8 of the forces
-    const int n_s = c->mixed_knots > 0 ? c->mixed_knots : std::max(256, (nr + 3) / 4);
+    const int n_s = std::max(256, (nr + 3) / 4);
     const double rmax = t->r0 + t->dr * (double)(nr - 1);
     const double s0 = t->r0 * t->r0, ds = (rmax * rmax - s0) / (double)(n_s - 1);
     // largest |F'| over the central 80% of the embedding tables
@@ -1110,6 +1172,25 @@ extern "C" int mmd_md_setup(mmd_ctx *c, const mmd_md_params *p) {
   })
 }
 
+extern "C" int mmd_set_half_stencil(mmd_ctx *c, int32_t on) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc) return rc;
+    if (!c->has_stencil) return fail(MMD_E_STATE, "mmd_set_half_stencil: set the stencil first");
+    if (on && !c->half_ok)
+      return fail(MMD_E_INVALID, "mmd_set_half_stencil: the stencil is not symmetric (a pair would be seen once)");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    ctx_drop_graphs(c);
+    dfree(c->slots), c->half = false;
+    if (!on) return MMD_OK;
+    const size_t n = (size_t)std::max(std::max(c->nf[0], c->nf[1]), 1) * (size_t)c->g.n_local;
+    if ((rc = dalloc(&c->slots, n))) return rc;
+    c->half = true;
+    return MMD_OK;
+  })
+}
+
 extern "C" int mmd_ghost_refresh(mmd_ctx *c, uint32_t fields) {
   GUARD({
     clear_error();
@@ -1128,7 +1209,7 @@ extern "C" int mmd_md_prime(mmd_ctx *c) {
     if (rc) return rc;
     if (!c->md_ready) return fail(MMD_E_STATE, "mmd_md_prime: call mmd_md_setup first");
     if ((rc = run_ghost(c, MMD_F_X)) || (rc = run_rho_df(c, c->md.cutoff, false)) || (rc = run_ghost(c, MMD_F_DF)) ||
-        (rc = run_force(c, c->md.cutoff, false)))
+        (rc = run_force(c, c->md.cutoff, false, true)))
       return rc;
     return MMD_OK;
   })
@@ -1196,7 +1277,7 @@ extern "C" int mmd_energy(mmd_ctx *c, double *epot, double *ekin) {
     if ((rc = ensure_energy_cap(c, 1))) return rc;
     HIP_TRY(launch_fill_i32(c->stream, c->d_counter, 1, 0));
     if ((rc = run_ghost(c, MMD_F_X)) || (rc = run_rho_df(c, c->md.cutoff, true)) || (rc = run_ghost(c, MMD_F_DF)) ||
-        (rc = run_force(c, c->md.cutoff, true)) || (rc = run_kick(c, 0.0, false, true)) ||
+        (rc = run_force(c, c->md.cutoff, true, true)) || (rc = run_kick(c, 0.0, false, true)) ||
         (rc = run_finalize(c, c->d_energy, c->d_counter)))
       return rc;
     if ((rc = itl_check(c))) return rc;

@@ -30,6 +30,14 @@ struct mmd_ctx {
   int32_t *d_off = nullptr;
   int32_t nk_even = 0, nk_odd = 0;
   mmd::Halo halo{};
+  // Newton's-third-law half stencil of the periodic step: device image
+  // (kHalfImage words), forward / backward entries per parity, whether the
+  // stencil allows it (symmetric), whether it is on, and the forward entries'
+  // pair-force slots ([max nf][n_local])
+  int32_t *d_half = nullptr;
+  int32_t nf[2] = {0, 0}, nb[2] = {0, 0};
+  bool half_ok = false, half = false;
+  double *slots = nullptr;
 
   // tables
   double *d_tab = nullptr, *d_mixed = nullptr;
@@ -40,7 +48,6 @@ struct mmd_ctx {
   int32_t max_type = -1;  // largest atom type uploaded (checked against n_types before every pass)
   double r_max = 0.0;
   int32_t prec = MMD_PREC_FP64;
-  int32_t mixed_knots = 0;  // 0: as many r^2 knots as r knots
 
   // energy partials (one per site block) and the per-step record
   double *e_rho = nullptr, *e_force = nullptr, *e_ke = nullptr;
@@ -98,6 +105,10 @@ Halo make_halo(const int32_t *st3[2], const int32_t nk[2]);
 int32_t stencil_lds_offset(int p, int64_t dX, int64_t dY, int64_t dZ, const Halo &h);
 // the stencil's device image for the pair passes (layout at kStencilImage)
 std::vector<int32_t> stencil_image(const int32_t *st3[2], const int32_t nk[2], const Halo &h);
+// the half stencil's device image (layout at kHalfImage) and its forward /
+// backward entry counts; empty when the stencil is not symmetric
+std::vector<int32_t> half_stencil_image(const int32_t *st3[2], const int32_t nk[2], const Halo &h, int32_t nf[2],
+                                        int32_t nb[2]);
 // fails with MMD_E_RANGE when the tile of h does not fit the device's LDS
 int check_tile_lds(int device, const Halo &h);
 
@@ -106,7 +117,9 @@ void ctx_drop_graphs(mmd_ctx *c);
 int timed_begin(mmd_ctx *c, int cls, hipEvent_t *ev);
 int timed_end(mmd_ctx *c, int cls, hipEvent_t start);
 int run_rho_df(mmd_ctx *c, double cutoff, bool want_e);
-int run_force(mmd_ctx *c, double cutoff, bool want_e);
+// periodic: the caller is the periodic single-domain step (the ghost shell
+// holds the local sites' images), where the half stencil may be used
+int run_force(mmd_ctx *c, double cutoff, bool want_e, bool periodic = false);
 int run_ghost(mmd_ctx *c, uint32_t fields);
 
 // interstitial passes (itl.hip): added on top of the lattice passes

@@ -29,6 +29,14 @@ constexpr int kMaxStencil = 512;  // entries per parity
 constexpr int kStencilBits = 2 * kMaxStencil;
 constexpr int kStencilHalfFlag = kStencilBits + 2 * (kMaxStencil / 32);
 constexpr int kStencilImage = kStencilHalfFlag + 1;
+// Device image of the Newton's-third-law half stencil (half_stencil_image):
+// [2][kMaxStencil] LDS offsets of the forward entries (ideal displacement
+// lexicographically positive in (z, y, x)), then [2][kMaxStencil] of the
+// backward entries, then [2][kMaxStencil] giving for each backward entry the
+// number of the forward entry, at the neighbour's parity, that reaches back
+// to the centre.  Rows are zero-padded.
+constexpr int kHalfFwd = 0, kHalfBack = 2 * kMaxStencil, kHalfSlot = 4 * kMaxStencil;
+constexpr int kHalfImage = 6 * kMaxStencil;
 constexpr int kMaxTypes = 8;
 constexpr int kXcds = 8;          // MI355X: 8 accelerator complex dies, blocks dispatched round-robin
 

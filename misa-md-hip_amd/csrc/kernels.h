@@ -22,6 +22,16 @@ hipError_t launch_force(hipStream_t st, const Geom &g, const Halo &h, const int3
                         int32_t nk_odd, const double4 *P, const double *DF, double *FX, double *FY, double *FZ,
                         const TabF64 &tf, const TabMixed &tm, double rc2, int prec, double *e_part);
 hipError_t pair_kernels_allow_lds(size_t bytes);
+// Newton's-third-law half stencil (periodic step only): forward pass (own
+// force + pair-force slots SL[nf_max][n_local]) then backward pass (adds the
+// reactions read from the slots); img = half_stencil_image, nf / nb = forward
+// / backward entries per parity
+hipError_t launch_force_half(hipStream_t st, const Geom &g, const Halo &h, const int32_t *img, const int32_t nf[2],
+                             const int32_t nb[2], const double4 *P, const double *DF, double *FX, double *FY,
+                             double *FZ, double *SL, const TabF64 &tf, const TabMixed &tm, double rc2, int prec,
+                             double *e_part);
+size_t half_back_lds_bytes(const Halo &h);
+hipError_t half_kernels_allow_lds(size_t bytes);
 
 // layout conversion between the reference layout (host view) and the device layout;
 // positions: three staged reference-layout arrays <-> (x, y, z, type) records

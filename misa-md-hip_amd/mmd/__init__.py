@@ -155,6 +155,7 @@ def lib():
         "mmd_host_alloc": (C.c_int, [C.POINTER(P), C.c_size_t]),
         "mmd_host_free": (C.c_int, [P]),
         "mmd_md_setup": (C.c_int, [P, C.POINTER(MdParams)]),
+        "mmd_set_half_stencil": (C.c_int, [P, C.c_int32]),
         "mmd_ghost_refresh": (C.c_int, [P, C.c_uint32]),
         "mmd_md_prime": (C.c_int, [P]),
         "mmd_md_run": (C.c_int, [P, C.c_int32, _d, C.c_int32]),
@@ -442,6 +443,10 @@ class Context:
     def md_setup(self, box_len, a, dt, rc, energy_every=0):
         p = MdParams((C.c_double * 3)(*box_len), a, dt, rc, energy_every)
         _check(lib().mmd_md_setup(self.h, C.byref(p)))
+
+    def set_half_stencil(self, on=True):
+        """Newton's-third-law half stencil for the periodic step (md_prime, md_run, energy)."""
+        _check(lib().mmd_set_half_stencil(self.h, 1 if on else 0))
 
     def ghost_refresh(self, fields):
         _check(lib().mmd_ghost_refresh(self.h, fields))

@@ -215,6 +215,22 @@ def test_dropin_rejects_type_without_table():
         assert np.array_equal(recs["rho"][occ], good[occ])
 
 
+def test_set_tables_rejects_pair_tables_beyond_record_limit():
+    """Pair-table records are addressed with 24-bit multiply-adds: 8 types x
+    262,144 r knots (2^24 records) is refused with MMD_E_RANGE before anything
+    is built, and the context keeps working with valid tables."""
+    c = W.case(**CASES["ragged_7x9x5"])
+    nt, nr = 8, 1 << 18
+    big = dict(n_types=nt, n_r=nr, r0=0.5, dr=1e-5, n_rho=5, rho0=0.0, drho=1.0, rho_r=np.zeros((nt, nr)),
+               emb=np.zeros((nt, 5)), phi_r=np.zeros((nt, nt, nr)), rc=RC)
+    with _ctx(c) as ctx:
+        with pytest.raises(mmd.MmdError) as e:
+            ctx.set_tables(big, (55.845,) * nt)
+        assert e.value.code == mmd.E_RANGE
+        ctx.set_tables(c["tab"], c["masses"])
+        ctx.rho_df(RC)
+
+
 def test_ghost_refresh_matches_oracle():
     c = W.case(**CASES["ragged_7x9x5"])
     st = c["st"].copy()
