@@ -338,13 +338,24 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
   }
 }
 
-template <int PREC, bool WANT_E>
+// local site number of local cell (xl, yl, zl), parity s: the device order of
+// the local box alone (the half stencil's slot index)
+__device__ __forceinline__ uint32_t local_q(const Geom &g, int32_t s, int32_t xl, int32_t yl, int32_t zl) {
+  return ((uint32_t)(zl * g.by + yl) * 2u + (uint32_t)s) * (uint32_t)g.bxh + (uint32_t)xl;
+}
+
+// HALF: the forward pass of the Newton's-third-law half stencil (kernels_half.hip):
+// off is the half-stencil image (forward rows first), nk_even / nk_odd the
+// forward entry counts, every entry counts its pair energy once, and the pair
+// force scalar of forward entry k is also stored in SL[k][q] (0 beyond the
+// cutoff, for a vacant neighbour or a vacant centre).
+template <int PREC, bool WANT_E, bool HALF = false>
 __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_t lb, const int32_t *__restrict__ off,
                                             int32_t nk_even, int32_t nk_odd, const double4 *__restrict__ P,
                                             const double *__restrict__ DF, double *__restrict__ FX,
                                             double *__restrict__ FY, double *__restrict__ FZ, const TabF64 &tf,
                                             const TabMixed &tm, double rc2, double *__restrict__ e_part, double2 *lds,
-                                            double *red) {
+                                            double *red, double *__restrict__ SL = nullptr) {
   const TileSite t = map_tile(g, h, lb);
   double2 *lxy = lds, *lzd = lds + h.n;
   int8_t *lt = reinterpret_cast<int8_t *>(lds + 2 * h.n);
@@ -353,12 +364,17 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
   if (t.active) {
     const double2 cxy = lxy[t.c], czd = lzd[t.c];
     const int t0 = lt[t.c];
+    const int nk = t.s ? nk_odd : nk_even;
+    const size_t nl = (size_t)g.n_local;
+    const uint32_t q = HALF ? local_q(g, t.s, t.x0 + t.tx, t.y0 + t.ty, t.z0 + t.tz) : 0u;
+    if (HALF && t0 < 0)
+      for (int k = 0; k < nk; k++) SL[(size_t)k * nl + q] = 0.0;
     if (t0 >= 0) {
+      static_assert(!HALF || kHalfFwd == 0, "forward rows lead the half-stencil image");
       const int32_t *o = off + t.s * kMaxStencil;
       // entries that count the pair energy (each pair once for a symmetric stencil)
       const int32_t *ebits = off + kStencilBits + t.s * (kMaxStencil / 32);
-      const double ew = off[kStencilHalfFlag] ? 1.0 : 0.5;
-      const int nk = t.s ? nk_odd : nk_even;
+      const double ew = HALF || off[kStencilHalfFlag] ? 1.0 : 0.5;
       const double dfi = czd.y;
       double ax = 0.0, ay = 0.0, az = 0.0, ep = 0.0;
       // vacancies sit at the far point (see rho_df_block)
@@ -382,13 +398,20 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           in[u] = d2[u] < rc2 && (!tail || k + u < nk);
           any |= in[u];
         }
-        if (__ballot(any) == 0) return;
+        if (__ballot(any) == 0) {
+          if (HALF) {
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+              if (!tail || k + u < nk) SL[(size_t)(k + u) * nl + q] = 0.0;
+          }
+          return;
+        }
         double fp[kUnroll], phi[kUnroll];
         bool ce[kUnroll];  // wave-uniform: this entry counts the pair energy
-        const uint32_t hw = WANT_E ? (uint32_t)ebits[k >> 5] : 0u;
+        const uint32_t hw = WANT_E && !HALF ? (uint32_t)ebits[k >> 5] : 0u;
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
-          ce[u] = WANT_E && ((hw >> ((k & 31) + u)) & 1u);
+          ce[u] = WANT_E && (HALF || ((hw >> ((k & 31) + u)) & 1u));
           phi[u] = 0.0;
         }
         if constexpr (PREC == 1) {
@@ -416,6 +439,7 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           ay = fma(dy[u], f, ay);
           az = fma(dz[u], f, az);
           if (WANT_E) ep += in[u] ? phi[u] : 0.0;
+          if (HALF && (!tail || k + u < nk)) SL[(size_t)(k + u) * nl + q] = f;
         }
       };
       const int nk4 = nk & ~(kUnroll - 1);

@@ -29,14 +29,9 @@ namespace mmd {
 constexpr int kBackUnroll = 8;  // backward entries whose slot loads are in flight together
 static_assert(kBackUnroll <= 8, "stencil padding too short for the unroll");
 
-// local site number of local cell (xl, yl, zl), parity s: the device order of
-// the local box alone
-__device__ __forceinline__ uint32_t local_q(const Geom &g, int32_t s, int32_t xl, int32_t yl, int32_t zl) {
-  return ((uint32_t)(zl * g.by + yl) * 2u + (uint32_t)s) * (uint32_t)g.bxh + (uint32_t)xl;
-}
-
 __device__ __forceinline__ int32_t wrap1(int32_t v, int32_t n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
+// forward pass: force_block's HALF mode (forward entries, slot stores)
 template <int PREC, bool WANT_E>
 __global__ __launch_bounds__(kBlock) void k_force_fwd(Geom g, Halo h, const int32_t *__restrict__ img, int32_t nf_even,
                                                         int32_t nf_odd, const double4 *__restrict__ P,
@@ -46,98 +41,8 @@ __global__ __launch_bounds__(kBlock) void k_force_fwd(Geom g, Halo h, const int3
                                                         double *__restrict__ e_part) {
   extern __shared__ double2 lds_tile[];
   __shared__ double red[kWavesPerBlock];
-  const int64_t lb = xcd_block();
-  const TileSite t = map_tile(g, h, lb);
-  double2 *lxy = lds_tile, *lzd = lds_tile + h.n;
-  int8_t *lt = reinterpret_cast<int8_t *>(lds_tile + 2 * h.n);
-  stage_tile<true>(g, h, t, lb, P, DF, lxy, lzd, lt);
-  double e_loc = 0.0;
-  if (t.active) {
-    const uint32_t q = local_q(g, t.s, t.x0 + t.tx, t.y0 + t.ty, t.z0 + t.tz);
-    const int nk = t.s ? nf_odd : nf_even;
-    const int t0 = lt[t.c];
-    const size_t nl = (size_t)g.n_local;
-    if (t0 < 0) {
-      for (int k = 0; k < nk; k++) SL[(size_t)k * nl + q] = 0.0;
-    } else {
-      const int32_t *o = img + kHalfFwd + t.s * kMaxStencil;
-      const double2 cxy = lxy[t.c], czd = lzd[t.c];
-      const double dfi = czd.y;
-      double ax = 0.0, ay = 0.0, az = 0.0, ep = 0.0;
-      // vacancies sit at the far point (see rho_df_block)
-      const uint8_t *ltu = reinterpret_cast<const uint8_t *>(lt);
-      const uint32_t tmax = (uint32_t)(tf.nt - 1), ti = (uint32_t)t0;
-      const uint32_t ra_i = ti * (uint32_t)tm.n_s, row_i = ra_i * (uint32_t)tm.nt;
-      auto trip = [&](int k, bool tail) {
-        uint32_t tq[kUnroll];
-        double dx[kUnroll], dy[kUnroll], dz[kUnroll], d2[kUnroll], dfj[kUnroll];
-        bool in[kUnroll], any = false;
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-          const int32_t j = t.c + o[k + u];
-          const double2 pxy = lxy[j], pzd = lzd[j];
-          tq[u] = min((uint32_t)ltu[j], tmax);
-          dx[u] = cxy.x - pxy.x;
-          dy[u] = cxy.y - pxy.y;
-          dz[u] = czd.x - pzd.x;
-          dfj[u] = pzd.y;
-          d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-          in[u] = d2[u] < rc2 && (!tail || k + u < nk);
-          any |= in[u];
-        }
-        double fp[kUnroll], phi[kUnroll];
-        if (__ballot(any) == 0) {
-#pragma unroll
-          for (int u = 0; u < kUnroll; u++)
-            if (!tail || k + u < nk) SL[(size_t)(k + u) * nl + q] = 0.0;
-          return;
-        }
-        bool ce[kUnroll];  // every forward entry counts its pair's energy once
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-          ce[u] = WANT_E;
-          phi[u] = 0.0;
-        }
-        if constexpr (PREC == 1) {
-          trip_force_mixed<WANT_E, kUnroll>(tm, ti, row_i, ra_i, tq, d2, dfi, dfj, ce, fp, phi);
-          uint64_t redo = 0;
-#pragma unroll
-          for (int u = 0; u < kUnroll; u++) redo |= __ballot(in[u] && __builtin_isnan(fp[u]));
-          if (redo != 0) {
-#pragma unroll
-            for (int u = 0; u < kUnroll; u++)
-              if (in[u] && __builtin_isnan(fp[u]))
-                fp[u] = fpair_f64<WANT_E>(tf, t0, (int)tq[u], d2[u], dfi, dfj[u], ce[u], phi[u]);
-          }
-        } else {
-#pragma unroll
-          for (int u = 0; u < kUnroll; u++)
-            fp[u] = fpair_f64<WANT_E>(tf, t0, (int)tq[u], d2[u], dfi, dfj[u], ce[u], phi[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-          const double f = in[u] ? fp[u] : 0.0;
-          ax = fma(dx[u], f, ax);
-          ay = fma(dy[u], f, ay);
-          az = fma(dz[u], f, az);
-          if (WANT_E) ep += in[u] ? phi[u] : 0.0;
-          if (!tail || k + u < nk) SL[(size_t)(k + u) * nl + q] = f;
-        }
-      };
-      const int nk4 = nk & ~(kUnroll - 1);
-      for (int k = 0; k < nk4; k += kUnroll) trip(k, false);
-      if (nk4 < nk) trip(nk4, true);
-      const uint32_t i = (uint32_t)t.i;
-      FX[i] = ax;
-      FY[i] = ay;
-      FZ[i] = az;
-      if (WANT_E) e_loc = ep;
-    }
-  }
-  if (WANT_E) {
-    const double b = block_sum(e_loc, red);
-    if (threadIdx.x == 0) e_part[lb] = b;
-  }
+  force_block<PREC, WANT_E, true>(g, h, xcd_block(), img + kHalfFwd, nf_even, nf_odd, P, DF, FX, FY, FZ, tf, tm, rc2,
+                                  e_part, lds_tile, red, SL);
 }
 
 // LDS tile of the backward pass (n records): (x, y) [n], z [n], the local
