@@ -529,24 +529,26 @@ extern "C" int mmd_set_stencil(mmd_ctx *c, const int64_t *even, int32_t n_even, 
     const std::vector<int32_t> host = stencil_image(st3, nk, h);
     int32_t nf[2], nb[2];
     const std::vector<int32_t> half = half_stencil_image(st3, nk, h, nf, nb);
-    c->halo = h;
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (!c->d_off && (rc = dalloc(&c->d_off, kStencilImage))) return rc;
-    HIP_TRY(hipMemcpy(c->d_off, host.data(), host.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!c->d_half && (rc = dalloc(&c->d_half, kHalfImage))) return rc;
+    // from here on the context holds the new stencil, or none if a copy fails;
     // a new stencil turns the half stencil off (its slots follow the stencil)
+    c->has_stencil = false;
+    ctx_drop_graphs(c);
     dfree(c->slots), c->half = false;
+    HIP_TRY(hipMemcpy(c->d_off, host.data(), host.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     // every entry needs its reverse at the neighbour's parity, as for the
     // pair energy's half masks (a one-sided pair would be counted differently)
     c->half_ok = !half.empty() && host[kStencilHalfFlag] == 1;
     if (c->half_ok) {
-      if (!c->d_half && (rc = dalloc(&c->d_half, kHalfImage))) return rc;
       HIP_TRY(hipMemcpy(c->d_half, half.data(), half.size() * sizeof(int32_t), hipMemcpyHostToDevice));
       c->nf[0] = nf[0], c->nf[1] = nf[1], c->nb[0] = nb[0], c->nb[1] = nb[1];
     }
+    c->halo = h;
     c->nk_even = n_even;
     c->nk_odd = n_odd;
     c->has_stencil = true;
-    ctx_drop_graphs(c);
     return MMD_OK;
   })
 }
