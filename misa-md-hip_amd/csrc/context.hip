@@ -361,11 +361,12 @@ static int md_close(mmd_ctx *c, bool want_e) {
 
 static int ensure_energy_cap(mmd_ctx *c, int64_t n) {
   if (n <= c->energy_cap) return MMD_OK;
+  ctx_drop_graphs(c);  // graphs hold the old pointer
   dfree(c->d_energy);
+  c->energy_cap = 0;
   int rc = dalloc(&c->d_energy, (size_t)(2 * n));
   if (rc) return rc;
   c->energy_cap = n;
-  ctx_drop_graphs(c);  // graphs hold the old pointer
   return MMD_OK;
 }
 
@@ -834,6 +835,8 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
 
     HIP_TRY(hipStreamSynchronize(c->stream));
     ctx_drop_graphs(c);
+    // the old tables go first: until the new ones are in place the context has none
+    c->has_tables = false;
     dfree(c->d_tab), dfree(c->d_mixed);
     char *dev = nullptr;
     if ((rc = dalloc(&dev, A.buf.size()))) return rc;

@@ -799,7 +799,7 @@ void itl_free(mmd_ctx *c) {
     c->itl_state = nullptr;
   }
   c->n_itl = c->itl_cap = c->n_wide = 0;
-  c->itl_part_n = 0;
+  c->itl_part_n = c->itl_kick_n = 0;
 }
 
 static ItlDev make_dev(mmd_ctx *c) {
@@ -831,11 +831,11 @@ static int ensure_bins(mmd_ctx *c) {
   const int nx = std::max(1, (int)(c->md.box_len[0] / bs)), ny = std::max(1, (int)(c->md.box_len[1] / bs)),
             nz = std::max(1, (int)(c->md.box_len[2] / bs));
   if (s.bins && nx == s.nbx && ny == s.nby && nz == s.nbz) return MMD_OK;
+  ctx_drop_graphs(c);
   dfree_t(s.bins);
   const size_t n = (size_t)nx * ny * nz;
   ITL_TRY(hipMalloc((void **)&s.bins, n * (1 + kBinCap) * sizeof(int32_t)));
   s.nbx = nx, s.nby = ny, s.nbz = nz;
-  ctx_drop_graphs(c);
   return MMD_OK;
 }
 
