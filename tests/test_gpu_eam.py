@@ -46,6 +46,9 @@ CASES = {
                             seed=2),
     "ragged_7x9x5": dict(cells=(7, 9, 5), n_types=3, sigma=0.05, temperature=1200.0, vacancy_frac=0.02, seed=3),
     "wide_row_70x3x4": dict(cells=(70, 3, 4), n_types=2, sigma=0.05, temperature=800.0, seed=4, ghost_cells=2),
+    # rho_t differs per type, so rho by the neighbour's type is tested
+    "bcc12_3type_distinct_rho": dict(cells=(12, 12, 12), n_types=3, sigma=0.05, temperature=800.0, vacancy_frac=0.01,
+                                     seed=6, distinct_rho=True),
 }
 
 
@@ -337,7 +340,7 @@ def _fallback_fraction(c, ranges):
     return 1.0 - inside.mean()
 
 
-@pytest.mark.parametrize("name", ["bcc12_3type_vac", "ragged_7x9x5"])
+@pytest.mark.parametrize("name", ["bcc12_3type_vac", "ragged_7x9x5", "bcc12_3type_distinct_rho"])
 def test_mixed_precision_within_target(name):
     c = W.case(**CASES[name])
     res = {}

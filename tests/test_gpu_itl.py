@@ -24,6 +24,9 @@ ITL_CASES = {
                         interstitial_frac=0.006, seed=22),
     "blobs_12": dict(cells=(12, 12, 12), n_types=3, sigma=0.05, temperature=1200.0, vacancy_frac=0.01,
                      interstitial_frac=0.004, n_blobs=2, blob_sigma=6.0, seed=23),
+    # rho_t differs per type: each side of a lattice-interstitial pair takes the other's rho
+    "bcc10_3type_distinct_rho": dict(cells=(10, 9, 8), n_types=3, sigma=0.05, temperature=1200.0,
+                                     vacancy_frac=0.005, interstitial_frac=0.006, seed=24, distinct_rho=True),
 }
 
 

@@ -174,7 +174,8 @@ def test_finnis_sinclair_fe_cohesive_energy():
 
 @pytest.mark.parametrize("n_types,vac", [(1, 0.0), (3, 0.05)])
 def test_oracle_matches_bruteforce(n_types, vac):
-    c = W.case(cells=(4, 4, 4), n_types=n_types, sigma=0.05, temperature=800.0, vacancy_frac=vac, seed=11)
+    c = W.case(cells=(4, 4, 4), n_types=n_types, sigma=0.05, temperature=800.0, vacancy_frac=vac, seed=11,
+               distinct_rho=True)
     _, E = _run_oracle(c, want_e=True)
     st = c["st"]
     li = st.local_index()
@@ -188,6 +189,25 @@ def test_oracle_matches_bruteforce(n_types, vac):
     assert abs(E - Ebf) < 1e-11 * abs(Ebf)
     if vac:
         assert (st.type[li] < 0).sum() > 0
+
+
+@pytest.mark.parametrize("slip", ["rho_centre_type", "swap_df", "sign", "drop_phi", "energy_double"])
+def test_bruteforce_comparison_catches_planted_slips(slip):
+    """The brute-force comparison above has teeth: each plausible mistake,
+    planted in the numpy restatement, breaks the agreement with the oracle
+    (3-type alloy with vacancies, so type and df slips change the result)."""
+    c = W.case(cells=(4, 4, 4), n_types=3, sigma=0.05, temperature=800.0, vacancy_frac=0.05, seed=11,
+               distinct_rho=True)
+    _, E = _run_oracle(c, want_e=True)
+    st = c["st"]
+    li = st.local_index()
+    occ = li[st.type[li] >= 0]
+    pos = np.stack([st.x[occ], st.y[occ], st.z[occ]], 1)
+    rho, df, F, Ebf = W.brute_force(pos, st.type[occ], c["box"], W.NpTables(c["tab"]), RC, slip=slip)
+    Fo = np.stack([st.fx[occ], st.fy[occ], st.fz[occ]], 1)
+    agree = (np.allclose(st.rho[occ], rho, rtol=1e-12, atol=0) and np.abs(Fo - F).max() < 1e-11 * np.abs(F).max()
+             and abs(E - Ebf) < 1e-11 * abs(Ebf))
+    assert not agree
 
 
 def test_force_is_minus_gradient_and_momentum_conserved():
@@ -318,7 +338,7 @@ def test_anchor_is_nearest_site():
 @pytest.mark.parametrize("n_types", [1, 3])
 def test_oracle_interstitials_match_bruteforce(n_types):
     c = W.case(cells=(6, 6, 6), n_types=n_types, sigma=0.04, temperature=600.0, vacancy_frac=0.01,
-               interstitial_frac=0.01, seed=21)
+               interstitial_frac=0.01, seed=21, distinct_rho=True)
     st, itl, dom, box = c["st"], c["itl"], c["dom"], c["box"]
     assert itl.n >= 3
     T = O.Tables(c["tab"], c["masses"])

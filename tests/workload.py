@@ -12,8 +12,19 @@ RC = 5.6
 SKIN = 0.3
 
 
+def _distinct_rho(tab, step=0.15):
+    """The tables with rho_t scaled by 1 + step * t: the named synthetic and the
+    Finnis-Sinclair forms share one rho(r) across types, which would leave the
+    rule "rho by the neighbour's type" (src/kernels/kernel_pairs.hpp:26)
+    untested."""
+    out = dict(tab)
+    out["rho_r"] = np.asarray(tab["rho_r"], float) * (1.0 + step * np.arange(tab["n_types"]))[:, None]
+    return out
+
+
 def case(cells=(6, 6, 6), n_types=1, sigma=0.03, temperature=600.0, vacancy_frac=0.0, interstitial_frac=0.0,
-         seed=7, ghost_cells=3, jitter=None, n_r=5000, n_rho=5000, n_blobs=0, blob_sigma=8.0, skin=SKIN):
+         seed=7, ghost_cells=3, jitter=None, n_r=5000, n_rho=5000, n_blobs=0, blob_sigma=8.0, skin=SKIN,
+         distinct_rho=False):
     spec = mmd.lattice_spec(cells, ghost_cells=ghost_cells, a=A, n_types=n_types, sigma=sigma,
                             temperature=temperature, vacancy_frac=vacancy_frac, interstitial_frac=interstitial_frac,
                             n_blobs=n_blobs, blob_sigma=blob_sigma, seed=seed)
@@ -21,6 +32,8 @@ def case(cells=(6, 6, 6), n_types=1, sigma=0.03, temperature=600.0, vacancy_frac
     dom, box, st, itl = mmd.lattice_init(spec, itl_cap=cap)
     tab = mmd.fs_tables(n_types=n_types, rc=RC, jitter=(0.1 if n_types > 1 else 0.0) if jitter is None else jitter,
                         seed=seed, n_r=n_r, n_rho=n_rho)
+    if distinct_rho:
+        tab = _distinct_rho(tab)
     st3 = mmd.stencil_build(A, RC + skin, 0)
     even = mmd.stencil_linear(dom, st3)
     odd = mmd.stencil_linear(dom, -st3)
@@ -84,8 +97,13 @@ class NpTables:
         self.phi = [[np_spline(tab["phi_r"][a][b], tab["dr"]) for b in range(nt)] for a in range(nt)]
 
 
-def brute_force(pos, types, L, nt_tab, rc):
-    """O(N^2) minimum-image EAM: rho, df, forces, total potential energy."""
+def brute_force(pos, types, L, nt_tab, rc, slip=None):
+    """O(N^2) minimum-image EAM: rho, df, forces, total potential energy.
+
+    slip plants one plausible mistake (to show the comparison with the oracle
+    catches it): "rho_centre_type" (rho by the centre's type), "swap_df"
+    (df_i and df_j transposed), "sign" (fpair sign flipped), "drop_phi" (pair
+    term left out of the force), "energy_double" (pair energy counted twice)."""
     T = nt_tab
     pos = np.asarray(pos, float)
     types = np.asarray(types, int)
@@ -97,7 +115,7 @@ def brute_force(pos, types, L, nt_tab, rc):
     r = np.sqrt(np.where(mask, r2, 1.0))
     rho = np.zeros(N)
     for t in range(T.nt):
-        sel = mask & (types[None, :] == t)
+        sel = mask & ((types[:, None] if slip == "rho_centre_type" else types[None, :]) == t)
         rho += np.where(sel, np_value(T.rho[t], T.r0, T.idr, r), 0.0).sum(1)
     df = np.zeros(N)
     Femb = np.zeros(N)
@@ -118,10 +136,11 @@ def brute_force(pos, types, L, nt_tab, rc):
             phiv = np.where(sel, np_value(T.phi[a][b], T.r0, T.idr, r), phiv)
     rhojp = np.take_along_axis(rhop.transpose(1, 2, 0), tj[:, :, None], 2)[:, :, 0]
     rhoip = np.take_along_axis(rhop.transpose(1, 2, 0), ti[:, :, None], 2)[:, :, 0]
-    psip = df[:, None] * rhojp + df[None, :] * rhoip + phip
-    fpair = np.where(mask, -psip / r, 0.0)
+    dfi, dfj = (df[None, :], df[:, None]) if slip == "swap_df" else (df[:, None], df[None, :])
+    psip = dfi * rhojp + dfj * rhoip + (0.0 if slip == "drop_phi" else phip)
+    fpair = np.where(mask, (psip if slip == "sign" else -psip) / r, 0.0)
     F = (d * fpair[:, :, None]).sum(1)
-    E = Femb.sum() + 0.5 * np.where(mask, phiv, 0.0).sum()
+    E = Femb.sum() + (1.0 if slip == "energy_double" else 0.5) * np.where(mask, phiv, 0.0).sum()
     return rho, df, F, E
 
 
