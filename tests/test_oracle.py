@@ -191,7 +191,8 @@ def test_oracle_matches_bruteforce(n_types, vac):
         assert (st.type[li] < 0).sum() > 0
 
 
-@pytest.mark.parametrize("slip", ["rho_centre_type", "swap_df", "sign", "drop_phi", "energy_double"])
+@pytest.mark.parametrize("slip", ["rho_centre_type", "swap_df", "swap_rho_types", "sign", "drop_phi",
+                                  "energy_double"])
 def test_bruteforce_comparison_catches_planted_slips(slip):
     """The brute-force comparison above has teeth: each plausible mistake,
     planted in the numpy restatement, breaks the agreement with the oracle

@@ -102,7 +102,8 @@ def brute_force(pos, types, L, nt_tab, rc, slip=None):
 
     slip plants one plausible mistake (to show the comparison with the oracle
     catches it): "rho_centre_type" (rho by the centre's type), "swap_df"
-    (df_i and df_j transposed), "sign" (fpair sign flipped), "drop_phi" (pair
+    (df_i and df_j transposed), "swap_rho_types" (rho'_ti and rho'_tj
+    transposed in the force), "sign" (fpair sign flipped), "drop_phi" (pair
     term left out of the force), "energy_double" (pair energy counted twice)."""
     T = nt_tab
     pos = np.asarray(pos, float)
@@ -137,6 +138,8 @@ def brute_force(pos, types, L, nt_tab, rc, slip=None):
     rhojp = np.take_along_axis(rhop.transpose(1, 2, 0), tj[:, :, None], 2)[:, :, 0]
     rhoip = np.take_along_axis(rhop.transpose(1, 2, 0), ti[:, :, None], 2)[:, :, 0]
     dfi, dfj = (df[None, :], df[:, None]) if slip == "swap_df" else (df[:, None], df[None, :])
+    if slip == "swap_rho_types":
+        rhojp, rhoip = rhoip, rhojp
     psip = dfi * rhojp + dfj * rhoip + (0.0 if slip == "drop_phi" else phip)
     fpair = np.where(mask, (psip if slip == "sign" else -psip) / r, 0.0)
     F = (d * fpair[:, :, None]).sum(1)
