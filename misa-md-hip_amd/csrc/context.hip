@@ -449,8 +449,11 @@ extern "C" int mmd_set_domain(mmd_ctx *c, const mmd_domain *d) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     ctx_drop_graphs(c);
     free_state(c);
+    // the MD parameters (box lengths, cutoff) belong to the old domain: the
+    // step needs mmd_md_setup again, whether or not the new state allocates
     c->has_domain = false;
     c->has_stencil = false;
+    c->md_ready = false;
     c->n_itl = 0;
     c->g = make_geom(*d);
     const Geom &g = c->g;
@@ -1202,6 +1205,7 @@ extern "C" int mmd_ghost_refresh(mmd_ctx *c, uint32_t fields) {
     int rc = ctx_bind(c);
     if (rc) return rc;
     if (!c->md_ready) return fail(MMD_E_STATE, "mmd_ghost_refresh: call mmd_md_setup first");
+    if ((rc = check_ready(c, "mmd_ghost_refresh", false))) return rc;
     if (fields & ~(MMD_F_X | MMD_F_DF)) return fail(MMD_E_INVALID, "mmd_ghost_refresh: only MMD_F_X and MMD_F_DF");
     return run_ghost(c, fields);
   })
@@ -1213,6 +1217,7 @@ extern "C" int mmd_md_prime(mmd_ctx *c) {
     int rc = ctx_bind(c);
     if (rc) return rc;
     if (!c->md_ready) return fail(MMD_E_STATE, "mmd_md_prime: call mmd_md_setup first");
+    if ((rc = check_ready(c, "mmd_md_prime")) || (rc = check_cutoff(c, c->md.cutoff, "mmd_md_prime"))) return rc;
     if ((rc = run_ghost(c, MMD_F_X)) || (rc = run_rho_df(c, c->md.cutoff, false)) || (rc = run_ghost(c, MMD_F_DF)) ||
         (rc = run_force(c, c->md.cutoff, false, true)))
       return rc;
@@ -1226,6 +1231,7 @@ extern "C" int mmd_md_run(mmd_ctx *c, int32_t nsteps, double *energies, int32_t 
     int rc = ctx_bind(c);
     if (rc) return rc;
     if (!c->md_ready) return fail(MMD_E_STATE, "mmd_md_run: call mmd_md_setup first");
+    if ((rc = check_ready(c, "mmd_md_run")) || (rc = check_cutoff(c, c->md.cutoff, "mmd_md_run"))) return rc;
     if (nsteps < 0) return fail(MMD_E_INVALID, "mmd_md_run: nsteps < 0");
     const int every = c->md.energy_every;
     const int64_t n_rec = every > 0 ? nsteps / every : 0;
@@ -1279,6 +1285,7 @@ extern "C" int mmd_energy(mmd_ctx *c, double *epot, double *ekin) {
     int rc = ctx_bind(c);
     if (rc) return rc;
     if (!c->md_ready) return fail(MMD_E_STATE, "mmd_energy: call mmd_md_setup first");
+    if ((rc = check_ready(c, "mmd_energy")) || (rc = check_cutoff(c, c->md.cutoff, "mmd_energy"))) return rc;
     if ((rc = ensure_energy_cap(c, 1))) return rc;
     HIP_TRY(launch_fill_i32(c->stream, c->d_counter, 1, 0));
     if ((rc = run_ghost(c, MMD_F_X)) || (rc = run_rho_df(c, c->md.cutoff, true)) || (rc = run_ghost(c, MMD_F_DF)) ||

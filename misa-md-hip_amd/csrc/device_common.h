@@ -71,6 +71,9 @@ __host__ __device__ inline size_t tile_lds_bytes(const Halo &h, bool with_df) {
 
 // Loads through a uniform base pointer with a 32-bit element index, so the
 // address is an SGPR base + 32-bit VGPR offset (no 64-bit VALU arithmetic).
+// The byte offset is 32-bit: only for arrays below 4 GiB (the spline tables,
+// at most 2^24 records of 48 bytes: mmd_set_tables checks the record count).
+// Lattice arrays (up to 2^31 sites) are indexed with 64-bit offsets.
 template <typename T>
 __device__ __forceinline__ T ld(const T *__restrict__ base, uint32_t idx) {
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + idx * (uint32_t)sizeof(T));

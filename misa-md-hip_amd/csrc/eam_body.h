@@ -250,8 +250,8 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const Halo &h, const T
       double4 q = make_double4(0.0, 0.0, 0.0, -1.0);
       double f = 0.0;
       if (src >= 0) {
-        q = ld(P, (uint32_t)src);
-        if (WITH_DF) f = ld(DF, (uint32_t)src);
+        q = P[src];
+        if (WITH_DF) f = DF[src];
       }
       if (q.w < 0.0) q.x = q.y = q.z = kFarPos, f = 0.0;
       lxy[idx] = make_double2(q.x, q.y);

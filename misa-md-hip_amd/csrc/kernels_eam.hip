@@ -35,9 +35,9 @@ __global__ __launch_bounds__(kBlock) void k_df(Geom g, const int8_t *__restrict_
   Site st;
   if (!map_site(g, st) || !st.active) return;
   const uint32_t i = (uint32_t)st.i;
-  const int t0 = ld(T, i);
+  const int t0 = T[i];
   if (t0 < 0) return;
-  const Seg se = seg_f64(ld(RHO, i), tf.rho0, tf.inv_drho, tf.n_rho);
+  const Seg se = seg_f64(RHO[i], tf.rho0, tf.inv_drho, tf.n_rho);
   DF[i] = spl_deriv(tf.emb, (uint32_t)(t0 * tf.n_rho + se.m), se.p);
 }
 

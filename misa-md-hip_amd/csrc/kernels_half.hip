@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_force_back(Geom g, Halo h, const int
       double4 p = make_double4(0.0, 0.0, 0.0, -1.0);
       uint32_t qi = 0;
       if (src >= 0) {
-        p = ld(P, (uint32_t)src);
+        p = P[src];
         const int32_t lxh = idx % h.lx, r = idx / h.lx, s = r & 1, yz = r >> 1;
         const int32_t ly = yz % h.ly, lzi = yz / h.ly;
         qi = local_q(g, s, wrap1(t.x0 + lxh - h.rx, g.bxh), wrap1(t.y0 + ly - h.ry, g.by),
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kBlock) void k_force_back(Geom g, Halo h, const int
 #pragma unroll
     for (int u = 0; u < kBackUnroll; u++) {
       j[u] = t.c + o[k + u];
-      f[u] = ld(SL + (size_t)sk[k + u] * nl, lq[j[u]]);
+      f[u] = SL[(size_t)sk[k + u] * nl + lq[j[u]]];
     }
 #pragma unroll
     for (int u = 0; u < kBackUnroll; u++) {

@@ -410,11 +410,13 @@ int fill_lists(mmd_server *S, ListBuf &b, const std::vector<int> &reps, bool is_
   }
   if (n_site >= (size_t)INT32_MAX || n_tile >= (size_t)INT32_MAX || n_ext >= (size_t)INT32_MAX)
     return fail(MMD_E_RANGE, "serving: work list too long");
+  // graphs hold the list pointers and the grid sizes: they go before any list
+  // is freed or regrown, so a failed allocation leaves no graph on freed lists
+  if (is_step) drop_graphs(S);
   int rc = grow(&b.h_site, &b.d_site, &b.cap_site, n_site);
   if (!rc) rc = grow(&b.h_tile, &b.d_tile, &b.cap_tile, n_tile);
   if (!rc) rc = grow(&b.h_ext, &b.d_ext, &b.cap_ext, n_ext);
   if (rc) return rc;
-  if (is_step) drop_graphs(S);  // graphs hold the list pointers and the grid sizes
   size_t is = 0, it = 0, ie = 0;
   for (size_t k = 0; k < reps.size(); k++) {
     const int slot = (*S->R)[(size_t)reps[k]].slot;
