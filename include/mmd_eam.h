@@ -111,7 +111,9 @@ const char *mmd_last_error(void);
 
 /* hip_domain_init (src/hip_arch_acc_imp.cpp:47-99).  Allocates the
  * device-resident ghost-extended SoA state, zero-filled, types = -1.
- * Requires even ext/ghost/start along x and ext = box + 2*ghost. */
+ * Requires even ext/ghost/start along x and ext = box + 2*ghost, and fewer
+ * than 2^31 ext sites (MMD_E_RANGE; about 300 GB of state, more than one
+ * MI355X holds).  A new domain needs mmd_md_setup again before MD calls. */
 int mmd_set_domain(mmd_ctx *ctx, const mmd_domain *dom);
 /* hip_nei_offset_init (src/hip_arch_acc_imp.cpp:101-144): linear offsets in
  * the reference layout for even / odd global x parity (NeiOffset values). */
@@ -201,6 +203,21 @@ int mmd_ghost_refresh(mmd_ctx *ctx, uint32_t fields);
  * graph on first use and replays it. */
 int mmd_md_prime(mmd_ctx *ctx);
 int mmd_md_run(mmd_ctx *ctx, int32_t nsteps, double *energies, int32_t use_graph);
+/* mmd_md_run (eager launches, no graph) that allocates nothing: no device or
+ * host allocation, no graph instantiation, no event creation.  No counterpart
+ * in the reference plugin; it is for callers that cannot allocate memory
+ * while they step (a serving loop, a harness that forbids allocation).  The
+ * per-block energy partials, the energy records and their counter live in a
+ * library-owned device array of 2^20 doubles (8 MiB) per GPU, so
+ *   2 * pair-pass tiles + site blocks + interstitial partials
+ *     + 2 * max(nsteps / energy_every, 1) + 1
+ * must fit in 2^20 doubles (MMD_E_RANGE otherwise, before any launch; a
+ * 256^3-cell box with 0.5 % interstitials takes about 0.5 M).  That array
+ * is shared by every call on one GPU: calls that overlap on one GPU race.
+ * Interstitial bins must already be sized for the box (mmd_md_prime does it;
+ * MMD_E_STATE otherwise).  Synchronous on return; not for stream capture.
+ * Results equal mmd_md_run's bit for bit. */
+int mmd_md_run_noalloc(mmd_ctx *ctx, int32_t nsteps, double *energies);
 /* Potential and kinetic energy of the current state (forces current). */
 int mmd_energy(mmd_ctx *ctx, double *epot, double *ekin);
 

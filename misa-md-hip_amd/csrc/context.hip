@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <utility>
 #include <vector>
@@ -370,6 +371,67 @@ static int ensure_energy_cap(mmd_ctx *c, int64_t n) {
   return MMD_OK;
 }
 
+// ------------------------------------------- allocation-free MD run scratch
+//
+// mmd_md_run_noalloc keeps the per-block energy partials of the lattice and
+// interstitial passes, the energy records and the record counter of one call
+// in this array instead of the context's own buffers.  Every GPU holds its
+// own copy for the life of the process, shared by all calls on that GPU; a
+// call clears what it uses first, and calls that overlap on one GPU race.
+// Host code never passes `scratch` itself to a kernel or a copy (its host-side
+// name is not a device address): it uses the per-device address that
+// hipGetSymbolAddress returned when the first context on that device was
+// created.
+constexpr size_t kScratchDoubles = (size_t)1 << 20;  // 8 MiB per GPU
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static std::mutex g_scratch_mu;
+static double *g_scratch[kScratchDevices] = {};
+
+// the calling thread's current device must be `device`
+static int scratch_bind(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kScratchDevices)
+    return fail(MMD_E_RANGE, "device %d beyond the %d the library's scratch table covers", device, kScratchDevices);
+  if (g_scratch[device]) return MMD_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(MMD_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+  g_scratch[device] = static_cast<double *>(p);
+  return MMD_OK;
+}
+
+static double *scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return device >= 0 && device < kScratchDevices ? g_scratch[device] : nullptr;
+}
+
+// Points the context's energy partials, records and counter at the scratch
+// array for the duration of one call (and turns event timing off, whose
+// events are created on demand); restores the context's own on exit.
+struct ScratchUse {
+  mmd_ctx *c;
+  double *e_rho, *e_force, *e_ke, *e_itl, *d_energy;
+  int32_t *d_counter;
+  bool timing;
+  ScratchUse(mmd_ctx *cc, double *S, int64_t nt, int64_t nb, int64_t ni, int64_t n_rec)
+      : c(cc), e_rho(cc->e_rho), e_force(cc->e_force), e_ke(cc->e_ke), e_itl(cc->e_itl), d_energy(cc->d_energy),
+        d_counter(cc->d_counter), timing(cc->timing) {
+    c->e_rho = S;
+    c->e_force = S + nt;
+    c->e_ke = S + 2 * nt;
+    c->e_itl = ni > 0 ? S + 2 * nt + nb : c->e_itl;
+    c->d_energy = S + 2 * nt + nb + ni;
+    c->d_counter = reinterpret_cast<int32_t *>(c->d_energy + 2 * n_rec);
+    c->timing = false;
+  }
+  ~ScratchUse() {
+    c->e_rho = e_rho, c->e_force = e_force, c->e_ke = e_ke, c->e_itl = e_itl, c->d_energy = d_energy;
+    c->d_counter = d_counter, c->timing = timing;
+  }
+};
+
 }  // namespace mmd
 
 // ================================================================== C ABI
@@ -383,6 +445,7 @@ extern "C" int mmd_ctx_create(int32_t device, mmd_ctx **out) {
     HIP_TRY(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(MMD_E_INVALID, "mmd_ctx_create: device %d of %d", device, n);
     HIP_TRY(hipSetDevice(device));
+    if (int rs = scratch_bind(device)) return rs;
     mmd_ctx *c = new mmd_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -1275,6 +1338,52 @@ extern "C" int mmd_md_run(mmd_ctx *c, int32_t nsteps, double *energies, int32_t 
         energies[2 * step + 1] = buf[2 * r + 1];
       }
     }
+    return MMD_OK;
+  })
+}
+
+extern "C" int mmd_md_run_noalloc(mmd_ctx *c, int32_t nsteps, double *energies) {
+  GUARD({
+    clear_error();
+    int rc = ctx_bind(c);
+    if (rc) return rc;
+    if (!c->md_ready) return fail(MMD_E_STATE, "mmd_md_run_noalloc: call mmd_md_setup first");
+    if ((rc = check_ready(c, "mmd_md_run_noalloc")) || (rc = check_cutoff(c, c->md.cutoff, "mmd_md_run_noalloc")))
+      return rc;
+    if (nsteps < 0) return fail(MMD_E_INVALID, "mmd_md_run_noalloc: nsteps < 0");
+    double *S = scratch_of(c->device);
+    if (!S) return fail(MMD_E_STATE, "mmd_md_run_noalloc: no scratch address for device %d", c->device);
+    const int every = c->md.energy_every;
+    const int64_t n_rec = every > 0 ? nsteps / every : 0;
+    const int64_t nt = tile_blocks(c->g), nb = site_blocks(c->g);
+    const int64_t ni = c->n_itl > 0 ? c->itl_part_n + c->itl_kick_n : 0;
+    // partials, records, then one double holding the record counter
+    const int64_t need = 2 * nt + nb + ni + 2 * std::max<int64_t>(n_rec, 1) + 1;
+    if (need > (int64_t)kScratchDoubles)
+      return fail(MMD_E_RANGE,
+                  "mmd_md_run_noalloc: %lld energy partials and records exceed the %zu doubles of scratch "
+                  "(fewer energy records or a smaller domain)",
+                  (long long)need, kScratchDoubles);
+    if (c->n_itl > 0 && !itl_bins_ready(c))
+      return fail(MMD_E_STATE, "mmd_md_run_noalloc: interstitial bins are not sized for this box (mmd_md_prime "
+                               "sizes them)");
+    ScratchUse use(c, S, nt, nb, ni, n_rec);
+    // the previous call's contents: partials of blocks a grid does not reach and the counter start at zero
+    HIP_TRY(launch_fill_f64(c->stream, S, need, 0.0));
+    auto records = [&](int32_t s) { return every > 0 && (s + 1) % every == 0; };
+    for (int32_t s = 0; s < nsteps; s++) {
+      const bool want_e = records(s), fused = s > 0, prev_e = fused && records(s - 1);
+      if ((rc = md_step(c, fused, prev_e, want_e))) return rc;
+    }
+    if (nsteps > 0 && (rc = md_close(c, records(nsteps - 1)))) return rc;
+    if ((rc = itl_check(c))) return rc;
+    if (energies)
+      for (int64_t r = 0; r < n_rec; r++) {
+        const int64_t step = (r + 1) * every - 1;
+        HIP_TRY(hipMemcpyAsync(energies + 2 * step, c->d_energy + 2 * r, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+      }
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return MMD_OK;
   })
 }

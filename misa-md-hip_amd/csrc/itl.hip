@@ -824,12 +824,25 @@ static ItlDev make_dev(mmd_ctx *c) {
   return I;
 }
 
+static void bin_grid(const mmd_ctx *c, int &nx, int &ny, int &nz) {
+  const double bs = 2.0 * c->md.cutoff;
+  nx = std::max(1, (int)(c->md.box_len[0] / bs)), ny = std::max(1, (int)(c->md.box_len[1] / bs)),
+  nz = std::max(1, (int)(c->md.box_len[2] / bs));
+}
+
+bool itl_bins_ready(mmd_ctx *c) {
+  if (!c->itl_state || !c->md_ready) return false;
+  const ItlState &s = state(c);
+  int nx, ny, nz;
+  bin_grid(c, nx, ny, nz);
+  return s.bins && nx == s.nbx && ny == s.nby && nz == s.nbz;
+}
+
 // bins sized from the MD box; (re)allocated when the box changes
 static int ensure_bins(mmd_ctx *c) {
   ItlState &s = state(c);
-  const double bs = 2.0 * c->md.cutoff;
-  const int nx = std::max(1, (int)(c->md.box_len[0] / bs)), ny = std::max(1, (int)(c->md.box_len[1] / bs)),
-            nz = std::max(1, (int)(c->md.box_len[2] / bs));
+  int nx, ny, nz;
+  bin_grid(c, nx, ny, nz);
   if (s.bins && nx == s.nbx && ny == s.nby && nz == s.nbz) return MMD_OK;
   ctx_drop_graphs(c);
   dfree_t(s.bins);

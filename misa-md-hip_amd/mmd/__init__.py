@@ -159,6 +159,7 @@ def lib():
         "mmd_ghost_refresh": (C.c_int, [P, C.c_uint32]),
         "mmd_md_prime": (C.c_int, [P]),
         "mmd_md_run": (C.c_int, [P, C.c_int32, _d, C.c_int32]),
+        "mmd_md_run_noalloc": (C.c_int, [P, C.c_int32, _d]),
         "mmd_energy": (C.c_int, [P, _d, _d]),
         "mmd_itl_set": (C.c_int, [P, C.c_int32, _d, _d, _d, _d, _d, _d, _i8, _i32, C.c_int32, C.c_int32]),
         "mmd_itl_get": (C.c_int, [P, _i32, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _d, _i8]),
@@ -456,6 +457,9 @@ class Context:
 
     def md_run(self, nsteps, energies=None, use_graph=False):
         _check(lib().mmd_md_run(self.h, nsteps, ptr(energies), 1 if use_graph else 0))
+
+    def md_run_noalloc(self, nsteps, energies=None):
+        _check(lib().mmd_md_run_noalloc(self.h, nsteps, ptr(energies)))
 
     def energy(self):
         ep, ek = C.c_double(), C.c_double()
