@@ -15,13 +15,13 @@ The same run also measures
   * mixed precision against FP64 on the defect box (same 128^3 3-type alloy
     plus 0.5% vacancies and 0.2% interstitials at tetrahedral sites, 1200 K,
     200 steps): atom-steps/s of both, force error of mixed vs FP64 on
-    identical states (start and end of the run), the per-step energy
-    difference per atom;
+    identical states (start and end of the run, and every 10 steps along the
+    mixed trajectory), the per-step energy difference per atom;
   * the Newton's-third-law half stencil against the full stencil on the
     headline box, FP64 and mixed (`half_stencil`);
   * the same comparison on the large box (256^3 cells, 33.4 M atoms, 1%
-    vacancies, 0.5% interstitials at uniformly random tetrahedral sites,
-    1800 K, 100 steps), reported as `large_box`;
+    vacancies and 0.5% interstitials clustered in 20 Gaussian blobs of
+    sigma 8 A, 1800 K, 100 steps), reported as `large_box`;
   * the same FP64 step and the same mixed defect-box comparison with
     Finnis-Sinclair Fe tables (a mechanically stable lattice; the synthetic
     forms are far from equilibrium and disorder within ~100 steps), plus
@@ -67,14 +67,14 @@ def log(msg):
 
 
 def make_workload(cells, seed=2024, temperature=800.0, vacancy_frac=0.0, interstitial_frac=0.0, potential="synthetic",
-                  n_types=3):
+                  n_types=3, n_blobs=0, blob_sigma=8.0):
     """The benchmark box.  potential "synthetic": the functional forms the
     benchmark configurations name (phi = A (rc-r)^2 (1 + B/r), rho = (rc-r)^2
     exp(-r), F = -sqrt(rho) + C rho^2, A, B, C jittered +-10% per type / pair);
     "fs": Finnis-Sinclair Fe, a mechanically stable lattice (reported apart)."""
     spec = mmd.lattice_spec((cells, cells, cells), ghost_cells=3, a=A, n_types=n_types, sigma=0.05,
                             temperature=temperature, vacancy_frac=vacancy_frac,
-                            interstitial_frac=interstitial_frac, seed=seed)
+                            interstitial_frac=interstitial_frac, n_blobs=n_blobs, blob_sigma=blob_sigma, seed=seed)
     cap = int(interstitial_frac * 2 * cells ** 3 * 1.5) + 16
     dom, box, st, itl = mmd.lattice_init(spec, itl_cap=cap)
     jit = 0.1 if n_types > 1 else 0.0
@@ -261,7 +261,30 @@ def half_stencil(w, steps, warmup):
     return out
 
 
-def mixed_vs_fp64(w, steps, warmup, migrate=False, desc=None):
+def force_err_along(w, steps, warmup, every=ENERGY_EVERY):
+    """Mixed against FP64 forces on identical states every `every` steps of the
+    mixed trajectory (untimed): the mixed run is stopped after each chunk, its
+    state copied into an FP64 context and the forces of both compared."""
+    ctx = open_ctx(w, mmd.PREC_MIXED)
+    if warmup:
+        ctx.md_run(warmup)
+    out = []
+    for s in range(every, steps + 1, every):
+        ctx.md_run(every)
+        st = w["st"].copy()
+        ctx.download(st, mmd.F_X | mmd.F_V | mmd.F_TYPE | mmd.F_F)
+        it = ctx.itl_get(w["itl"].n) if w["itl"].n else w["itl"]
+        fm = forces(ctx, w)
+        ref = open_ctx(w, mmd.PREC_FP64, state=st, itl=it)
+        f64 = forces(ref, w)
+        ref.close()
+        nw, pa = force_err(fm, f64)
+        out.append({"step": s, "normwise": nw, "max_over_rms": pa})
+    ctx.close()
+    return out
+
+
+def mixed_vs_fp64(w, steps, warmup, migrate=False, desc=None, per_step_err=False):
     n_atoms = n_atoms_of(w)
     r64 = run_gpu(w, mmd.PREC_FP64, steps, warmup, keep_state=True)
     rmx = run_gpu(w, mmd.PREC_MIXED, steps, warmup, keep_state=True)
@@ -302,6 +325,7 @@ def mixed_vs_fp64(w, steps, warmup, migrate=False, desc=None):
             "roofline": mixed_roof,
             "rho_df_kernel_ms": round(mk[mmd.K_RHO] / max(mn[mmd.K_RHO], 1), 4),
             "interstitial_ms_per_step": round(mk[mmd.K_ITL] / steps, 4),
+            **({"force_err_every_10_steps": force_err_along(w, steps, warmup)} if per_step_err else {}),
             **({"migration": mig} if mig else {})}
 
 
@@ -462,9 +486,9 @@ def main():
     flops_rho = (8 * S + 10 * Nc + 10) * n_atoms
     bytes_force = 57 * n_atoms
     achieved = flops_force / t_force / 1e12
-    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "roof_note": "compute roof: FP64 vector ALU (no matrix instructions in this kernel); MI355X's FP64 "
-                             "vector and matrix peaks are both 78.6 TFLOP/s",
+    roofline = {"bound": "valu_fp64", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "roof_note": "compute roof: the FP64 vector ALU (the kernel uses no matrix instructions; MI355X's FP64 "
+                             "vector and matrix peaks are both 78.6 TFLOP/s)",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": load_traffic(),
                 "kernel": "k_force<FP64>", "kernel_ms": round(t_force * 1e3, 4),
                 "flops_per_atom": round(flops_atom, 1), "alg_bytes_per_atom": 57,
@@ -495,18 +519,18 @@ def main():
         log(f"bench: mixed vs FP64 on the defect box (0.5% vacancies, 0.2% interstitials, 1200 K), "
             f"{args.mixed_steps} steps")
         wd = make_workload(args.cells, seed=2025, temperature=1200.0, vacancy_frac=0.005, interstitial_frac=0.002)
-        out["mixed"] = mixed_vs_fp64(wd, args.mixed_steps, args.warmup)
+        out["mixed"] = mixed_vs_fp64(wd, args.mixed_steps, args.warmup, per_step_err=True)
         del wd
 
     if not args.no_large and not args.no_mixed:
         log(f"bench: large box, 256^3 cells, mixed vs FP64, {args.large_steps} steps")
-        wl = make_workload(256, seed=2026, temperature=1800.0, vacancy_frac=0.01, interstitial_frac=0.005)
+        wl = make_workload(256, seed=2026, temperature=1800.0, vacancy_frac=0.01, interstitial_frac=0.005,
+                           n_blobs=20, blob_sigma=8.0)
         out["large_box"] = mixed_vs_fp64(
             wl, args.large_steps, args.warmup,
-            desc=f"BCC 256^3 cells ({n_atoms_of(wl)} atoms), 3-type alloy, 1% vacancies, {wl['itl'].n} "
-                 "interstitials (0.5%) at uniformly random tetrahedral sites (not the 20 Gaussian blobs of the "
-                 "named configuration, whose density the interstitial lists do not support), 1800 K, dt 1 fs, "
-                 "fully device resident with in-place ghost refresh")
+            desc=f"BCC 256^3 cells ({n_atoms_of(wl)} atoms), 3-type alloy, 1% vacancies and {wl['itl'].n} "
+                 "interstitials (0.5%) clustered in 20 seeded Gaussian blobs (sigma 8 A) at tetrahedral sites, "
+                 "1800 K, dt 1 fs, fully device resident with in-place ghost refresh")
         del wl
 
     if not args.no_fs:

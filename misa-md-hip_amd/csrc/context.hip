@@ -1239,7 +1239,8 @@ extern "C" int mmd_md_setup(mmd_ctx *c, const mmd_md_params *p) {
     c->md = *p;
     c->md_ready = true;
     ctx_drop_graphs(c);
-    return MMD_OK;
+    // interstitial bins follow the box, lattice constant and cutoff: sized here, never inside a step
+    return itl_setup(c);
   })
 }
 

@@ -130,6 +130,8 @@ int itl_kick(mmd_ctx *c, double h, bool drift, bool want_ke);
 int itl_kick_fused(mmd_ctx *c, double h, bool want_ke);
 void itl_free(mmd_ctx *c);
 int itl_check(mmd_ctx *c);
+// sizes the interstitial bins and energy partials for the MD box (no-op without a list)
+int itl_setup(mmd_ctx *c);
 // the interstitial bins are sized for the current MD box (the passes then allocate nothing)
 bool itl_bins_ready(mmd_ctx *c);
 

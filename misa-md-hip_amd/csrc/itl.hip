@@ -8,12 +8,16 @@
 //     box) and every other interstitial within rc (minimum image);
 //   * lattice<->interstitial pairs are {(i,k): site(i) in W(anchor k),
 //     type_i >= 0, r_ik < rc}; a ghost site stands for its periodic source.
-// Determinism without atomics on physical quantities: each affected lattice
-// site has one owner, the lowest-numbered interstitial interacting with it,
-// whose lane adds every interstitial contribution to that site in increasing
-// interstitial order.  Interstitial neighbour lists (pairs closer than 2 rc,
-// the only interstitials that can share a lattice site) come from periodic
-// bins filled with atomics and are then sorted, so their order is fixed.
+// Every step the interstitials are binned by the cell of their anchor with an
+// exact counting sort (counts, prefix sum, placement, then a rank by id inside
+// each bin), so the bins hold any density, in a fixed order.  Two passes then
+// evaluate every pair from each side, with no atomics on physical quantities:
+//   * one wave per interstitial: its lattice sites of W (loads of a chunk of W
+//     in flight together, in-cutoff sites compacted through LDS) and the
+//     interstitials of the bins around its anchor;
+//   * one wave per bin of lattice sites: each lane a site, every interstitial
+//     anchored within W's reach tested against it in bin order, the site's
+//     rho and df (or force) updated by that lane alone.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -27,25 +31,32 @@
 
 namespace mmd {
 
-constexpr int kItlNbrCap = 32;  // interstitials within 2 rc of one interstitial
-constexpr int kBinCap = 64;     // interstitials per bin
-
+// Interstitial bins: the local box's cells cut into nb[0] x nb[1] x nb[2]
+// bins of at least bc[d] cells along each axis; an interstitial belongs to
+// the bin of its anchor's cell.
 struct ItlDev {
   const double *x, *y, *z;  // positions as stored (unwrapped)
   double *fx, *fy, *fz, *rho, *df;
   const int8_t *type;
-  int32_t *ax, *ay, *az;        // anchor site, local lattice coordinates (x doubled)
-  double *px, *py, *pz;         // positions shifted by the same periodic image as the anchor
-  int32_t *nbr, *nnb;           // [n][kItlNbrCap] sorted neighbour ids, counts
-  int32_t *bin_count, *bin_list;
-  int32_t *err;
-  const int32_t *wide;          // [n_wide][3], parity 0
+  int32_t *ax, *ay, *az;  // anchor site, local lattice coordinates (x doubled), wrapped into the box
+  double *px, *py, *pz;   // positions shifted by the same periodic image as the anchor
+  int32_t *bin;           // [cap] bin of each interstitial
+  int32_t *bin_count;     // [nbins] interstitials per bin, then the placement counters
+  int32_t *bin_start;     // [nbins + 1] exclusive prefix sum of the counts
+  int32_t *scan_tot;      // [scan blocks] per-block totals of that sum
+  int32_t *tmp;           // [cap] placement by bin before the rank by id
+  double4 *spos;          // [cap] (px, py, pz, type) in (bin, id) order
+  int4 *sanc;             // [cap] (ax, ay, az, id) in (bin, id) order
+  int32_t *err;           // bit 0: an interstitial position beyond the lattice's coordinate range
+  const int32_t *wide;    // [n_wide][3] wide stencil W, parity 0
   int32_t n, n_wide, max_d2q;
-  int32_t nbx, nby, nbz;
-  double bsx, bsy, bsz;
-  double Lx, Ly, Lz, a, rc2, rc2x4;
+  int32_t nb[3], nbins;   // bins per axis
+  int32_t bc[3];          // fewest cells per bin along each axis
+  int32_t rii[3];         // interstitial <-> interstitial search reach, cells
+  int32_t rw[3];          // reach of W, cells
+  int32_t ii_staged;      // anchors within twice W's radius include every interstitial within rc
+  double Lx, Ly, Lz, a, rc2;
 };
-
 namespace {
 
 constexpr double kFtm2v = 9.64853321e-3;
@@ -64,33 +75,35 @@ __device__ __forceinline__ int wrap_n(int v, int n) {
   return v < 0 ? v + n : v;
 }
 
-// centred periodic difference in [-n/2, n/2)
-__device__ __forceinline__ int cdiff(int v, int n) {
-  v = wrap_n(v, n);
-  return v >= n - n / 2 ? v - n : v;
+// centred periodic difference s - a of two coordinates in [0, n): in [-n/2, n/2)
+// (n even; for odd n in [-(n-1)/2, (n-1)/2])
+__device__ __forceinline__ int cdiff0(int s, int a, int n) {
+  int v = s - a;
+  v = v >= n - n / 2 ? v - n : v;
+  return v < -(n / 2) ? v + n : v;
 }
 
 __device__ __forceinline__ double minimg(double d, double L) { return d - L * rint(d / L); }
 
-// Is lattice site (sx, sy, sz) (local, x doubled) in W of interstitial m and
-// closer than rc?  Returns the ext device index of the image seen from m.
-__device__ __forceinline__ bool li_pair(const Geom &g, const ItlDev &I, const double4 *P, int m, int sx, int sy,
-                                        int sz, int64_t &e, double &d2, double &dx, double &dy, double &dz) {
-  const int aX = I.ax[m], aY = I.ay[m], aZ = I.az[m];
-  const int dX = cdiff(sx - aX, 2 * g.bxh), dY = cdiff(sy - aY, g.by), dZ = cdiff(sz - aZ, g.bz);
-  const int sa = aX & 1, s1 = (aX + dX) & 1;
+// Is lattice site (sx, sy, sz) (local, x doubled) in W around anchor (aX, aY,
+// aZ) and closer than rc to the interstitial at (mx, my, mz) (its position
+// shifted by the anchor's periodic image)?  The site is read through its image
+// seen from the anchor (a ghost record near the box faces), so the
+// displacement site - interstitial is computed exactly as from the
+// interstitial's side.
+__device__ __forceinline__ bool li_test(const Geom &g, const ItlDev &I, const double4 *__restrict__ P, int aX, int aY,
+                                        int aZ, double mx, double my, double mz, int sx, int sy, int sz, double &d2,
+                                        double &dx, double &dy, double &dz) {
+  const int dX = cdiff0(sx, aX, 2 * g.bxh), dY = cdiff0(sy, aY, g.by), dZ = cdiff0(sz, aZ, g.bz);
+  const int sa = aX & 1, s1 = sx & 1;
   const int hy = 2 * dY + s1 - sa, hz = 2 * dZ + s1 - sa;
   if (dX * dX + hy * hy + hz * hz > I.max_d2q) return false;
   const int xe = aX + dX + 2 * g.gxh, ye = aY + dY + g.gy, ze = aZ + dZ + g.gz;
-  e = dev_index(g, xe & 1, xe >> 1, ye, ze);
-  const double4 q = P[e];
-  dx = q.x - I.px[m];
-  dy = q.y - I.py[m];
-  dz = q.z - I.pz[m];
+  const double4 q = P[dev_index(g, xe & 1, xe >> 1, ye, ze)];
+  dx = q.x - mx, dy = q.y - my, dz = q.z - mz;
   d2 = dx * dx + dy * dy + dz * dz;
   return d2 < I.rc2;
 }
-
 // Nearest BCC site of a position (global lattice coordinates, x doubled): the
 // closer of the nearest corner-sub-lattice and body-sub-lattice sites, a tie
 // going to the corner, i.e. the site whose Wigner-Seitz cell holds the point.
@@ -120,34 +133,151 @@ __device__ __forceinline__ long long wrap_img64(long long &v, long long n) {
   return q;
 }
 
-// anchor (nearest lattice site), wrapped into the box, and bins
-__global__ __launch_bounds__(kBlock) void k_itl_prepare(Geom g, ItlDev I) {
+// exclusive prefix sum of v over the block (kBlock threads); total in tot
+__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t &tot, int32_t *lds) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  int32_t incl = v;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int32_t t = __shfl_up(incl, d, kWave);
+    if (lane >= d) incl += t;
+  }
+  if (lane == kWave - 1) lds[w] = incl;
+  __syncthreads();
+  int32_t off = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < kWavesPerBlock; k++) {
+    const int32_t c = lds[k];
+    off += k < w ? c : 0;
+    all += c;
+  }
+  __syncthreads();
+  tot = all;
+  return off + incl - v;
+}
+
+// Positions this far from the origin (in lattice constants) are refused: the
+// anchor's integer coordinates must stay exact.
+constexpr double kItlMaxCoord = 1e9;
+
+// Bin of cell c along an axis of n cells cut into nb bins of floor(n / nb) or
+// ceil(n / nb) cells (no short last bin).
+__host__ __device__ __forceinline__ int bin_of_cell(int c, int n, int nb) { return (int)((int64_t)c * nb / n); }
+
+// anchor (nearest lattice site) of each interstitial, wrapped into the box, and its bin's count
+__global__ __launch_bounds__(kBlock) void k_itl_anchor(Geom g, ItlDev I) {
   const int k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= I.n) return;
   double px = I.x[k], py = I.y[k], pz = I.z[k];
-  long long X, Y, Z;
-  ws_site(I.a, px, py, pz, X, Y, Z);
+  long long X = 0, Y = 0, Z = 0;
+  const double lim = kItlMaxCoord * I.a;
+  if (fabs(px) < lim && fabs(py) < lim && fabs(pz) < lim) {  // false for NaN too
+    ws_site(I.a, px, py, pz, X, Y, Z);
+  } else {
+    atomicOr(I.err, 1);
+    px = py = pz = 0.0;
+  }
   px = fma(-(double)wrap_img64(X, 2 * (long long)g.bxh), I.Lx, px);
   py = fma(-(double)wrap_img64(Y, (long long)g.by), I.Ly, py);
   pz = fma(-(double)wrap_img64(Z, (long long)g.bz), I.Lz, pz);
   I.ax[k] = (int)X, I.ay[k] = (int)Y, I.az[k] = (int)Z;
   I.px[k] = px, I.py[k] = py, I.pz[k] = pz;
-  // bin of the wrapped position (positions may sit a little outside [0, L))
-  const double wx = px - I.Lx * floor(px / I.Lx), wy = py - I.Ly * floor(py / I.Ly), wz = pz - I.Lz * floor(pz / I.Lz);
-  const int ibx = min((int)(wx / I.bsx), I.nbx - 1), iby = min((int)(wy / I.bsy), I.nby - 1),
-            ibz = min((int)(wz / I.bsz), I.nbz - 1);
-  const int b = (ibz * I.nby + iby) * I.nbx + ibx;
-  const int slot = atomicAdd(&I.bin_count[b], 1);
-  if (slot < kBinCap)
-    I.bin_list[b * kBinCap + slot] = k;
-  else
-    atomicOr(I.err, 1);
+  const int b = (bin_of_cell((int)Z, g.bz, I.nb[2]) * I.nb[1] + bin_of_cell((int)Y, g.by, I.nb[1])) * I.nb[0] +
+                bin_of_cell((int)X >> 1, g.bxh, I.nb[0]);
+  I.bin[k] = b;
+  atomicAdd(&I.bin_count[b], 1);
 }
 
-__device__ __forceinline__ int bin_of(const ItlDev &I, double p, double L, double bs, int nb) {
-  const double w = p - L * floor(p / L);
-  return min((int)(w / bs), nb - 1);
+constexpr int kScanPer = 8;                     // bin counts per thread in the prefix sum
+constexpr int kScanChunk = kScanPer * kBlock;   // bins per block
+
+// per block: exclusive prefix sum of its bins' counts, block total to scan_tot
+__global__ __launch_bounds__(kBlock) void k_bin_scan_local(ItlDev I) {
+  __shared__ int32_t lds[kWavesPerBlock];
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+  int32_t v[kScanPer], s = 0;
+#pragma unroll
+  for (int u = 0; u < kScanPer; u++) {
+    v[u] = base + u < I.nbins ? I.bin_count[base + u] : 0;
+    s += v[u];
+  }
+  int32_t tot;
+  int32_t run = block_excl_scan(s, tot, lds);
+#pragma unroll
+  for (int u = 0; u < kScanPer; u++) {
+    if (base + u < I.nbins) I.bin_start[base + u] = run;
+    run += v[u];
+  }
+  if (threadIdx.x == 0) I.scan_tot[blockIdx.x] = tot;
 }
+
+// one block: exclusive prefix sum of the block totals; the grand total closes bin_start
+__global__ __launch_bounds__(kBlock) void k_bin_scan_tot(ItlDev I, int32_t nblk) {
+  __shared__ int32_t lds[kWavesPerBlock];
+  int32_t carry = 0;
+  for (int32_t c0 = 0; c0 < nblk; c0 += kBlock) {
+    const int32_t i = c0 + threadIdx.x;
+    const int32_t x = i < nblk ? I.scan_tot[i] : 0;
+    int32_t tot;
+    const int32_t ex = block_excl_scan(x, tot, lds);
+    if (i < nblk) I.scan_tot[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) I.bin_start[I.nbins] = carry;
+}
+
+// adds each block's offset; the counts restart at zero as placement counters
+__global__ __launch_bounds__(kBlock) void k_bin_scan_add(ItlDev I) {
+  const int64_t base = (int64_t)blockIdx.x * kScanChunk + (int64_t)threadIdx.x * kScanPer;
+  const int32_t off = I.scan_tot[blockIdx.x];
+#pragma unroll
+  for (int u = 0; u < kScanPer; u++)
+    if (base + u < I.nbins) {
+      I.bin_start[base + u] += off;
+      I.bin_count[base + u] = 0;
+    }
+}
+
+// placement in the bin's range (in whatever order the atomics give)
+__global__ __launch_bounds__(kBlock) void k_itl_place(ItlDev I) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= I.n) return;
+  const int b = I.bin[k];
+  I.tmp[I.bin_start[b] + atomicAdd(&I.bin_count[b], 1)] = k;
+}
+
+// final position = bin start + the number of lower ids in the bin: the (bin,
+// id) order, whatever the placement's order was
+__global__ __launch_bounds__(kBlock) void k_itl_rank(ItlDev I) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= I.n) return;
+  const int b = I.bin[k];
+  const int32_t s0 = I.bin_start[b], s1 = I.bin_start[b + 1];
+  int32_t r = 0;
+  for (int32_t j = s0; j < s1; j++) r += I.tmp[j] < k ? 1 : 0;
+  I.spos[s0 + r] = make_double4(I.px[k], I.py[k], I.pz[k], (double)I.type[k]);
+  I.sanc[s0 + r] = make_int4(I.ax[k], I.ay[k], I.az[k], k);
+  I.bin_count[b] = 0;  // the next binning starts from zero counts (every interstitial of the bin stores the same 0)
+}
+
+// Bins along one periodic axis of n cells (nb bins, bin_of_cell) that hold
+// cells lo..hi: the first bin and how many follow it cyclically, each bin at
+// most once.
+__host__ __device__ __forceinline__ void bin_span(int lo, int hi, int n, int nb, int &b0, int &cnt) {
+  if (hi - lo + 1 >= n) {
+    b0 = 0, cnt = nb;
+    return;
+  }
+  const int wl = ((lo % n) + n) % n, wh = ((hi % n) + n) % n;
+  const int bl = bin_of_cell(wl, n, nb), bh = bin_of_cell(wh, n, nb);
+  b0 = bl;
+  if (wl <= wh)
+    cnt = bh - bl + 1;
+  else
+    cnt = bh >= bl ? nb : nb - bl + bh + 1;
+}
+
+__device__ __forceinline__ int bin_next(int b, int nb) { return b + 1 == nb ? 0 : b + 1; }
 
 // exclusive prefix sum of v over the wave; total in tot
 __device__ __forceinline__ int32_t wave_excl_scan(int32_t v, int32_t &tot) {
@@ -162,64 +292,41 @@ __device__ __forceinline__ int32_t wave_excl_scan(int32_t v, int32_t &tot) {
   return incl - v;
 }
 
-// Sorted list of the interstitials closer than 2 rc (minimum image): one wave
-// per interstitial, a lane per neighbouring bin; candidates are gathered in
-// LDS and placed by rank (ids are distinct), so the list is in increasing id
-// order.  The 27 bins are those of the offsets -1..1 per axis, fewer along an
-// axis with fewer than 3 bins so that no bin is visited twice.
-__global__ __launch_bounds__(kBlock) void k_itl_nbr(ItlDev I) {
-  __shared__ int32_t cand[kWavesPerBlock][kItlNbrCap];
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  const int k = blockIdx.x * kWavesPerBlock + w;
-  const bool active = k < I.n;
-  int32_t found = 0, b = 0, nb = 0;
-  double pkx = 0.0, pky = 0.0, pkz = 0.0;
-  if (active && lane < 27) {
-    pkx = I.px[k], pky = I.py[k], pkz = I.pz[k];
-    const int ox = lane % 3 - 1, oy = (lane / 3) % 3 - 1, oz = lane / 9 - 1;
-    const bool vx = I.nbx >= 3 || (ox >= 0 && ox < I.nbx), vy = I.nby >= 3 || (oy >= 0 && oy < I.nby),
-               vz = I.nbz >= 3 || (oz >= 0 && oz < I.nbz);
-    if (vx && vy && vz) {
-      const int bx = bin_of(I, pkx, I.Lx, I.bsx, I.nbx), by = bin_of(I, pky, I.Ly, I.bsy, I.nby),
-                bz = bin_of(I, pkz, I.Lz, I.bsz, I.nbz);
-      b = (wrap_n(bz + oz, I.nbz) * I.nby + wrap_n(by + oy, I.nby)) * I.nbx + wrap_n(bx + ox, I.nbx);
-      nb = min(I.bin_count[b], kBinCap);
-    }
+// The candidate interstitials of one search: the bins of the spans b0 / nb
+// (at most 4 per axis, so at most 64: the set-up checks it), flattened in bin
+// order.  Lane l < nbt holds bin l's start, count and exclusive prefix.
+struct Cands {
+  int32_t start, cnt, pre;  // this lane's bin
+  int32_t nbt, total;
+};
+
+__device__ __forceinline__ void cands_setup(const ItlDev &I, const int b0[3], const int nb[3], Cands &C) {
+  const int lane = threadIdx.x & (kWave - 1);
+  C.nbt = nb[0] * nb[1] * nb[2];
+  C.start = C.cnt = 0;
+  if (lane < C.nbt) {
+    int bx = b0[0] + lane % nb[0], by = b0[1] + (lane / nb[0]) % nb[1], bz = b0[2] + lane / (nb[0] * nb[1]);
+    bx -= bx >= I.nb[0] ? I.nb[0] : 0;
+    by -= by >= I.nb[1] ? I.nb[1] : 0;
+    bz -= bz >= I.nb[2] ? I.nb[2] : 0;
+    const int b = (bz * I.nb[1] + by) * I.nb[0] + bx;
+    C.start = I.bin_start[b];
+    C.cnt = I.bin_start[b + 1] - C.start;
   }
-  // pass 1: count this lane's candidates; pass 2: store them at the wave offset
-  for (int pass = 0; pass < 2; pass++) {
-    int32_t off = 0, tot = 0;
-    if (pass == 1) {
-      off = wave_excl_scan(found, tot);
-      if (active && lane == 0 && tot > kItlNbrCap) atomicOr(I.err, 2);
-    }
-    int32_t c = 0;
-    for (int q = 0; q < nb; q++) {
-      const int m = I.bin_list[b * kBinCap + q];
-      if (m == k) continue;
-      const double dx = minimg(pkx - I.px[m], I.Lx), dy = minimg(pky - I.py[m], I.Ly), dz = minimg(pkz - I.pz[m], I.Lz);
-      if (dx * dx + dy * dy + dz * dz >= I.rc2x4) continue;
-      if (pass == 1 && off + c < kItlNbrCap) cand[w][off + c] = m;
-      c++;
-    }
-    if (pass == 0) found = c;
-    if (pass == 1) found = min(tot, kItlNbrCap);
-  }
-  __syncthreads();
-  if (active && lane < found) {
-    const int32_t v = cand[w][lane];
-    int32_t rank = 0;
-    for (int q = 0; q < found; q++) rank += cand[w][q] < v ? 1 : 0;
-    I.nbr[k * kItlNbrCap + rank] = v;
-  }
-  if (active && lane == 0) I.nnb[k] = found;
+  int32_t tot;
+  C.pre = wave_excl_scan(C.cnt, tot);
+  C.total = tot;
 }
 
-// local lattice coordinates (x doubled) of the periodic source of ext site (xe, ye, ze)
-__device__ __forceinline__ void src_local(const Geom &g, int xe, int ye, int ze, int &sx, int &sy, int &sz) {
-  sx = wrap_n(xe - 2 * g.gxh, 2 * g.bxh);
-  sy = wrap_n(ye - g.gy, g.by);
-  sz = wrap_n(ze - g.gz, g.bz);
+// index in the sorted arrays of flattened candidate t (every lane calls; the
+// result is meaningful for t < total)
+__device__ __forceinline__ int32_t cand_index(const Cands &C, int32_t t) {
+  int32_t j = 0;
+  for (int l = 0; l < C.nbt; l++) {
+    const int32_t p = __shfl(C.pre, l, kWave), c = __shfl(C.cnt, l, kWave), s = __shfl(C.start, l, kWave);
+    j = t >= p && t < p + c ? s + (t - p) : j;
+  }
+  return j;
 }
 
 // ext device index of wide-stencil entry w around anchor (aX, aY, aZ)
@@ -312,31 +419,94 @@ __device__ __forceinline__ void wave_lds_sync() {
 constexpr int kItlPer = 4;                 // wide-stencil entries per lane in one chunk
 constexpr int kItlChunk = kItlPer * kWave;  // entries tested per chunk
 
-// One wave per interstitial.  MODE 0: rho (+ owner-side lattice rho/df update),
-// MODE 1: force (+ owner-side lattice force update).
+// local lattice coordinates (x doubled) of the periodic source of ext site (xe, ye, ze)
+__device__ __forceinline__ void src_local(const Geom &g, int xe, int ye, int ze, int &sx, int &sy, int &sz) {
+  sx = wrap_n(xe - 2 * g.gxh, 2 * g.bxh);
+  sy = wrap_n(ye - g.gy, g.by);
+  sz = wrap_n(ze - g.gz, g.bz);
+}
+
+// One wave per interstitial k.
+//   * Its lattice sites of W: phase 1 loads a chunk of W's records together
+//     and compacts the in-cutoff sites through LDS; then a round of those
+//     pairs takes one per lane: k's own term (MODE 0: rho_k; MODE 1: the
+//     force on k), and the site's side.  The site's side is computed by its
+//     owner, the lowest-numbered interstitial interacting with it: the lane
+//     scans the interstitials anchored within twice W's radius of k's anchor
+//     (every one that can share a site with k), from the bins around it in
+//     bin order, staged 64 at a time in LDS, stops as soon as one with a lower number
+//     interacts with the site, and otherwise adds every interacting one's
+//     term in that scan order (MODE 0: to the site's lattice rho, then a new
+//     df, the energy partial taking F(rho) - F(lattice rho); MODE 1: to the
+//     site's lattice force, one full pair energy per pair).
+//   * The interstitials within rc (minimum image), anchored in the bins
+//     around k's anchor, a lane per candidate (MODE 1: half the pair energy
+//     per side).
 template <int MODE, bool WANT_E, int PREC>
-__global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const double4 *__restrict__ P,
-                                                       double *__restrict__ RHO,
-                                                       double *__restrict__ DF, double *__restrict__ FX,
-                                                       double *__restrict__ FY, double *__restrict__ FZ, TabF64 tf,
-                                                       TabMixed tmx, double *__restrict__ e_part) {
+__global__ __launch_bounds__(kBlock) void k_itl_side(Geom g, ItlDev I, const double4 *__restrict__ P,
+                                                       double *__restrict__ RHO, double *__restrict__ DF,
+                                                       double *__restrict__ FX, double *__restrict__ FY,
+                                                       double *__restrict__ FZ, TabF64 tf, TabMixed tmx,
+                                                       double *__restrict__ e_part) {
   __shared__ double lds[kWavesPerBlock];
   // in-cutoff entries of the current chunk, per wave: wide-stencil index | site type << 24, and the site's position
   __shared__ int32_t wlist[kWavesPerBlock][kItlChunk];
   __shared__ double3 wpos[kWavesPerBlock][kItlChunk];
-  const int lane = threadIdx.x & 63;
-  const int k = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+  // staged candidates of the sites' side, per wave
+  __shared__ double4 cpos[kWavesPerBlock][kWave];
+  __shared__ int4 canc[kWavesPerBlock][kWave];
+  __shared__ double cdf[kWavesPerBlock][kWave];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int k = blockIdx.x * kWavesPerBlock + wv;
   double acc = 0.0, fxk = 0.0, fyk = 0.0, fzk = 0.0, e_loc = 0.0;
   if (k < I.n) {
     const int tk = I.type[k];
     const int aX = I.ax[k], aY = I.ay[k], aZ = I.az[k];
     const int sgn = (aX & 1) ? -1 : 1;
-    const int nn = I.nnb[k];
-    const int *nb = I.nbr + k * kItlNbrCap;
     const double dfk = MODE == 1 ? I.df[k] : 0.0;
     const double pxk = I.px[k], pyk = I.py[k], pzk = I.pz[k];
-    int32_t *cw = wlist[threadIdx.x >> 6];
-    double3 *cq = wpos[threadIdx.x >> 6];
+    int32_t *cw = wlist[wv];
+    double3 *cq = wpos[wv];
+    double4 *sp = cpos[wv];
+    int4 *sa = canc[wv];
+    double *sd = cdf[wv];
+    const int ac[3] = {aX >> 1, aY, aZ}, nc[3] = {g.bxh, g.by, g.bz};
+    // candidates of the sites' side: anchored within twice W's reach of k's anchor
+    Cands C2;
+    {
+      int b0[3], nb[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++)
+        bin_span(ac[d] - 2 * I.rw[d], ac[d] + 2 * I.rw[d], nc[d], I.nb[d], b0[d], nb[d]);
+      cands_setup(I, b0, nb, C2);
+    }
+    // stages candidates t0.. (at most 64), keeping those whose anchor lies
+    // within twice W's radius of k's anchor (only they can share a site with
+    // k), in their order; returns how many were kept
+    auto stage = [&](int32_t t0) {
+      const int32_t t = t0 + lane;
+      const int32_t j = cand_index(C2, t);
+      bool keep = false;
+      int4 am = make_int4(0, 0, 0, 0);
+      if (t < C2.total) {
+        am = I.sanc[j];
+        const int dX = cdiff0(am.x, aX, 2 * g.bxh), dY = cdiff0(am.y, aY, g.by), dZ = cdiff0(am.z, aZ, g.bz);
+        const int sd2 = (am.x & 1) - (aX & 1), hy = 2 * dY + sd2, hz = 2 * dZ + sd2;
+        keep = dX * dX + hy * hy + hz * hz <= 4 * I.max_d2q;
+      }
+      const uint64_t mk = __ballot(keep);
+      wave_lds_sync();  // the previous staging is no longer read
+      if (keep) {
+        const int slot = __popcll(mk & ((1ull << lane) - 1));
+        sp[slot] = I.spos[j];
+        sa[slot] = am;
+        if (MODE == 1) sd[slot] = I.df[am.w];
+      }
+      wave_lds_sync();
+      return (int)__popcll(mk);
+    };
+    const bool one_chunk = C2.total <= kWave;
+    const int n_one = one_chunk ? stage(0) : 0;
     for (int w0 = 0; w0 < I.n_wide; w0 += kItlChunk) {
       // phase 1: every lane tests its entries of the chunk, the record loads
       // (position and type) of all of them in flight together; the entries
@@ -365,114 +535,120 @@ __global__ __launch_bounds__(kBlock) void k_itl_pass(Geom g, ItlDev I, const dou
         }
       }
       wave_lds_sync();
-      // phase 2: one in-cutoff pair per lane (rarely two)
-      for (int c = lane; c < found; c += kWave) {
-        const int32_t wt = cw[c];
-        const int w = wt & 0xffffff, ti = wt >> 24;
-        const int xe = aX + sgn * I.wide[3 * w] + 2 * g.gxh, ye = aY + sgn * I.wide[3 * w + 1] + g.gy,
-                  ze = aZ + sgn * I.wide[3 * w + 2] + g.gz;
-        const int64_t e = dev_index(g, xe & 1, xe >> 1, ye, ze);
-        const double3 qp = cq[c];
-        double dx, dy, dz;
-        const double d2 = itl_d2(make_double4(qp.x, qp.y, qp.z, 0.0), pxk, pyk, pzk, dx, dy, dz);
-        if (d2 >= I.rc2) continue;  // the phase-1 test, repeated on the same values
-        const double r = sqrt(d2);
-        double p;
-        const int m = seg(r, tf.r0, tf.inv_dr, tf.n_r, p);
-        // this pair's term, reused when the owner adds interstitial k to the site
-        double v_k = 0.0, fp_k = 0.0;
-        if (MODE == 0) {
-          v_k = itl_rho<PREC>(tf, tmx, ti, d2, m, p);
-          acc += v_k;
-        } else {
-          // force on the lattice atom along del = x_i - x_k
-          fp_k = itl_fpair<PREC>(tf, tmx, ti, tk, d2, r, m, p, DF[e], dfk);
-          fxk -= dx * fp_k, fyk -= dy * fp_k, fzk -= dz * fp_k;
-        }
-        // owner: no lower-numbered neighbour interacts with this lattice site
-        int sx, sy, sz;
-        src_local(g, xe, ye, ze, sx, sy, sz);
-        bool owner = true;
-        for (int q = 0; q < nn && owner; q++) {
-          const int mm = nb[q];
-          if (mm > k) break;
-          int64_t e2;
-          double d2b, ex, ey, ez;
-          if (li_pair(g, I, P, mm, sx, sy, sz, e2, d2b, ex, ey, ez)) owner = false;
-        }
-        if (!owner) continue;
-        const int64_t src = dev_index(g, sx & 1, (sx >> 1) + g.gxh, sy + g.gy, sz + g.gz);
-        double rho_i = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, epair = 0.0;
-        if (MODE == 0) rho_i = RHO[src];
-        const double dfi = MODE == 1 ? DF[src] : 0.0;
-        // every interacting interstitial in increasing order (k merged into its neighbours)
-        int placed_k = 0;
-        for (int q = 0; q < nn + 1; q++) {
-          int mm;
-          if (!placed_k && (q - placed_k >= nn || nb[q - placed_k] > k)) {
-            mm = k;
-            placed_k = 1;
-          } else {
-            mm = nb[q - placed_k];
-          }
-          if (mm == k) {  // the pair evaluated above: same image, same distance
-            if (MODE == 0) {
-              rho_i += tk == ti ? v_k : itl_rho<PREC>(tf, tmx, tk, d2, m, p);
-            } else {
-              gx += dx * fp_k, gy += dy * fp_k, gz += dz * fp_k;
-              if (WANT_E) epair += itl_phi<PREC>(tf, tmx, ti, tk, d2, m, p);
-            }
-            continue;
-          }
-          int64_t e2;
-          double d2b, ex, ey, ez;
-          if (!li_pair(g, I, P, mm, sx, sy, sz, e2, d2b, ex, ey, ez)) continue;
-          const int tm = I.type[mm];
-          const double rb = sqrt(d2b);
-          double pb;
-          const int mb = seg(rb, tf.r0, tf.inv_dr, tf.n_r, pb);
+      // phase 2: rounds of one in-cutoff pair per lane
+      for (int c0 = 0; c0 < found; c0 += kWave) {
+        const int c = c0 + lane;
+        bool own = false;
+        int ti = 0, sx = 0, sy = 0, sz = 0;
+        int64_t src = 0;
+        if (c < found) {
+          const int32_t wt = cw[c];
+          const int w = wt & 0xffffff;
+          ti = wt >> 24;
+          const double3 qp = cq[c];
+          double dx, dy, dz;
+          const double d2 = itl_d2(make_double4(qp.x, qp.y, qp.z, 0.0), pxk, pyk, pzk, dx, dy, dz);
+          const int xe = aX + sgn * I.wide[3 * w] + 2 * g.gxh, ye = aY + sgn * I.wide[3 * w + 1] + g.gy,
+                    ze = aZ + sgn * I.wide[3 * w + 2] + g.gz;
+          const int64_t e = dev_index(g, xe & 1, xe >> 1, ye, ze);
+          const double r = sqrt(d2);
+          double p;
+          const int m = seg(r, tf.r0, tf.inv_dr, tf.n_r, p);
           if (MODE == 0) {
-            rho_i += itl_rho<PREC>(tf, tmx, tm, d2b, mb, pb);
+            acc += itl_rho<PREC>(tf, tmx, ti, d2, m, p);
           } else {
-            const double fp = itl_fpair<PREC>(tf, tmx, ti, tm, d2b, rb, mb, pb, dfi, I.df[mm]);
-            gx += ex * fp, gy += ey * fp, gz += ez * fp;
-            if (WANT_E) epair += itl_phi<PREC>(tf, tmx, ti, tm, d2b, mb, pb);
+            // the pair force along del = x_site - x_k, with the site's df read at its ext record
+            const double fp = itl_fpair<PREC>(tf, tmx, ti, tk, d2, r, m, p, DF[e], dfk);
+            fxk -= dx * fp, fyk -= dy * fp, fzk -= dz * fp;
+          }
+          src_local(g, xe, ye, ze, sx, sy, sz);
+          src = dev_index(g, sx & 1, (sx >> 1) + g.gxh, sy + g.gy, sz + g.gz);
+          own = true;
+        }
+        double rho_i = 0.0, dfi = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, ep = 0.0;
+        if (own) {
+          if (MODE == 0) rho_i = RHO[src];
+          if (MODE == 1) dfi = DF[src];
+        }
+        const double rho_lat = rho_i;
+        for (int32_t t0 = 0; t0 < C2.total; t0 += kWave) {
+          const int n_chunk = one_chunk ? n_one : stage(t0);
+          for (int cc = 0; own && cc < n_chunk; cc++) {
+            const int4 am = sa[cc];
+            const double4 pm = sp[cc];
+            double d2b, ex, ey, ez;
+            if (!li_test(g, I, P, am.x, am.y, am.z, pm.x, pm.y, pm.z, sx, sy, sz, d2b, ex, ey, ez)) continue;
+            if (am.w < k) {
+              own = false;  // a lower-numbered interstitial interacts with this site: it owns the site
+              break;
+            }
+            const int tm = (int)pm.w;
+            const double rb = sqrt(d2b);
+            double pb;
+            const int mb = seg(rb, tf.r0, tf.inv_dr, tf.n_r, pb);
+            if (MODE == 0) {
+              rho_i += itl_rho<PREC>(tf, tmx, tm, d2b, mb, pb);
+            } else {
+              const double fp = itl_fpair<PREC>(tf, tmx, ti, tm, d2b, rb, mb, pb, dfi, sd[cc]);
+              gx += ex * fp, gy += ey * fp, gz += ez * fp;
+              if (WANT_E) ep += itl_phi<PREC>(tf, tmx, ti, tm, d2b, mb, pb);
+            }
           }
         }
-        if (MODE == 0) {
-          double pe;
-          const int me = seg(rho_i, tf.rho0, tf.inv_drho, tf.n_rho, pe);
-          if (WANT_E) {
-            double po;
-            const int mo = seg(RHO[src], tf.rho0, tf.inv_drho, tf.n_rho, po);
-            e_loc += spl_value(tf.emb, ti * tf.n_rho + me, pe) - spl_value(tf.emb, ti * tf.n_rho + mo, po);
+        if (own) {
+          if (MODE == 0) {
+            double pe;
+            const int me = seg(rho_i, tf.rho0, tf.inv_drho, tf.n_rho, pe);
+            if (WANT_E) {
+              double po;
+              const int mo = seg(rho_lat, tf.rho0, tf.inv_drho, tf.n_rho, po);
+              e_loc += spl_value(tf.emb, ti * tf.n_rho + me, pe) - spl_value(tf.emb, ti * tf.n_rho + mo, po);
+            }
+            RHO[src] = rho_i;
+            DF[src] = spl_deriv(tf.emb, ti * tf.n_rho + me, pe);
+          } else {
+            FX[src] += gx, FY[src] += gy, FZ[src] += gz;
+            if (WANT_E) e_loc += ep;
           }
-          RHO[src] = rho_i;
-          DF[src] = spl_deriv(tf.emb, ti * tf.n_rho + me, pe);
-        } else {
-          FX[src] += gx, FY[src] += gy, FZ[src] += gz;
-          if (WANT_E) e_loc += epair;
         }
       }
       wave_lds_sync();  // the list is rewritten by the next chunk
     }
-    // interstitial <-> interstitial
-    for (int q = lane; q < nn; q += kWave) {
-      const int mm = nb[q];
-      const double dx = minimg(I.px[k] - I.px[mm], I.Lx), dy = minimg(I.py[k] - I.py[mm], I.Ly),
-                   dz = minimg(I.pz[k] - I.pz[mm], I.Lz);
+    // interstitials within rc (minimum image), a lane per candidate: the
+    // staged candidates of the sites' side when they fit one staging and hold
+    // every interstitial close enough (ii_staged), else those anchored in the
+    // bins around k's anchor cell
+    auto ii_pair = [&](const double4 &q, int mm, double dfm) {
+      if (mm == k) return;
+      const double dx = minimg(pxk - q.x, I.Lx), dy = minimg(pyk - q.y, I.Ly), dz = minimg(pzk - q.z, I.Lz);
       const double d2 = dx * dx + dy * dy + dz * dz;
-      if (d2 >= I.rc2) continue;
-      const int tm = I.type[mm];
+      if (d2 >= I.rc2) return;
+      const int tm = (int)q.w;
       const double r = sqrt(d2);
       double p;
       const int m = seg(r, tf.r0, tf.inv_dr, tf.n_r, p);
       if (MODE == 0) {
         acc += itl_rho<PREC>(tf, tmx, tm, d2, m, p);
       } else {
-        const double fp = itl_fpair<PREC>(tf, tmx, tk, tm, d2, r, m, p, dfk, I.df[mm]);
+        const double fp = itl_fpair<PREC>(tf, tmx, tk, tm, d2, r, m, p, dfk, dfm);
         fxk += dx * fp, fyk += dy * fp, fzk += dz * fp;
         if (WANT_E) e_loc += 0.5 * itl_phi<PREC>(tf, tmx, tk, tm, d2, m, p);
+      }
+    };
+    if (one_chunk && I.ii_staged) {
+      if (lane < n_one) ii_pair(sp[lane], sa[lane].w, MODE == 1 ? sd[lane] : 0.0);
+    } else {
+      int b0[3], nb[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) bin_span(ac[d] - I.rii[d], ac[d] + I.rii[d], nc[d], I.nb[d], b0[d], nb[d]);
+      Cands C;
+      cands_setup(I, b0, nb, C);
+      for (int32_t t0 = 0; t0 < C.total; t0 += kWave) {
+        const int32_t t = t0 + lane;
+        const int32_t j = cand_index(C, t);
+        if (t >= C.total) continue;
+        const int mm = I.sanc[j].w;
+        ii_pair(I.spos[j], mm, MODE == 1 ? I.df[mm] : 0.0);
       }
     }
     if (MODE == 0) {
@@ -572,29 +748,6 @@ __device__ __forceinline__ bool mig_leaves(const MigDev &M, int64_t li, int64_t 
   long long X, Y, Z;
   ws_site(M.a, p.x, p.y, p.z, X, Y, Z);
   return X != xl || Y != yl || Z != zl;
-}
-
-// exclusive prefix sum of v over the block (kBlock threads); total in tot
-__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t &tot, int32_t *lds) {
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  int32_t incl = v;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const int32_t t = __shfl_up(incl, d, kWave);
-    if (lane >= d) incl += t;
-  }
-  if (lane == kWave - 1) lds[w] = incl;
-  __syncthreads();
-  int32_t off = 0, all = 0;
-#pragma unroll
-  for (int k = 0; k < kWavesPerBlock; k++) {
-    const int32_t c = lds[k];
-    off += k < w ? c : 0;
-    all += c;
-  }
-  __syncthreads();
-  tot = all;
-  return off + incl - v;
 }
 
 __global__ __launch_bounds__(kBlock) void k_mig_count(MigDev M) {
@@ -762,12 +915,31 @@ __global__ __launch_bounds__(kBlock) void k_itl_kick_fused(ItlDev I, double *vx,
       return fail(MMD_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
   } while (0)
 
+// the bin grid of a box: cells per bin, bins, search reaches (cells per axis)
+struct BinPlan {
+  int32_t nb[3], bc[3], rii[3], rw[3];
+  int32_t nbins;
+  bool operator==(const BinPlan &o) const {
+    for (int d = 0; d < 3; d++)
+      if (nb[d] != o.nb[d] || bc[d] != o.bc[d] || rii[d] != o.rii[d] || rw[d] != o.rw[d]) return false;
+    return nbins == o.nbins;
+  }
+};
+
 struct ItlState {
-  int32_t *ax = nullptr, *ay = nullptr, *az = nullptr, *nbr = nullptr, *nnb = nullptr;
+  int32_t *ax = nullptr, *ay = nullptr, *az = nullptr, *bin = nullptr, *tmp = nullptr;
   double *px = nullptr, *py = nullptr, *pz = nullptr;
-  int32_t *bins = nullptr;  // counts followed by lists
+  double4 *spos = nullptr;
+  int4 *sanc = nullptr;
   int32_t *err = nullptr;
-  int32_t nbx = 0, nby = 0, nbz = 0, max_d2q = 0;
+  int32_t max_d2q = 0;
+  int32_t wreach[3] = {0, 0, 0};  // reach of W in cells per axis
+  // bins, sized by ensure_bins for the box, lattice constant, cutoff and W
+  int32_t *bin_count = nullptr, *bin_start = nullptr, *scan_tot = nullptr;
+  BinPlan plan{};
+  int32_t nscan = 0;
+  int64_t side_blocks = 0, kick_blocks = 0;  // energy partials of a pass, of a kick
+  bool bins_ok = false;
   // migration work arrays
   int32_t *mig_blk = nullptr, *mig_claim = nullptr, *mig_cand = nullptr, *mig_cnt = nullptr;
   int64_t mig_nblk = 0, mig_nlocal = 0;
@@ -775,7 +947,6 @@ struct ItlState {
 };
 
 static ItlState &state(mmd_ctx *c) {
-  static_assert(sizeof(ItlState) > 0, "");
   if (!c->itl_state) c->itl_state = new ItlState();
   return *static_cast<ItlState *>(c->itl_state);
 }
@@ -786,14 +957,21 @@ static void dfree_t(T *&p) {
   p = nullptr;
 }
 
+static void free_bins(mmd_ctx *c, ItlState &s) {
+  dfree_t(s.bin_count), dfree_t(s.bin_start), dfree_t(s.scan_tot), dfree_t(c->e_itl);
+  s.bins_ok = false;
+  c->itl_part_n = c->itl_kick_n = 0;
+}
+
 void itl_free(mmd_ctx *c) {
   dfree_t(c->ix), dfree_t(c->iy), dfree_t(c->iz), dfree_t(c->ivx), dfree_t(c->ivy), dfree_t(c->ivz);
   dfree_t(c->ifx), dfree_t(c->ify), dfree_t(c->ifz), dfree_t(c->irho), dfree_t(c->idf), dfree_t(c->itype);
   dfree_t(c->d_wide), dfree_t(c->e_itl);
   if (c->itl_state) {
     ItlState &s = state(c);
-    dfree_t(s.ax), dfree_t(s.ay), dfree_t(s.az), dfree_t(s.nbr), dfree_t(s.nnb);
-    dfree_t(s.px), dfree_t(s.py), dfree_t(s.pz), dfree_t(s.bins), dfree_t(s.err);
+    free_bins(c, s);
+    dfree_t(s.ax), dfree_t(s.ay), dfree_t(s.az), dfree_t(s.bin), dfree_t(s.tmp);
+    dfree_t(s.px), dfree_t(s.py), dfree_t(s.pz), dfree_t(s.spos), dfree_t(s.sanc), dfree_t(s.err);
     dfree_t(s.mig_blk), dfree_t(s.mig_claim), dfree_t(s.mig_cand), dfree_t(s.mig_cnt);
     delete static_cast<ItlState *>(c->itl_state);
     c->itl_state = nullptr;
@@ -809,84 +987,159 @@ static ItlDev make_dev(mmd_ctx *c) {
   I.fx = c->ifx, I.fy = c->ify, I.fz = c->ifz, I.rho = c->irho, I.df = c->idf;
   I.type = c->itype;
   I.ax = s.ax, I.ay = s.ay, I.az = s.az, I.px = s.px, I.py = s.py, I.pz = s.pz;
-  I.nbr = s.nbr, I.nnb = s.nnb;
-  const int nbins = s.nbx * s.nby * s.nbz;
-  I.bin_count = s.bins, I.bin_list = s.bins + nbins;
+  I.bin = s.bin, I.bin_count = s.bin_count, I.bin_start = s.bin_start, I.scan_tot = s.scan_tot;
+  I.tmp = s.tmp, I.spos = s.spos, I.sanc = s.sanc;
   I.err = s.err;
   I.wide = c->d_wide;
   I.n = c->n_itl, I.n_wide = c->n_wide, I.max_d2q = s.max_d2q;
-  I.nbx = s.nbx, I.nby = s.nby, I.nbz = s.nbz;
+  for (int d = 0; d < 3; d++) I.nb[d] = s.plan.nb[d], I.bc[d] = s.plan.bc[d], I.rii[d] = s.plan.rii[d], I.rw[d] = s.plan.rw[d];
+  I.nbins = s.plan.nbins;
+  // two interstitials closer than rc have anchors closer than rc + 2 r_ws
+  // (r_ws = a sqrt(5) / 4, the Wigner-Seitz cell's reach); the sites' side
+  // stages the anchors within twice W's radius, sqrt(max_d2q) a / 2, from
+  // bins searched 2 wreach cells wide
+  const double a = c->md.lattice_const;
+  bool staged = std::sqrt((double)s.max_d2q) * a >= c->md.cutoff + 0.5 * std::sqrt(5.0) * a + 1e-9 * a;
+  for (int d = 0; d < 3; d++) staged = staged && 2 * s.plan.rw[d] >= s.plan.rii[d];
+  I.ii_staged = staged ? 1 : 0;
   I.Lx = c->md.box_len[0], I.Ly = c->md.box_len[1], I.Lz = c->md.box_len[2];
-  I.bsx = I.Lx / s.nbx, I.bsy = I.Ly / s.nby, I.bsz = I.Lz / s.nbz;
   I.a = c->md.lattice_const;
   I.rc2 = c->md.cutoff * c->md.cutoff;
-  I.rc2x4 = 4.0 * I.rc2;
   return I;
 }
 
-static void bin_grid(const mmd_ctx *c, int &nx, int &ny, int &nz) {
-  const double bs = 2.0 * c->md.cutoff;
-  nx = std::max(1, (int)(c->md.box_len[0] / bs)), ny = std::max(1, (int)(c->md.box_len[1] / bs)),
-  nz = std::max(1, (int)(c->md.box_len[2] / bs));
+// Bins of the current box.  Two interstitials closer than rc have anchor cells
+// at most rc / a + 1.5 apart along an axis (a position lies within a / 2 of
+// its anchor along each axis, the Wigner-Seitz cell's reach, and the anchor
+// within a / 2 of its cell's corner): the interstitial pairs' search reaches
+// rii cells.  W reaches wreach cells, so two interstitials that share a
+// lattice site have anchors at most 2 wreach apart: the sites' side searches
+// that far.  A bin spans at least a third of the wider search (and
+// kBinCells), so a search covers at most four bins per axis (checked by
+// ensure_bins).
+constexpr int32_t kBinCells = 5;
+
+static BinPlan bin_plan(const mmd_ctx *c, const ItlState &s) {
+  BinPlan p{};
+  const int32_t nc[3] = {c->g.bxh, c->g.by, c->g.bz};
+  const int32_t rii = (int32_t)std::floor(c->md.cutoff / c->md.lattice_const + 1.62);
+  p.nbins = 1;
+  for (int d = 0; d < 3; d++) {
+    p.rii[d] = rii;
+    p.rw[d] = s.wreach[d];
+    const int32_t reach = std::max(rii, 2 * s.wreach[d]);
+    p.bc[d] = std::min(std::max((2 * reach + 1 + 2) / 3, kBinCells), nc[d]);  // fewest cells per bin
+    p.nb[d] = std::max(nc[d] / p.bc[d], 1);
+    p.nbins *= p.nb[d];
+  }
+  return p;
 }
 
 bool itl_bins_ready(mmd_ctx *c) {
   if (!c->itl_state || !c->md_ready) return false;
-  const ItlState &s = state(c);
-  int nx, ny, nz;
-  bin_grid(c, nx, ny, nz);
-  return s.bins && nx == s.nbx && ny == s.nby && nz == s.nbz;
+  ItlState &s = state(c);
+  return s.bins_ok && bin_plan(c, s) == s.plan;
 }
 
-// bins sized from the MD box; (re)allocated when the box changes
+// The most bins a search of the given reach spans along one axis (the
+// device's bin_span), over every cell it can start from.
+static int max_span(int n, int nb, int reach) {
+  int m = 0;
+  for (int c = 0; c < n; c++) {
+    int b0, cnt;
+    bin_span(c - reach, c + reach, n, nb, b0, cnt);
+    m = std::max(m, cnt);
+  }
+  return m;
+}
+
+// Sizes the bins and the energy partials for the current box (outside any
+// capture: the passes themselves never allocate).
 static int ensure_bins(mmd_ctx *c) {
   ItlState &s = state(c);
-  int nx, ny, nz;
-  bin_grid(c, nx, ny, nz);
-  if (s.bins && nx == s.nbx && ny == s.nby && nz == s.nbz) return MMD_OK;
-  ctx_drop_graphs(c);
-  dfree_t(s.bins);
-  const size_t n = (size_t)nx * ny * nz;
-  ITL_TRY(hipMalloc((void **)&s.bins, n * (1 + kBinCap) * sizeof(int32_t)));
-  s.nbx = nx, s.nby = ny, s.nbz = nz;
+  const BinPlan p = bin_plan(c, s);
+  if (s.bins_ok && p == s.plan) return MMD_OK;
+  const int32_t nc[3] = {c->g.bxh, c->g.by, c->g.bz};
+  int spans = 1;
+  for (int d = 0; d < 3; d++) spans *= max_span(nc[d], p.nb[d], std::max(p.rii[d], 2 * p.rw[d]));
+  if (spans > kWave)
+    return fail(MMD_E_RANGE, "interstitial bins: a search would span %d bins (the passes take at most %d)", spans,
+                kWave);
+  ITL_TRY(hipStreamSynchronize(c->stream));
+  ctx_drop_graphs(c);  // graphs hold the bin arrays and the grids
+  free_bins(c, s);
+  const int64_t nscan = ((int64_t)p.nbins + kScanChunk - 1) / kScanChunk;
+  const int64_t cap = std::max(c->itl_cap, 1);
+  s.side_blocks = (cap + kWavesPerBlock - 1) / kWavesPerBlock;
+  s.kick_blocks = (cap + kBlock - 1) / kBlock;
+  const int64_t n_e = 2 * s.side_blocks + s.kick_blocks;
+  ITL_TRY(hipMalloc((void **)&s.bin_count, (size_t)p.nbins * sizeof(int32_t)));
+  ITL_TRY(hipMalloc((void **)&s.bin_start, ((size_t)p.nbins + 1) * sizeof(int32_t)));
+  ITL_TRY(hipMalloc((void **)&s.scan_tot, (size_t)nscan * sizeof(int32_t)));
+  ITL_TRY(hipMalloc((void **)&c->e_itl, (size_t)n_e * sizeof(double)));
+  ITL_TRY(launch_fill_f64(c->stream, c->e_itl, n_e, 0.0));
+  ITL_TRY(launch_fill_i32(c->stream, s.bin_count, p.nbins, 0));
+  ITL_TRY(hipStreamSynchronize(c->stream));
+  s.plan = p;
+  s.nscan = (int32_t)nscan;
+  // energy partials: rho pass, force pass, kicks
+  c->itl_part_n = 2 * s.side_blocks;
+  c->itl_kick_n = s.kick_blocks;
+  s.bins_ok = true;
   return MMD_OK;
 }
 
+int itl_setup(mmd_ctx *c) {
+  if (!c->md_ready || !c->itl_state || c->itl_cap < 1) return MMD_OK;
+  return ensure_bins(c);
+}
+
+// anchors, wrapped positions and the (bin, id) order of this step's positions
 static int itl_prepare(mmd_ctx *c) {
-  if (!c->md_ready) return fail(MMD_E_STATE, "interstitial passes need mmd_md_setup (box, lattice constant, cutoff)");
-  int rc = ensure_bins(c);
-  if (rc) return rc;
+  if (!itl_bins_ready(c))
+    return fail(MMD_E_STATE, "interstitial bins are not sized for this box (mmd_md_setup and mmd_itl_set size them)");
   ItlState &s = state(c);
   ItlDev I = make_dev(c);
-  const int nbins = s.nbx * s.nby * s.nbz;
-  ITL_TRY(launch_fill_i32(c->stream, s.bins, nbins, 0));
   const dim3 grid((c->n_itl + kBlock - 1) / kBlock);
-  k_itl_prepare<<<grid, kBlock, 0, c->stream>>>(c->g, I);
+  k_itl_anchor<<<grid, kBlock, 0, c->stream>>>(c->g, I);  // counts are zero: set at sizing, cleared by k_itl_rank
   ITL_TRY(hipGetLastError());
-  k_itl_nbr<<<dim3((c->n_itl + kWavesPerBlock - 1) / kWavesPerBlock), kBlock, 0, c->stream>>>(I);
+  k_bin_scan_local<<<dim3((unsigned)s.nscan), kBlock, 0, c->stream>>>(I);
+  ITL_TRY(hipGetLastError());
+  k_bin_scan_tot<<<1, kBlock, 0, c->stream>>>(I, s.nscan);
+  ITL_TRY(hipGetLastError());
+  k_bin_scan_add<<<dim3((unsigned)s.nscan), kBlock, 0, c->stream>>>(I);
+  ITL_TRY(hipGetLastError());
+  k_itl_place<<<grid, kBlock, 0, c->stream>>>(I);
+  ITL_TRY(hipGetLastError());
+  k_itl_rank<<<grid, kBlock, 0, c->stream>>>(I);
   ITL_TRY(hipGetLastError());
   c->itl_stale = false;
   return MMD_OK;
 }
 
-// the interstitial pass in the context's precision
+// a pass in the context's precision; ep = the pass's energy partials
+template <int MODE, bool WANT_E, int PREC>
+static void launch_itl_both(mmd_ctx *c, const ItlDev &I, double *ep) {
+  const dim3 g_side((c->n_itl + kWavesPerBlock - 1) / kWavesPerBlock);
+  k_itl_side<MODE, WANT_E, PREC><<<g_side, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx, c->fy, c->fz,
+                                                                   c->tf, c->tm, ep);
+}
+
 template <int MODE>
-static void launch_itl_pass(mmd_ctx *c, const ItlDev &I, dim3 grid, double *ep, bool want_e) {
+static int launch_itl_pass(mmd_ctx *c, const ItlDev &I, double *ep, bool want_e) {
   if (c->prec == MMD_PREC_MIXED) {
     if (want_e)
-      k_itl_pass<MODE, true, 1><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx, c->fy, c->fz,
-                                                                c->tf, c->tm, ep);
+      launch_itl_both<MODE, true, 1>(c, I, ep);
     else
-      k_itl_pass<MODE, false, 1><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx, c->fy, c->fz,
-                                                                 c->tf, c->tm, ep);
+      launch_itl_both<MODE, false, 1>(c, I, ep);
   } else {
     if (want_e)
-      k_itl_pass<MODE, true, 0><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx, c->fy, c->fz,
-                                                                c->tf, c->tm, ep);
+      launch_itl_both<MODE, true, 0>(c, I, ep);
     else
-      k_itl_pass<MODE, false, 0><<<grid, kBlock, 0, c->stream>>>(c->g, I, c->pos, c->rho, c->df, c->fx, c->fy, c->fz,
-                                                                 c->tf, c->tm, ep);
+      launch_itl_both<MODE, false, 0>(c, I, ep);
   }
+  ITL_TRY(hipGetLastError());
+  return MMD_OK;
 }
 
 int itl_rho_df(mmd_ctx *c, double cutoff, bool want_e) {
@@ -897,13 +1150,9 @@ int itl_rho_df(mmd_ctx *c, double cutoff, bool want_e) {
   hipEvent_t ev;
   int rc = timed_begin(c, MMD_K_ITL, &ev);
   if (rc || (rc = itl_prepare(c))) return rc;
-  ItlDev I = make_dev(c);
-  const dim3 grid((c->n_itl + kWavesPerBlock - 1) / kWavesPerBlock);
-  launch_itl_pass<0>(c, I, grid, c->e_itl, want_e);
-  ITL_TRY(hipGetLastError());
+  if ((rc = launch_itl_pass<0>(c, make_dev(c), c->e_itl, want_e))) return rc;
   return timed_end(c, MMD_K_ITL, ev);
 }
-
 int itl_df(mmd_ctx *c) {
   ItlDev I = make_dev(c);
   k_itl_df<<<dim3((c->n_itl + kBlock - 1) / kBlock), kBlock, 0, c->stream>>>(I, c->tf);
@@ -920,15 +1169,10 @@ int itl_force(mmd_ctx *c, double cutoff, bool want_e) {
   int rc = timed_begin(c, MMD_K_ITL, &ev);
   if (rc) return rc;
   if (c->itl_stale && (rc = itl_prepare(c))) return rc;  // positions changed since the rho pass
-  ItlDev I = make_dev(c);
-  const dim3 grid((c->n_itl + kWavesPerBlock - 1) / kWavesPerBlock);
-  // the embedding-energy partials of the rho pass live in the first half of e_itl
-  double *ep = c->e_itl + c->itl_part_n / 2;
-  launch_itl_pass<1>(c, I, grid, ep, want_e);
-  ITL_TRY(hipGetLastError());
+  // the rho pass's partials fill the first half of e_itl
+  if ((rc = launch_itl_pass<1>(c, make_dev(c), c->e_itl + c->itl_part_n / 2, want_e))) return rc;
   return timed_end(c, MMD_K_ITL, ev);
 }
-
 int itl_kick(mmd_ctx *c, double h, bool drift, bool want_ke) {
   ItlDev I = make_dev(c);
   const dim3 grid((c->n_itl + kBlock - 1) / kBlock);
@@ -974,8 +1218,9 @@ int itl_check(mmd_ctx *c) {
   int32_t e = 0;
   ITL_TRY(hipMemcpyAsync(&e, s.err, sizeof(e), hipMemcpyDeviceToHost, c->stream));
   ITL_TRY(hipStreamSynchronize(c->stream));
-  if (e & 1) return fail(MMD_E_RANGE, "interstitial bin overflow (more than %d per bin)", kBinCap);
-  if (e & 2) return fail(MMD_E_RANGE, "more than %d interstitials within 2 rc of one interstitial", kItlNbrCap);
+  if (e & 1)
+    return fail(MMD_E_NONFINITE, "an interstitial position is not finite or lies beyond %g lattice constants",
+                kItlMaxCoord);
   return MMD_OK;
 }
 
@@ -992,13 +1237,43 @@ extern "C" int mmd_itl_set(mmd_ctx *c, int32_t n, const double *x, const double 
   if (n < 0 || cap < n || cap > (1 << 24)) return fail(MMD_E_RANGE, "mmd_itl_set: need 0 <= n <= cap <= 2^24");
   if (n > 0 && (!x || !y || !z || !vx || !vy || !vz || !type || !wide3 || n_wide < 1 || n_wide > 4096))
     return fail(MMD_E_INVALID, "mmd_itl_set: arrays or wide stencil missing");
-  int32_t max_d2q = 0;
+  // W: within the ghost shell, a ball of sites around the anchor (every site
+  // within its largest squared distance, each once), and narrower than half
+  // the box, so a site's offset from an anchor is its centred periodic
+  // difference
+  int32_t max_d2q = 0, reach[3] = {0, 0, 0}, maxd[3] = {0, 0, 0};
+  std::vector<int64_t> keys;
   for (int32_t w = 0; w < n_wide; w++) {
     const int dX = wide3[3 * w], dY = wide3[3 * w + 1], dZ = wide3[3 * w + 2];
     if (std::abs(dX) > c->dom.ghost[0] || std::abs(dY) > c->dom.ghost[1] || std::abs(dZ) > c->dom.ghost[2])
       return fail(MMD_E_RANGE, "mmd_itl_set: wide stencil entry %d leaves the ghost shell", w);
     const int s1 = dX & 1;
     max_d2q = std::max(max_d2q, dX * dX + (2 * dY + s1) * (2 * dY + s1) + (2 * dZ + s1) * (2 * dZ + s1));
+    auto fd2 = [](int v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); };
+    reach[0] = std::max(reach[0], std::max(std::abs(fd2(dX)), std::abs(fd2(1 - dX))));
+    reach[1] = std::max(reach[1], std::abs(dY));
+    reach[2] = std::max(reach[2], std::abs(dZ));
+    maxd[0] = std::max(maxd[0], std::abs(dX)), maxd[1] = std::max(maxd[1], std::abs(dY));
+    maxd[2] = std::max(maxd[2], std::abs(dZ));
+    keys.push_back(((int64_t)(dZ + 1024) * 2048 + (dY + 1024)) * 2048 + (dX + 1024));
+  }
+  if (n_wide > 0) {
+    std::sort(keys.begin(), keys.end());
+    int64_t ball = 0;
+    const int rx = (int)std::ceil(std::sqrt((double)max_d2q)) + 1, ry = rx / 2 + 1;
+    for (int dZ = -ry; dZ <= ry; dZ++)
+      for (int dY = -ry; dY <= ry; dY++)
+        for (int dX = -rx; dX <= rx; dX++) {
+          const int s1 = dX & 1;
+          if (dX * dX + (2 * dY + s1) * (2 * dY + s1) + (2 * dZ + s1) * (2 * dZ + s1) <= max_d2q) ball++;
+        }
+    if (std::adjacent_find(keys.begin(), keys.end()) != keys.end() || ball != n_wide)
+      return fail(MMD_E_INVALID, "mmd_itl_set: the wide stencil must hold every site within its largest distance "
+                                 "of the anchor, each once (%d entries, the ball has %lld)",
+                  n_wide, (long long)ball);
+    if (2 * maxd[0] >= c->dom.box[0] || 2 * maxd[1] >= c->dom.box[1] || 2 * maxd[2] >= c->dom.box[2])
+      return fail(MMD_E_RANGE, "mmd_itl_set: the wide stencil reaches half the box or more (box %d x %d x %d)",
+                  c->dom.box[0], c->dom.box[1], c->dom.box[2]);
   }
   int32_t max_type = -1;
   for (int32_t k = 0; k < n; k++) {
@@ -1016,31 +1291,24 @@ extern "C" int mmd_itl_set(mmd_ctx *c, int32_t n, const double *x, const double 
     double **d[] = {&c->ix, &c->iy, &c->iz, &c->ivx, &c->ivy, &c->ivz, &c->ifx, &c->ify, &c->ifz, &c->irho,
                     &c->idf, &s.px, &s.py, &s.pz};
     for (double **p : d) ITL_TRY(hipMalloc((void **)p, nc * sizeof(double)));
-    int32_t **ip[] = {&s.ax, &s.ay, &s.az, &s.nnb};
+    int32_t **ip[] = {&s.ax, &s.ay, &s.az, &s.bin, &s.tmp};
     for (int32_t **p : ip) ITL_TRY(hipMalloc((void **)p, nc * sizeof(int32_t)));
-    ITL_TRY(hipMalloc((void **)&s.nbr, nc * kItlNbrCap * sizeof(int32_t)));
+    ITL_TRY(hipMalloc((void **)&s.spos, nc * sizeof(double4)));
+    ITL_TRY(hipMalloc((void **)&s.sanc, nc * sizeof(int4)));
     ITL_TRY(hipMalloc((void **)&c->itype, nc));
     ITL_TRY(hipMalloc((void **)&s.err, sizeof(int32_t)));
     ITL_TRY(hipMalloc((void **)&c->d_wide, (size_t)std::max(n_wide, 1) * 3 * sizeof(int32_t)));
-    // energy partials: rho-pass blocks, force-pass blocks, kinetic blocks
-    const int64_t nb_pass = (cap + kWavesPerBlock - 1) / kWavesPerBlock;
-    const int64_t nb_kick = (cap + kBlock - 1) / kBlock;
-    c->itl_part_n = 2 * nb_pass;
-    c->itl_kick_n = nb_kick;
-    ITL_TRY(hipMalloc((void **)&c->e_itl, (size_t)(2 * nb_pass + nb_kick) * sizeof(double)));
     c->itl_cap = cap;
   }
   ItlState &s = state(c);
   s.max_d2q = max_d2q;
+  for (int d = 0; d < 3; d++) s.wreach[d] = reach[d];
   c->n_wide = n_wide;
   c->n_itl = n;
   c->itl_max_type = max_type;
   c->itl_stale = true;
   if (n_wide > 0) ITL_TRY(hipMemcpy(c->d_wide, wide3, (size_t)n_wide * 3 * sizeof(int32_t), hipMemcpyHostToDevice));
   ITL_TRY(launch_fill_i32(c->stream, s.err, 1, 0));
-  const int64_t nb_pass = (c->itl_cap + kWavesPerBlock - 1) / kWavesPerBlock;
-  const int64_t nb_kick = (c->itl_cap + kBlock - 1) / kBlock;
-  ITL_TRY(launch_fill_f64(c->stream, c->e_itl, 2 * nb_pass + nb_kick, 0.0));
   if (n > 0) {
     const size_t b = (size_t)n * sizeof(double);
     ITL_TRY(hipMemcpy(c->ix, x, b, hipMemcpyHostToDevice));
@@ -1057,9 +1325,10 @@ extern "C" int mmd_itl_set(mmd_ctx *c, int32_t n, const double *x, const double 
     ITL_TRY(launch_fill_f64(c->stream, c->idf, n, 0.0));
   }
   ITL_TRY(hipStreamSynchronize(c->stream));
-  return MMD_OK;
+  // a new W or capacity changes the bins and partials: sized now when the MD box is known (else by mmd_md_setup)
+  s.bins_ok = false;
+  return itl_setup(c);
 }
-
 extern "C" int mmd_itl_get(mmd_ctx *c, int32_t *n, double *x, double *y, double *z, double *vx, double *vy, double *vz,
                            double *fx, double *fy, double *fz, double *rho, double *df, int8_t *type) {
   int rc = ctx_bind(c);

@@ -11,10 +11,12 @@ atoms), 3-type alloy 90/5/5 with the named synthetic tables (A, B, C jittered
   of the RMS force per atom, and normwise).
 * The Newton's-third-law half stencil on the headline box against the full
   stencil, both precisions.
-* The large box (256^3 cells, 33.4 M sites, 1 % vacancies, 0.5 % interstitials
-  at random tetrahedral sites, 1800 K) by properties only: the net force over
-  lattice and interstitial atoms is zero in both precisions, and mixed agrees
-  with FP64 within the same bound.
+* The large box as named (256^3 cells, 33.4 M sites, 1 % vacancies and 0.5 %
+  interstitials clustered in 20 Gaussian blobs of sigma 8 A, 1800 K) by
+  properties only: the net force over lattice and interstitial atoms is zero
+  in both precisions, and mixed agrees with FP64 within the same bound (the
+  oracle's interstitial pairs are O(N^2); its parity at the blobs' packing
+  density runs on a reduced box in test_gpu_itl.py).
 """
 import os
 
@@ -114,12 +116,13 @@ def _device_pass_itl(c, prec):
 
 
 def test_large_box_momentum_and_mixed_within_target():
-    """The large box (256^3 cells, 1 % vacancies, 0.5 % interstitials at
-    random tetrahedral sites, 1800 K): lattice and interstitial forces sum to
-    zero in both precisions, and mixed agrees with FP64 within the bound."""
-    print("full-size case: 256^3 cells, 3 types, 1% vacancies, 0.5% interstitials, 1800 K", flush=True)
+    """The large box as named (256^3 cells, 1 % vacancies and 0.5 %
+    interstitials in 20 Gaussian blobs of sigma 8 A, 1800 K): lattice and
+    interstitial forces sum to zero in both precisions, and mixed agrees with
+    FP64 within the bound."""
+    print("full-size case: 256^3 cells, 3 types, 1% vacancies, 0.5% interstitials in 20 blobs, 1800 K", flush=True)
     c = W.case(cells=(256, 256, 256), n_types=3, sigma=0.05, temperature=1800.0, vacancy_frac=0.01,
-               interstitial_frac=0.005, seed=2026)
+               interstitial_frac=0.005, n_blobs=20, blob_sigma=8.0, seed=2026)
     c["tab"] = mmd.synth_tables(n_types=3, rc=RC, jitter=0.1, seed=2026)
     assert c["itl"].n > 160000
     Fa = _device_pass_itl(c, mmd.PREC_FP64)

@@ -27,7 +27,26 @@ ITL_CASES = {
     # rho_t differs per type: each side of a lattice-interstitial pair takes the other's rho
     "bcc10_3type_distinct_rho": dict(cells=(10, 9, 8), n_types=3, sigma=0.05, temperature=1200.0,
                                      vacancy_frac=0.005, interstitial_frac=0.006, seed=24, distinct_rho=True),
+    # one blob of sigma 8 A, as the large-box configuration places them, filled to the initial
+    # conditions' packing limit (at most one interstitial per 3x3x3 cells around each):
+    # 829 interstitials, up to 22 within 2 rc of one and 4 within rc
+    "blob_packed_24": dict(cells=(24, 24, 24), n_types=3, sigma=0.05, temperature=1200.0, vacancy_frac=0.01,
+                           interstitial_frac=0.03, n_blobs=1, blob_sigma=8.0, seed=25),
 }
+
+
+def test_packed_blob_case_is_dense():
+    """The packed-blob case holds the densest interstitial neighbourhoods the
+    initial conditions make (the bins and both pass sides see them)."""
+    c = W.case(**ITL_CASES["blob_packed_24"])
+    it, L = c["itl"], c["box"]
+    P = np.stack([it.x, it.y, it.z], 1)
+    d = P[:, None, :] - P[None, :, :]
+    d -= L * np.round(d / L)
+    r2 = (d ** 2).sum(-1)
+    assert it.n > 800
+    assert ((r2 < (2 * RC) ** 2).sum(1) - 1).max() >= 20
+    assert ((r2 < RC ** 2).sum(1) - 1).max() >= 3
 
 
 def _setup(c, prec=mmd.PREC_FP64):
@@ -85,8 +104,9 @@ def test_interstitial_passes_match_oracle(name):
     assert np.abs(tot).max() < 1e-9 * np.abs(F).sum()
 
 
-def test_interstitial_md_matches_oracle():
-    c = W.case(**ITL_CASES["bcc10_3type"])
+@pytest.mark.parametrize("name", ["bcc10_3type", "blob_packed_24"])
+def test_interstitial_md_matches_oracle(name):
+    c = W.case(**ITL_CASES[name])
     ref, ritl, T, _ = _oracle(c)
     nsteps = 8
     en_ref = O.md_run_itl(c["dom"], c["even"], c["odd"], c["wide3"], A, T, RC, c["box"], 1.0, nsteps, ref, ritl)
