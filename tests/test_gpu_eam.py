@@ -152,14 +152,45 @@ def test_results_bit_reproducible():
     assert np.array_equal(outs[0].rho, outs[1].rho) and np.array_equal(outs[0].fz, outs[1].fz)
 
 
-def test_dropin_aos_hooks_match_oracle():
+class _PinnedRecords:
+    """n _cuAtomElement records in pinned host memory (mmd_host_alloc, the
+    allocation hip_create_atoms_mem makes), as a numpy view."""
+
+    def __init__(self, n):
+        import ctypes as C
+        self.p = C.c_void_p()
+        nbytes = n * mmd.REF_AOS_DTYPE.itemsize
+        mmd._check(mmd.lib().mmd_host_alloc(C.byref(self.p), nbytes))
+        self.recs = np.frombuffer((C.c_char * nbytes).from_address(self.p.value), dtype=mmd.REF_AOS_DTYPE)
+        self.recs[...] = np.zeros(1, mmd.REF_AOS_DTYPE)
+
+    def __enter__(self):
+        return self.recs
+
+    def __exit__(self, *a):
+        del self.recs
+        mmd.lib().mmd_host_free(self.p)
+
+
+@pytest.mark.parametrize("memory", ["pageable", "pinned"])
+def test_dropin_aos_hooks_match_oracle(memory):
     """hip_eam_rho_calc / hip_eam_force_calc contract: host AoS records in,
-    rho/df (or f) of local records out, everything else untouched."""
+    rho/df (or f) of local records out, everything else untouched.  Pageable
+    records take the whole-array copies; pinned ones move only the fields a
+    pass reads (strided copies) and writes (straight into the records)."""
     c = W.case(**CASES["ragged_7x9x5"])
+    n = c["st"].x.size
+    if memory == "pinned":
+        with _PinnedRecords(n) as recs:
+            _dropin_checks(c, recs)
+    else:
+        _dropin_checks(c, np.zeros(n, mmd.REF_AOS_DTYPE))
+
+
+def _dropin_checks(c, recs):
     ref, _ = _oracle_pass(c)
     st = c["st"]
     n = st.x.size
-    recs = np.zeros(n, mmd.REF_AOS_DTYPE)
     recs["id"] = np.arange(n)
     recs["type"] = st.type
     recs["x"] = np.stack([st.x, st.y, st.z], 1)
