@@ -425,8 +425,9 @@ def serving(n_replicas, load, seed=77, burst=512, steps=200, use_graph=True, gho
     de = [abs(r["epot"] + r["ekin"] - r["epot0"] - r["ekin0"]) / r["n_atoms"] / steps for r in done]
     return {"workload": f"{n_replicas} replicas, box edge log-uniform 10-64 BCC cells (2k-524k atoms), 3-type alloy, "
                         f"300-1500 K, {steps} steps each, Poisson arrivals at {load:.0%} of measured capacity, "
-                        "mixed precision, one batched launch per pass over (replica, block) work items"
-                        + (" under a captured HIP graph" if use_graph else ", launched directly"),
+                        "mixed precision, one batched launch per pass over (replica, block) work items, grids sized "
+                        "to the work lists' capacities" + (", replayed from one captured HIP graph for every mix"
+                                                            if use_graph else ", launched directly"),
             "value": round(st["atom_steps_per_s"], 1), "unit": "atom-steps/s",
             "capacity_atom_steps_per_s": round(capacity, 1), "arrival_rate_per_s": round(rate, 3),
             "offered_load": load, "latency_p50_s": round(st["latency_p50_s"], 4),
@@ -434,6 +435,7 @@ def serving(n_replicas, load, seed=77, burst=512, steps=200, use_graph=True, gho
             "wait_mean_s": round(st["wait_mean_s"], 4), "wall_s": round(st["wall_s"], 3),
             "atom_steps": st["atom_steps"], "completed": st["completed"], "failed": st["failed"],
             "max_active": st["max_active"], "batched_steps": st["batched_steps"],
+            "graph_captures": st["graph_captures"],
             "total_energy_change_per_atom_per_step_median": float(np.median(de)) if de else None}
 
 

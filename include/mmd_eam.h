@@ -314,6 +314,7 @@ typedef struct {
   int32_t completed, failed, max_active;
   int64_t batched_steps;
   double arena_bytes;
+  int64_t graph_captures;  /* step graphs captured during the run (0 without graphs) */
 } mmd_server_stats;
 
 int mmd_server_create(const mmd_server_opts *opts, const mmd_eam_tables *tab, mmd_server **out);

@@ -2,8 +2,11 @@
 // into one contiguous range per XCD (blocks are dispatched to the 8 XCDs
 // round-robin, so blockIdx.x % 8 names the XCD), so the blocks of an XCD take
 // neighbouring blocks of the same replica and share its halo in L2.  The
-// serving loop launches one block per item (grid = items rounded up to a
-// multiple of 8); a smaller grid makes the blocks loop over their items.
+// item counts are read from device memory, so a grid of fixed size serves
+// any list: the serving loop's step launches size their grids to the lists'
+// capacities (blocks past the count exit at once) and its priming launches
+// one block per item; a grid smaller than the list makes each block walk its
+// XCD's range with a stride.
 #include "batch.h"
 #include "eam_body.h"
 #include "kernels.h"

@@ -10,7 +10,9 @@
 // together: one batched step is the velocity-Verlet sequence of the
 // single-domain path (kick + drift, ghost x, rho + df, ghost df, force, kick),
 // each pass one launch over a device work list of (replica, block) items, one
-// block per item, replayed from a captured HIP graph.  A replica's
+// block per item with the grid sized to the list's capacity, replayed from
+// one captured HIP graph whatever the mix (between mixes only the device work
+// lists change; a graph is captured again only when a list grows).  A replica's
 // blocks run the same per-block code with the same block numbering as the
 // single-domain kernels and its energies are reduced in the same fixed order,
 // so its trajectory is bit-identical alone, in any mix and on the
@@ -278,7 +280,13 @@ struct mmd_server {
   BatchDom *h_dom = nullptr, *d_dom = nullptr;  // [max_replicas], indexed by slot
   ListBuf step, prime;
   std::vector<int> step_set;  // active replicas the step lists were built for
+  // The step launches' grids follow the work lists' capacities (grown
+  // geometrically), not the current item counts, which the kernels read from
+  // device memory: blocks past the count exit at once, so one captured graph
+  // serves every mix of replicas until a list has to grow (only the device
+  // work lists change between mixes).
   hipGraphExec_t graph[4] = {};  // captured steps by (fused opening kick, energy partials)
+  int64_t graph_captures = 0;
   double *h_e = nullptr;  // pinned [max_replicas][4]
 
   // worker pool
@@ -410,9 +418,9 @@ int fill_lists(mmd_server *S, ListBuf &b, const std::vector<int> &reps, bool is_
   }
   if (n_site >= (size_t)INT32_MAX || n_tile >= (size_t)INT32_MAX || n_ext >= (size_t)INT32_MAX)
     return fail(MMD_E_RANGE, "serving: work list too long");
-  // graphs hold the list pointers and the grid sizes: they go before any list
-  // is freed or regrown, so a failed allocation leaves no graph on freed lists
-  if (is_step) drop_graphs(S);
+  // graphs hold the list pointers: they go before any list is freed or
+  // regrown, so a failed allocation leaves no graph on freed lists
+  if (is_step && (n_site > b.cap_site || n_tile > b.cap_tile || n_ext > b.cap_ext)) drop_graphs(S);
   int rc = grow(&b.h_site, &b.d_site, &b.cap_site, n_site);
   if (!rc) rc = grow(&b.h_tile, &b.d_tile, &b.cap_tile, n_tile);
   if (!rc) rc = grow(&b.h_ext, &b.d_ext, &b.cap_ext, n_ext);
@@ -426,7 +434,13 @@ int fill_lists(mmd_server *S, ListBuf &b, const std::vector<int> &reps, bool is_
     b.h_reps[k] = slot;
   }
   b.h_n[0] = (int)n_site, b.h_n[1] = (int)n_tile, b.h_n[2] = (int)n_ext, b.h_n[3] = (int)reps.size();
-  b.g_site = batch_blocks((int64_t)n_site), b.g_tile = batch_blocks((int64_t)n_tile), b.g_ext = batch_blocks((int64_t)n_ext);
+  if (is_step) {  // grids of the lists' capacities: the captured graphs stay valid for any mix that fits them
+    b.g_site = batch_blocks((int64_t)b.cap_site), b.g_tile = batch_blocks((int64_t)b.cap_tile),
+    b.g_ext = batch_blocks((int64_t)b.cap_ext);
+  } else {  // priming: direct launches, one block per item
+    b.g_site = batch_blocks((int64_t)n_site), b.g_tile = batch_blocks((int64_t)n_tile),
+    b.g_ext = batch_blocks((int64_t)n_ext);
+  }
   if (n_site) HIP_TRY(hipMemcpyAsync(b.d_site, b.h_site, n_site * sizeof(int2), hipMemcpyHostToDevice, S->cs));
   if (n_tile) HIP_TRY(hipMemcpyAsync(b.d_tile, b.h_tile, n_tile * sizeof(int2), hipMemcpyHostToDevice, S->cs));
   if (n_ext) HIP_TRY(hipMemcpyAsync(b.d_ext, b.h_ext, n_ext * sizeof(int2), hipMemcpyHostToDevice, S->cs));
@@ -570,6 +584,7 @@ struct GpuExec final : Executor {
           hipError_t ie = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
           (void)hipGraphDestroy(gr);
           HIP_TRY(ie);
+          S->graph_captures++;
         }
         HIP_TRY(hipGraphLaunch(ge, S->cs));
       } else if ((rc = step_passes(S, L, S->step, fused, want_e))) {
@@ -817,6 +832,7 @@ extern "C" int mmd_server_run(mmd_server *S, const mmd_replica_spec *specs, int3
     S->step_set.clear();
     S->done_q.clear();
     S->t0 = std::chrono::steady_clock::now();
+    const int64_t captures0 = S->graph_captures;
     GpuExec X(S);
     SchedStats st;
     rc = serve_loop(R, S->o.max_replicas, S->arena_bytes, S->o.chunk_steps, X, st, MMD_E_NOMEM);
@@ -832,6 +848,7 @@ extern "C" int mmd_server_run(mmd_server *S, const mmd_replica_spec *specs, int3
     (void)hipStreamSynchronize(S->cs);
     for (int32_t i = 0; i < n; i++) fill_result(R[(size_t)i], results[i]);
     fill_stats(R, st, S->arena_bytes, stats);
+    if (stats) stats->graph_captures = S->graph_captures - captures0;
     S->R = nullptr;
     S->specs = nullptr;
     S->step_set.clear();

@@ -107,7 +107,8 @@ class ServerStats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("atom_steps", C.c_double), ("atom_steps_per_s", C.c_double),
                 ("latency_p50_s", C.c_double), ("latency_p99_s", C.c_double), ("latency_mean_s", C.c_double),
                 ("wait_mean_s", C.c_double), ("completed", C.c_int32), ("failed", C.c_int32),
-                ("max_active", C.c_int32), ("batched_steps", C.c_int64), ("arena_bytes", C.c_double)]
+                ("max_active", C.c_int32), ("batched_steps", C.c_int64), ("arena_bytes", C.c_double),
+                ("graph_captures", C.c_int64)]
 
 
 class SimParams(C.Structure):
