@@ -64,9 +64,19 @@ struct Halo {
   int32_t n;
 };
 
-// LDS bytes of a pair-pass block: (x, y), z or (z, df), type per staged record
+// Unlike-pair queue of the force pass, one per wave: kSlots record numbers
+// (uint32) followed by kSlots doubles.
+constexpr int kSlots = kWave;
+constexpr size_t kSlotWaveBytes = (size_t)kSlots * 12;
+
+// LDS offset of the force pass's queues: after the staged records, 16-byte aligned
+__host__ __device__ inline size_t tile_queue_offset(const Halo &h) { return ((size_t)h.n * 33 + 15) / 16 * 16; }
+
+// LDS bytes of a pair-pass block: (x, y), z or (z, df), type per staged
+// record; the force pass adds the per-wave unlike-pair queues
 __host__ __device__ inline size_t tile_lds_bytes(const Halo &h, bool with_df) {
-  return (size_t)h.n * (with_df ? 33 : 25);
+  if (!with_df) return (size_t)h.n * 25;
+  return tile_queue_offset(h) + (size_t)kWavesPerBlock * kSlotWaveBytes;
 }
 
 // Loads through a uniform base pointer with a 32-bit element index, so the

@@ -94,9 +94,30 @@ __device__ __forceinline__ double rho_f64(const TabF64 &tf, int tj, double d2) {
   return spl_value(tf.rho, (uint32_t)(tj * tf.n_r + sg.m), sg.p);
 }
 
+// rho'_t of FP64 derivative record rec at fraction p
+__device__ __forceinline__ double rho_deriv_f64(const TabF64 &tf, uint32_t rec, double p) {
+  const double2 a = ld(tf.rho.d01, rec);
+  return (a.x * p + a.y) * p + ld(tf.rho.d2, rec);
+}
+
 // fpair = -(df_i rho'_{tj} + df_j rho'_{ti} + phi'_{ti tj}) / r (the scalar that
-// multiplies del = x_i - x_j); phi receives phi_{ti tj}(r) when WANT_E and the
-// (wave-uniform) entry counts the pair's energy.
+// multiplies del = x_i - x_j) from the pair's force record rp at fraction p,
+// rinv = 1/r; an unlike pair brings rho'_{ti} in rhoip_u (a like pair reuses
+// rho'_{tj}).  phi receives phi_{ti tj}(r) when WANT_E and the (wave-uniform)
+// entry counts the pair's energy.
+template <bool WANT_E>
+__device__ __forceinline__ double fpair_f64_rec(const TabF64 &tf, uint32_t rp, double p, double rinv, double dfi,
+                                                double dfj, bool unlike, double rhoip_u, bool count_e, double &phi) {
+  const double2 c0 = ld(tf.fc, 3 * rp), c1 = ld(tf.fc, 3 * rp + 1), c2 = ld(tf.fc, 3 * rp + 2);
+  const double rhojp = (c0.x * p + c0.y) * p + c1.x;
+  const double phip = (c1.y * p + c2.x) * p + c2.y;
+  const double rhoip = unlike ? rhoip_u : rhojp;
+  if (WANT_E && count_e) phi = spl_value(tf.phi, rp, p);
+  return -(dfi * rhojp + dfj * rhoip + phip) * rinv;
+}
+
+// One FP64 pair from its squared distance, rho'_{ti} loaded here (the mixed
+// path's redo of a flagged pair)
 template <bool WANT_E>
 __device__ __forceinline__ double fpair_f64(const TabF64 &tf, int ti, int tj, double d2, double dfi, double dfj,
                                             bool count_e, double &phi) {
@@ -104,29 +125,65 @@ __device__ __forceinline__ double fpair_f64(const TabF64 &tf, int ti, int tj, do
   const Seg sg = seg_f64(d2 * rinv, tf.r0, tf.inv_dr, tf.n_r);
   const uint32_t rp = __umul24((uint32_t)(ti * tf.nt + tj), (uint32_t)tf.n_r) + (uint32_t)sg.m;
   const bool unlike = ti != tj;
-  const uint32_t ra = unlike ? __umul24((uint32_t)ti, (uint32_t)tf.n_r) + (uint32_t)sg.m : 0u;
-  const double2 c0 = ld(tf.fc, 3 * rp), c1 = ld(tf.fc, 3 * rp + 1), c2 = ld(tf.fc, 3 * rp + 2);
-  // rho'_{ti} only for unlike pairs: the load runs on those lanes alone (the
-  // load unit's time follows the active lanes; out-of-cutoff lanes are too
-  // few to pay for the same treatment), into registers set first so that no
-  // copy, and no wait, is needed where the lanes rejoin
-  double2 ua = make_double2(0.0, 0.0);
-  double uc = 0.0;
-  if (unlike) {
-    ua = ld(tf.rho.d01, ra);
-    uc = ld(tf.rho.d2, ra);
+  double rhoip = 0.0;
+  if (unlike) rhoip = rho_deriv_f64(tf, __umul24((uint32_t)ti, (uint32_t)tf.n_r) + (uint32_t)sg.m, sg.p);
+  return fpair_f64_rec<WANT_E>(tf, rp, sg.p, rinv, dfi, dfj, unlike, rhoip, count_e, phi);
+}
+
+// ------------------------------------------------------ unlike-pair queue
+//
+// An unlike pair (ti != tj) also needs rho'_{ti}(r).  Gathering it on the
+// unlike lanes of every stencil entry costs a full gather instruction per
+// entry although few lanes use it: the load unit's time follows the
+// instructions, not the active lanes.  In the FP64 force pass (two gathers
+// per unlike pair, 24-byte records) a trip instead numbers its unlike
+// in-cutoff pairs across its kUnroll entries (ballot + mbcnt) and writes
+// them to the wave's LDS queue as (record, fraction); each active lane of
+// the wave then evaluates one queued pair with one gather (more rounds only
+// when more pairs are queued than lanes are active) and the owners read the
+// values back.  A wave's LDS operations execute in order, so the phases
+// need only compiler barriers.
+
+__device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// out[u] = ev(rec[u], x[u]) for every u with ul[u]; qr / qx: the wave's queue
+template <int U, class Eval>
+__device__ __forceinline__ void unlike_queue(uint32_t *qr, double *qx, const bool *ul, const uint32_t *rec,
+                                             const double *x, double *out, Eval ev) {
+  uint32_t slot[U], tot = 0;
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const uint64_t m = __ballot(ul[u]);
+    slot[u] = tot + lanes_below(m);
+    tot += (uint32_t)__popcll(m);
   }
-  const double p = sg.p;
-  const double rhojp = (c0.x * p + c0.y) * p + c1.x;
-  const double phip = (c1.y * p + c2.x) * p + c2.y;
-  const double rhoip = unlike ? (ua.x * p + ua.y) * p + uc : rhojp;
-  if (WANT_E && count_e) phi = spl_value(tf.phi, rp, p);
-  return -(dfi * rhojp + dfj * rhoip + phip) * rinv;
+  if (tot == 0) return;
+  const uint64_t act = __ballot(true);
+  const uint32_t nact = (uint32_t)__popcll(act), rank = lanes_below(act);
+  for (uint32_t r0 = 0; r0 < tot; r0 += nact) {
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (ul[u] && slot[u] - r0 < nact) qr[slot[u] - r0] = rec[u], qx[slot[u] - r0] = x[u];
+    lds_order();
+    if (r0 + rank < tot) qx[rank] = ev(qr[rank], qx[rank]);
+    lds_order();
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (ul[u] && slot[u] - r0 < nact) out[u] = qx[slot[u] - r0];
+    lds_order();
+  }
 }
 
 // Phased evaluation of a trip (mixed): d/ds tables, -(dpsi/dr)/r = -2 dpsi/ds
 // (the force-side derivative records hold -2 dpsi/ds), no square root or
-// divide; NaN where the pair needs the FP64 spline.
+// divide; NaN where the pair needs the FP64 spline.  rho'_{ti} is gathered
+// on the unlike lanes together with the trip's other loads: routed through
+// the unlike-pair queue it measured slower here, its round trip adding to
+// the phased loads' latency.
 // row_i = ti * nt * n_s and ra_i = ti * n_s (the centre's rows).
 template <bool WANT_E, int U>
 __device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, uint32_t ti, uint32_t row_i, uint32_t ra_i,
@@ -147,7 +204,9 @@ __device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, uint32_t ti
   for (int u = 0; u < U; u++) {
     b0[u] = ld(tm.fb, 2 * rp[u]);
     b1[u] = ld(tm.fb, 2 * rp[u] + 1);
-    // rho'_{ti} only on unlike lanes (like pairs reuse rho'_{tj}); see fpair_f64
+    // rho'_{ti} only on unlike lanes (like pairs reuse rho'_{tj}), into
+    // registers set first so that no copy, and no wait, is needed where the
+    // lanes rejoin
     ua[u] = make_double2(0.0, 0.0);
     if (ti != tj[u]) ua[u] = ld(tm.rho.dp, ra[u]);
     if (WANT_E && count_e[u]) {
@@ -359,6 +418,9 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
   const TileSite t = map_tile(g, h, lb);
   double2 *lxy = lds, *lzd = lds + h.n;
   int8_t *lt = reinterpret_cast<int8_t *>(lds + 2 * h.n);
+  char *qw = reinterpret_cast<char *>(lds) + tile_queue_offset(h) + (threadIdx.x >> 6) * kSlotWaveBytes;
+  uint32_t *qr = reinterpret_cast<uint32_t *>(qw);
+  double *qx = reinterpret_cast<double *>(qw + kSlots * sizeof(uint32_t));
   stage_tile<true>(g, h, t, lb, P, DF, lxy, lzd, lt);
   double e_loc = 0.0;
   if (t.active) {
@@ -426,11 +488,27 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
                 fp[u] = fpair_f64<WANT_E>(tf, t0, (int)tq[u], d2[u], dfi, dfj[u], ce[u], phi[u]);
           }
         } else {
-          // FP64: pair by pair (its large tables are gathered from L2; this
-          // measured faster than phased loads)
+          // FP64: the trip's record numbers and fractions, rho'_{ti} of its
+          // unlike pairs through the queue, then pair by pair (its large
+          // tables are gathered from L2; this measured faster than phased loads)
+          double rinv[kUnroll], pr[kUnroll], rhoq[kUnroll];
+          uint32_t rp[kUnroll], ra[kUnroll];
+          bool ul[kUnroll];
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) {
+            rinv[u] = rsqrt64(d2[u]);
+            const Seg sg = seg_f64(d2[u] * rinv[u], tf.r0, tf.inv_dr, tf.n_r);
+            pr[u] = sg.p;
+            rp[u] = __umul24(ti * (uint32_t)tf.nt + tq[u], (uint32_t)tf.n_r) + (uint32_t)sg.m;
+            ra[u] = __umul24(ti, (uint32_t)tf.n_r) + (uint32_t)sg.m;
+            ul[u] = in[u] && tq[u] != ti;
+            rhoq[u] = 0.0;
+          }
+          unlike_queue<kUnroll>(qr, qx, ul, ra, pr, rhoq,
+                                [&](uint32_t rec, double x) { return rho_deriv_f64(tf, rec, x); });
 #pragma unroll
           for (int u = 0; u < kUnroll; u++)
-            fp[u] = fpair_f64<WANT_E>(tf, t0, (int)tq[u], d2[u], dfi, dfj[u], ce[u], phi[u]);
+            fp[u] = fpair_f64_rec<WANT_E>(tf, rp[u], pr[u], rinv[u], dfi, dfj[u], tq[u] != ti, rhoq[u], ce[u], phi[u]);
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
