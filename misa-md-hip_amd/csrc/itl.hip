@@ -433,8 +433,9 @@ __device__ __forceinline__ void src_local(const Geom &g, int xe, int ye, int ze,
 //     force on k), and the site's side.  The site's side is computed by its
 //     owner, the lowest-numbered interstitial interacting with it: the lane
 //     scans the interstitials anchored within twice W's radius of k's anchor
-//     (every one that can share a site with k), from the bins around it in
-//     bin order, staged 64 at a time in LDS, stops as soon as one with a lower number
+//     and within 2 rc of k (every one that can share a site with k), from the
+//     bins around it in bin order, staged in LDS (all at once when at most 64
+//     remain, else 64 at a time), stops as soon as one with a lower number
 //     interacts with the site, and otherwise adds every interacting one's
 //     term in that scan order (MODE 0: to the site's lattice rho, then a new
 //     df, the energy partial taking F(rho) - F(lattice rho); MODE 1: to the
@@ -480,33 +481,53 @@ __global__ __launch_bounds__(kBlock) void k_itl_side(Geom g, ItlDev I, const dou
         bin_span(ac[d] - 2 * I.rw[d], ac[d] + 2 * I.rw[d], nc[d], I.nb[d], b0[d], nb[d]);
       cands_setup(I, b0, nb, C2);
     }
-    // stages candidates t0.. (at most 64), keeping those whose anchor lies
-    // within twice W's radius of k's anchor (only they can share a site with
-    // k), in their order; returns how many were kept
-    auto stage = [&](int32_t t0) {
+    // A candidate is kept, in its (bin, id) order, when it can interact with
+    // one of k's in-cutoff sites: anchored within twice W's radius of k's
+    // anchor, and within 2 rc of k's position (a site within rc of both k
+    // and m puts m within 2 rc of k; the margin covers rounding).  Dropping
+    // the others changes neither a site's owner nor its summation order.
+    const double lim = 2.0 * sqrt(I.rc2) + 1e-6, lim2 = lim * lim;
+    // stages the kept candidates of flattened range [t0, t0 + 64) from LDS
+    // slot base on (slots past 63 are not written); returns how many it kept
+    auto stage_chunk = [&](int32_t t0, int base) {
       const int32_t t = t0 + lane;
       const int32_t j = cand_index(C2, t);
       bool keep = false;
       int4 am = make_int4(0, 0, 0, 0);
+      double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
       if (t < C2.total) {
         am = I.sanc[j];
         const int dX = cdiff0(am.x, aX, 2 * g.bxh), dY = cdiff0(am.y, aY, g.by), dZ = cdiff0(am.z, aZ, g.bz);
         const int sd2 = (am.x & 1) - (aX & 1), hy = 2 * dY + sd2, hz = 2 * dZ + sd2;
-        keep = dX * dX + hy * hy + hz * hz <= 4 * I.max_d2q;
+        if (dX * dX + hy * hy + hz * hz <= 4 * I.max_d2q) {
+          q = I.spos[j];
+          const double ux = minimg(q.x - pxk, I.Lx), uy = minimg(q.y - pyk, I.Ly), uz = minimg(q.z - pzk, I.Lz);
+          keep = ux * ux + uy * uy + uz * uz <= lim2;
+        }
       }
       const uint64_t mk = __ballot(keep);
-      wave_lds_sync();  // the previous staging is no longer read
-      if (keep) {
-        const int slot = __popcll(mk & ((1ull << lane) - 1));
-        sp[slot] = I.spos[j];
+      const int slot = base + __popcll(mk & ((1ull << lane) - 1));
+      if (keep && slot < kWave) {
+        sp[slot] = q;
         sa[slot] = am;
         if (MODE == 1) sd[slot] = I.df[am.w];
       }
-      wave_lds_sync();
       return (int)__popcll(mk);
     };
-    const bool one_chunk = C2.total <= kWave;
-    const int n_one = one_chunk ? stage(0) : 0;
+    // every kept candidate in one staging when they fit (one_chunk) ...
+    int n_kept = 0;
+    for (int32_t t0 = 0; t0 < C2.total && n_kept <= kWave; t0 += kWave) n_kept += stage_chunk(t0, n_kept);
+    wave_lds_sync();
+    const bool one_chunk = n_kept <= kWave;
+    const int n_one = one_chunk ? n_kept : 0;
+    // ... else chunk by chunk of the flattened list, restaged from slot 0
+    auto stage = [&](int32_t t0) {
+      wave_lds_sync();  // the previous staging is no longer read
+      const int n = stage_chunk(t0, 0);
+      wave_lds_sync();
+      return n;
+    };
+    const int32_t scan_total = one_chunk ? 1 : C2.total;
     for (int w0 = 0; w0 < I.n_wide; w0 += kItlChunk) {
       // phase 1: every lane tests its entries of the chunk, the record loads
       // (position and type) of all of them in flight together; the entries
@@ -571,7 +592,7 @@ __global__ __launch_bounds__(kBlock) void k_itl_side(Geom g, ItlDev I, const dou
           if (MODE == 1) dfi = DF[src];
         }
         const double rho_lat = rho_i;
-        for (int32_t t0 = 0; t0 < C2.total; t0 += kWave) {
+        for (int32_t t0 = 0; t0 < scan_total; t0 += kWave) {
           const int n_chunk = one_chunk ? n_one : stage(t0);
           for (int cc = 0; own && cc < n_chunk; cc++) {
             const int4 am = sa[cc];
