@@ -212,6 +212,16 @@ def ptr(a, ct=C.c_double):
     return a.ctypes.data_as(C.POINTER(ct))
 
 
+def _energy_rows(energies, nsteps):
+    """The MD runs write {potential, kinetic} into row `step` of a [nsteps][2]
+    float64 array: refuse a smaller or other array before the library writes."""
+    if energies is None:
+        return
+    if energies.dtype != np.float64 or not energies.flags["C_CONTIGUOUS"] or energies.size < 2 * max(nsteps, 0):
+        raise ValueError(f"energies must be a C-contiguous float64 array of at least [{nsteps}][2], "
+                         f"got {energies.dtype} {energies.shape}")
+
+
 # ---------------------------------------------------------------- host helpers
 
 def stencil_build(a, radius, parity, cap=4096):
@@ -457,9 +467,11 @@ class Context:
         _check(lib().mmd_md_prime(self.h))
 
     def md_run(self, nsteps, energies=None, use_graph=False):
+        _energy_rows(energies, nsteps)
         _check(lib().mmd_md_run(self.h, nsteps, ptr(energies), 1 if use_graph else 0))
 
     def md_run_noalloc(self, nsteps, energies=None):
+        _energy_rows(energies, nsteps)
         _check(lib().mmd_md_run_noalloc(self.h, nsteps, ptr(energies)))
 
     def energy(self):

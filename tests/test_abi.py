@@ -5,6 +5,9 @@ import os
 import re
 import subprocess
 
+import numpy as np
+import pytest
+
 import mmd
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -57,3 +60,14 @@ def test_errors_are_status_codes_not_exceptions():
     # device count on a host without a GPU is 0 or an error status, never a crash
     rc = L.mmd_ctx_create(99, C.byref(h))
     assert rc in (mmd.E_INVALID, mmd.E_HIP)
+
+
+def test_md_run_refuses_short_energy_arrays():
+    """The MD runs write a [nsteps][2] energy array row by step: a smaller one
+    is refused on the host before any library call (no context needed)."""
+    ctx = mmd.Context.__new__(mmd.Context)
+    for bad in (np.zeros((3, 2)), np.zeros((6, 2), np.float32), np.zeros((2, 6)).T):
+        with pytest.raises(ValueError):
+            ctx.md_run(6, bad)
+        with pytest.raises(ValueError):
+            ctx.md_run_noalloc(6, bad)

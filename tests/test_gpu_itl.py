@@ -146,6 +146,39 @@ def test_interstitial_graph_equals_eager_and_energy_conserved():
     assert np.ptp(ea[:, 1]) > 10 * np.ptp(tot)
 
 
+def test_interstitial_graph_without_prime_and_after_new_box():
+    """The interstitial bins are sized by mmd_itl_set and mmd_md_setup, never
+    inside a captured step: a graph-replayed run straight after set-up (no
+    mmd_md_prime), and again on one context after a new box of another size,
+    equals the eager run bit for bit."""
+    cases = [W.case(**ITL_CASES["blobs_12"]), W.case(**ITL_CASES["bcc10_3type"])]
+    outs = {}
+    for use_graph in (False, True):
+        ctx = mmd.Context(0)
+        res = []
+        for c in cases:
+            ctx.set_domain(c["dom"])
+            ctx.set_stencil(c["even"], c["odd"])
+            ctx.set_tables(c["tab"], c["masses"])
+            ctx.upload(c["st"], mmd.F_X | mmd.F_V | mmd.F_TYPE)
+            ctx.itl_set(c["itl"], c["wide3"])
+            ctx.md_setup(c["box"], A, 0.5, RC, energy_every=2)
+            en = np.zeros((6, 2))  # [nsteps][2]: rows of the recorded steps are written
+            ctx.md_run(6, en, use_graph=use_graph)
+            o = c["st"].copy()
+            ctx.download(o)
+            res.append((o, ctx.itl_get(c["itl"].n), en))
+        ctx.close()
+        outs[use_graph] = res
+    for (a, ia, ea), (b, ib, eb) in zip(outs[False], outs[True]):
+        for f in ("x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "rho", "df"):
+            assert np.array_equal(getattr(a, f), getattr(b, f)), f
+        for f in ("x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz"):
+            assert np.array_equal(getattr(ia, f), getattr(ib, f)), f
+        assert np.array_equal(ea, eb)
+        assert np.isfinite(ea).all() and (ea[1::2] != 0).all()
+
+
 def test_interstitials_mixed_precision_within_target():
     c = W.case(**ITL_CASES["bcc10_3type"])
     res = {}
