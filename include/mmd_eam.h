@@ -411,6 +411,15 @@ int mmd_fs_tables(const mmd_fs_params *p, double *rho_r, double *emb, double *ph
 int mmd_setfl_info(const char *path, int32_t *n_types, int32_t *n_r, int32_t *n_rho, double *cutoff);
 int mmd_setfl_read(const char *path, double *rho_r, double *emb, double *phi_r, double *mass, double *r0, double *dr,
                    double *rho0, double *drho);
+/* EAM tables from DYNAMO funcfl files (LAMMPS pair_style eam), one file per
+ * element type, the other on-disk source of those tables: F(rho), rho(r) and
+ * the effective charge Z(r), with phi_ij = 27.2 * 0.529 * Z_i(r) Z_j(r) / r
+ * (eV, angstrom).  The files must share one grid (mmd_funcfl_info checks it
+ * and gives the sizes); mmd_funcfl_read fills the arrays mmd_setfl_read does,
+ * n_types = n_files. */
+int mmd_funcfl_info(const char *const *paths, int32_t n_files, int32_t *n_r, int32_t *n_rho, double *cutoff);
+int mmd_funcfl_read(const char *const *paths, int32_t n_files, double *rho_r, double *emb, double *phi_r,
+                    double *mass, double *r0, double *dr, double *rho0, double *drho);
 
 /* Initial conditions for a periodic single-domain BCC box of cells[3] unit
  * cells: types drawn from `type_frac` (cumulative, n_types entries), Gaussian

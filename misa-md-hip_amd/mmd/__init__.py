@@ -179,6 +179,8 @@ def lib():
                                        _d, _d, _d, _d, _d, _d, _i8]),
         "mmd_setfl_info": (C.c_int, [C.c_char_p, _i32, _i32, _i32, _d]),
         "mmd_setfl_read": (C.c_int, [C.c_char_p, _d, _d, _d, _d, _d, _d, _d, _d]),
+        "mmd_funcfl_info": (C.c_int, [C.POINTER(C.c_char_p), C.c_int32, _i32, _i32, _d]),
+        "mmd_funcfl_read": (C.c_int, [C.POINTER(C.c_char_p), C.c_int32, _d, _d, _d, _d, _d, _d, _d, _d]),
         "mmd_server_create": (C.c_int, [C.POINTER(ServerOpts), C.POINTER(EamTables), C.POINTER(P)]),
         "mmd_server_destroy": (C.c_int, [P]),
         "mmd_server_run": (C.c_int, [P, C.POINTER(ReplicaSpec), C.c_int32, C.POINTER(ReplicaResult),
@@ -276,6 +278,23 @@ def setfl_read(path):
     mass = np.zeros(nt)
     vals = [C.c_double() for _ in range(4)]
     _check(lib().mmd_setfl_read(p, ptr(rho_r), ptr(emb), ptr(phi), ptr(mass), *[C.byref(v) for v in vals]))
+    tab = dict(n_types=nt, n_r=nr, r0=vals[0].value, dr=vals[1].value, n_rho=nrho, rho0=vals[2].value,
+               drho=vals[3].value, rho_r=rho_r, emb=emb, phi_r=phi, rc=rc.value)
+    return tab, tuple(mass)
+
+
+def funcfl_read(paths):
+    """EAM tables (and masses) from DYNAMO funcfl files, one per element type,
+    in the form synth_tables returns."""
+    paths = [str(p).encode() for p in paths]
+    arr = (C.c_char_p * len(paths))(*paths)
+    nr, nrho, rc = C.c_int32(), C.c_int32(), C.c_double()
+    _check(lib().mmd_funcfl_info(arr, len(paths), C.byref(nr), C.byref(nrho), C.byref(rc)))
+    nt, nr, nrho = len(paths), nr.value, nrho.value
+    rho_r, emb, phi = np.zeros((nt, nr)), np.zeros((nt, nrho)), np.zeros((nt, nt, nr))
+    mass = np.zeros(nt)
+    vals = [C.c_double() for _ in range(4)]
+    _check(lib().mmd_funcfl_read(arr, nt, ptr(rho_r), ptr(emb), ptr(phi), ptr(mass), *[C.byref(v) for v in vals]))
     tab = dict(n_types=nt, n_r=nr, r0=vals[0].value, dr=vals[1].value, n_rho=nrho, rho0=vals[2].value,
                drho=vals[3].value, rho_r=rho_r, emb=emb, phi_r=phi, rc=rc.value)
     return tab, tuple(mass)

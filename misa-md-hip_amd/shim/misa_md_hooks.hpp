@@ -114,12 +114,39 @@ struct HostTables {
     check(mmd_setfl_read(path, rho_r.data(), emb.data(), phi_r.data(), mass.data(), &r0, &dr, &rho0, &drho),
           "mmd_setfl_read");
   }
+  // list = funcfl files separated by ':', one per element type in type order
+  void load_funcfl(const char *list) {
+    std::vector<std::string> files;
+    std::string cur;
+    for (const char *p = list;; p++) {
+      if (*p == ':' || *p == '\0') {
+        if (!cur.empty()) files.push_back(cur);
+        cur.clear();
+        if (*p == '\0') break;
+      } else {
+        cur.push_back(*p);
+      }
+    }
+    std::vector<const char *> paths;
+    for (const std::string &f : files) paths.push_back(f.c_str());
+    double rc = 0.0;
+    check(mmd_funcfl_info(paths.data(), (int32_t)paths.size(), &n_r, &n_rho, &rc), "mmd_funcfl_info");
+    n_types = (int32_t)paths.size();
+    rho_r.assign((size_t)n_types * n_r, 0.0);
+    emb.assign((size_t)n_types * n_rho, 0.0);
+    phi_r.assign((size_t)n_types * n_types * n_r, 0.0);
+    mass.assign((size_t)n_types, 0.0);
+    check(mmd_funcfl_read(paths.data(), n_types, rho_r.data(), emb.data(), phi_r.data(), mass.data(), &r0, &dr, &rho0,
+                          &drho),
+          "mmd_funcfl_read");
+  }
 };
 
 // Knot tables of MISA-MD's eam object.  Its internals belong to the core,
 // which is not part of the reference tree, so the extraction is a
 // specialisation point: EamTablesOf<eam>::fill(const eam &, HostTables &).
-// Without one, hip_pot_init reads the setfl file named by MMD_SETFL.
+// Without one, hip_pot_init reads the setfl file named by MMD_SETFL, or the
+// funcfl files (one per element type, ':'-separated) named by MMD_FUNCFL.
 template <class Eam, class = void>
 struct EamTablesOf {
   static constexpr bool available = false;
@@ -131,11 +158,16 @@ void pot_tables(const Eam *pot, HostTables &t) {
     t.load_setfl(path);
     return;
   }
+  if (const char *list = std::getenv("MMD_FUNCFL")) {
+    t.load_funcfl(list);
+    return;
+  }
   if constexpr (EamTablesOf<Eam>::available) {
     EamTablesOf<Eam>::fill(*pot, t);
   } else {
     (void)pot;
-    std::fprintf(stderr, "mmd: hip_pot_init needs MMD_SETFL=<setfl file> or an EamTablesOf<eam> specialisation\n");
+    std::fprintf(stderr, "mmd: hip_pot_init needs MMD_SETFL=<setfl file>, MMD_FUNCFL=<funcfl files> or an "
+                         "EamTablesOf<eam> specialisation\n");
     std::abort();
   }
 }

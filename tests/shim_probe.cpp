@@ -42,5 +42,12 @@ int main(int argc, char **argv) {
     const mmd_eam_tables v = t.view();
     std::printf("tables %d %d %d %.17g %.17g %.17g\n", v.n_types, v.n_r, v.n_rho, v.dr, v.drho, v.phi_r[v.n_r + 7]);
   }
+  if (argc > 2) {  // ':'-separated funcfl files, as MMD_FUNCFL names them
+    mmd_shim::HostTables t;
+    t.load_funcfl(argv[2]);
+    const mmd_eam_tables v = t.view();
+    std::printf("funcfl %d %d %d %.17g %.17g %.17g\n", v.n_types, v.n_r, v.n_rho, v.dr, v.phi_r[v.n_r + 7],
+                v.mass[v.n_types - 1]);
+  }
   return 0;
 }

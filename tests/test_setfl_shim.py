@@ -52,6 +52,66 @@ def write_setfl(path, nt=2, nr=600, dr=0.01, nrho=500, drho=0.05, params=None):
     return vals, r
 
 
+HARTREE_BOHR = 27.2 * 0.529  # r phi_ij = 27.2 * 0.529 * Z_i Z_j in eV * angstrom (LAMMPS pair_style eam)
+
+
+def write_funcfl(path, z_scale=1.0, nr=600, dr=0.01, nrho=500, drho=0.05, C=0.01, mass=55.845):
+    """One-element funcfl file: F(rho) = -sqrt(rho) + C rho^2, Z(r) = z_scale
+    (rc - r)^2 (1 + r), rho(r) = (rc - r)^2 exp(-r), cut at rc."""
+    r = np.arange(nr) * dr
+    rho = np.arange(nrho) * drho
+    F = -np.sqrt(rho) + C * rho ** 2
+    Z = np.where(r < RC, z_scale * (RC - r) ** 2 * (1.0 + r), 0.0)
+    rr = np.where(r < RC, (RC - r) ** 2 * np.exp(-r), 0.0)
+    lines = ["synthetic funcfl", f"26 {mass!r} 2.855 BCC", f"{nrho} {drho!r} {nr} {dr!r} {RC!r}"]
+    for a in (F, Z, rr):
+        for k in range(0, len(a), 5):
+            lines.append(" ".join("%.17g" % v for v in a[k:k + 5]))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return dict(F=F, Z=Z, rho=rr)
+
+
+def test_funcfl_read_values(tmp_path):
+    """One file: F and rho as written, phi = 27.2 * 0.529 * Z^2 / r; two files:
+    the cross term from the product of the two charges."""
+    a, b = tmp_path / "a.funcfl", tmp_path / "b.funcfl"
+    va = write_funcfl(a)
+    vb = write_funcfl(b, z_scale=1.3, C=0.02, mass=58.693)
+    rk = np.arange(1, 600) * 0.01
+    tab, mass = mmd.funcfl_read([a])
+    assert (tab["n_types"], tab["n_r"], tab["n_rho"]) == (1, 600, 500) and mass == (55.845,)
+    assert tab["r0"] == 0.0 and tab["dr"] == 0.01 and tab["drho"] == 0.05 and tab["rc"] == RC
+    assert np.array_equal(tab["emb"][0], va["F"]) and np.array_equal(tab["rho_r"][0], va["rho"])
+    phi = tab["phi_r"][0, 0]
+    assert np.array_equal(phi[1:], HARTREE_BOHR * va["Z"][1:] * va["Z"][1:] / rk)
+    assert phi[0] == 2 * phi[1] - phi[2]
+    tab, mass = mmd.funcfl_read([a, b])
+    assert tab["n_types"] == 2 and mass == (55.845, 58.693)
+    assert np.array_equal(tab["emb"][1], vb["F"]) and np.array_equal(tab["rho_r"][1], vb["rho"])
+    for i, zi in enumerate((va["Z"], vb["Z"])):
+        for j, zj in enumerate((va["Z"], vb["Z"])):
+            z2r = HARTREE_BOHR * zi * zj if i >= j else HARTREE_BOHR * zj * zi
+            assert np.array_equal(tab["phi_r"][i, j][1:], z2r[1:] / rk), (i, j)
+    # a physical check of the conversion: at r = 1 A a unit charge product
+    # gives 27.2 * 0.529 = 14.39 eV, the Coulomb energy of two elementary
+    # charges one angstrom apart (14.40 eV) to the constants' rounding
+    assert abs(HARTREE_BOHR - 14.39964) < 0.02
+
+
+def test_funcfl_errors(tmp_path):
+    a, b = tmp_path / "a.funcfl", tmp_path / "b.funcfl"
+    write_funcfl(a)
+    write_funcfl(b, dr=0.011)  # another grid
+    txt = a.read_text().splitlines()
+    (tmp_path / "trunc.funcfl").write_text("\n".join(txt[: len(txt) // 2]) + "\n")
+    (tmp_path / "grid.funcfl").write_text("\n".join(txt[:2] + ["500 0.05 600 0 5.6"] + txt[3:]) + "\n")
+    for files in ([a, b], [tmp_path / "trunc.funcfl"], [tmp_path / "grid.funcfl"], [tmp_path / "missing.funcfl"]):
+        with pytest.raises(mmd.MmdError) as e:
+            mmd.funcfl_read(files)
+        assert e.value.code == mmd.E_INVALID
+
+
 def test_setfl_read_values(tmp_path):
     p = tmp_path / "a.setfl"
     vals, r = write_setfl(p)
@@ -92,7 +152,11 @@ def test_hook_adapter_conversions(tmp_path):
                    check=True)
     p = tmp_path / "a.setfl"
     write_setfl(p)
-    out = subprocess.run([str(exe), str(p)], check=True, capture_output=True, text=True).stdout.splitlines()
+    fa, fb = tmp_path / "a.funcfl", tmp_path / "b.funcfl"
+    write_funcfl(fa)
+    write_funcfl(fb, z_scale=1.3, mass=58.693)
+    out = subprocess.run([str(exe), str(p), f"{fa}:{fb}"], check=True, capture_output=True,
+                         text=True).stdout.splitlines()
     # ghost = local low - ghost low; box, ext as given; start = local low
     assert out[0] == "domain 6 3 3 | 100 50 60 | 112 56 66 | 40 10 20"
     assert out[1] == "stencil 4 2 -5000 -2"
@@ -102,3 +166,8 @@ def test_hook_adapter_conversions(tmp_path):
     f = out[3].split()
     assert f[:4] == ["tables", "2", "600", "500"]
     assert float(f[6]) == tab["phi_r"].reshape(-1)[600 + 7]
+    f = out[4].split()
+    assert f[:4] == ["funcfl", "2", "600", "500"]
+    tab2, _ = mmd.funcfl_read([fa, fb])
+    assert float(f[5]) == tab2["phi_r"].reshape(-1)[600 + 7]
+    assert float(f[6]) == 58.693
