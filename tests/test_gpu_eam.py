@@ -78,6 +78,35 @@ def test_rho_df_force_fp64_match_oracle(name):
     assert np.all(out.rho[vac] == 0) and np.all(out.fx[vac] == 0)  # vacancies untouched
 
 
+def test_funcfl_tables_match_oracle(tmp_path):
+    """Tables read from two funcfl files (mmd_funcfl_read: F, rho and the
+    effective charges, phi_ij = 27.2 * 0.529 * Z_i Z_j / r) drive the device
+    passes on a 2-type box with vacancies, to the oracle bar."""
+    fa, fb = tmp_path / "a.funcfl", tmp_path / "b.funcfl"
+    W.write_funcfl(fa)
+    W.write_funcfl(fb, z_scale=1.3, C=0.02, mass=58.693)
+    tab, mass = mmd.funcfl_read([fa, fb])
+    c = W.case(cells=(7, 6, 5), n_types=2, sigma=0.05, temperature=900.0, vacancy_frac=0.02, seed=11)
+    c["tab"], c["masses"] = tab, mass
+    ref, _ = _oracle_pass(c)
+    with _ctx(c) as ctx:
+        ctx.rho_df(RC)
+        ctx.md_setup(c["box"], A, 1.0, RC)
+        ctx.ghost_refresh(mmd.F_DF)
+        ctx.force(RC)
+        out = c["st"].copy()
+        ctx.download(out, mmd.F_RHO | mmd.F_DF | mmd.F_F)
+    li = out.local_index()
+    occ = li[out.type[li] >= 0]
+    assert set(np.unique(out.type[occ])) == {0, 1}
+    assert np.allclose(out.rho[occ], ref.rho[occ], rtol=1e-12, atol=0)
+    assert np.allclose(out.df[occ], ref.df[occ], rtol=1e-12, atol=1e-15)
+    F = np.stack([out.fx[occ], out.fy[occ], out.fz[occ]], 1)
+    Fr = np.stack([ref.fx[occ], ref.fy[occ], ref.fz[occ]], 1)
+    assert np.abs(Fr).max() > 0.1
+    assert np.abs(F - Fr).max() <= 1e-11 * np.abs(Fr).max()
+
+
 @pytest.mark.parametrize("prec", [mmd.PREC_FP64, mmd.PREC_MIXED])
 def test_wide_stencil_large_lds_tile(prec):
     """A 7.5 A stencil (168 entries, reach 3 cells each way) needs a 92 KB

@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import mmd
+from workload import HARTREE_BOHR, write_funcfl
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 RC = 5.6
@@ -50,26 +51,6 @@ def write_setfl(path, nt=2, nr=600, dr=0.01, nrho=500, drho=0.05, params=None):
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
     return vals, r
-
-
-HARTREE_BOHR = 27.2 * 0.529  # r phi_ij = 27.2 * 0.529 * Z_i Z_j in eV * angstrom (LAMMPS pair_style eam)
-
-
-def write_funcfl(path, z_scale=1.0, nr=600, dr=0.01, nrho=500, drho=0.05, C=0.01, mass=55.845):
-    """One-element funcfl file: F(rho) = -sqrt(rho) + C rho^2, Z(r) = z_scale
-    (rc - r)^2 (1 + r), rho(r) = (rc - r)^2 exp(-r), cut at rc."""
-    r = np.arange(nr) * dr
-    rho = np.arange(nrho) * drho
-    F = -np.sqrt(rho) + C * rho ** 2
-    Z = np.where(r < RC, z_scale * (RC - r) ** 2 * (1.0 + r), 0.0)
-    rr = np.where(r < RC, (RC - r) ** 2 * np.exp(-r), 0.0)
-    lines = ["synthetic funcfl", f"26 {mass!r} 2.855 BCC", f"{nrho} {drho!r} {nr} {dr!r} {RC!r}"]
-    for a in (F, Z, rr):
-        for k in range(0, len(a), 5):
-            lines.append(" ".join("%.17g" % v for v in a[k:k + 5]))
-    with open(path, "w") as f:
-        f.write("\n".join(lines) + "\n")
-    return dict(F=F, Z=Z, rho=rr)
 
 
 def test_funcfl_read_values(tmp_path):
