@@ -51,8 +51,10 @@ hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, int blocks, co
 hipError_t launch_batch_kick_fused(hipStream_t st, const BatchLists &L, int blocks, const Masses &M, double h,
                                    double dt, bool want_ke);
 hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, int blocks, uint32_t fields);
+// lean (mixed, MD steps): rho and df from the quadratic records; with want_e the embedding energy
+// from a second sum over the cubic value records
 hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, int blocks, const Halo &h, const TabF64 &tf,
-                               const TabMixed &tm, double rc2, int prec, bool want_e);
+                               const TabMixed &tm, double rc2, int prec, bool want_e, bool lean);
 hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, int blocks, const Halo &h, const TabF64 &tf,
                               const TabMixed &tm, double rc2, int prec, bool want_e);
 hipError_t batch_kernels_allow_lds(size_t bytes);

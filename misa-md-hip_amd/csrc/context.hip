@@ -261,12 +261,12 @@ static int timing_collect(mmd_ctx *c) {
 
 // ---------------------------------------------------------- pass helpers
 
-int run_rho_df(mmd_ctx *c, double cutoff, bool want_e) {
+int run_rho_df(mmd_ctx *c, double cutoff, bool want_e, bool lean) {
   hipEvent_t ev;
   int rc = timed_begin(c, MMD_K_RHO, &ev);
   if (rc) return rc;
   HIP_TRY(launch_rho_df(c->stream, c->g, c->halo, c->d_off, c->nk_even, c->nk_odd, c->pos, c->rho, c->df, c->tf, c->tm,
-                        cutoff * cutoff, c->prec, want_e ? c->e_rho : nullptr));
+                        cutoff * cutoff, c->prec, want_e ? c->e_rho : nullptr, lean));
   rc = timed_end(c, MMD_K_RHO, ev);
   if (rc) return rc;
   if (c->n_itl > 0) return itl_rho_df(c, cutoff, want_e);
@@ -347,7 +347,10 @@ static int md_step(mmd_ctx *c, bool fused, bool prev_e, bool want_e) {
     return rc;
   }
   if ((rc = run_ghost(c, MMD_F_X))) return rc;
-  if ((rc = run_rho_df(c, c->md.cutoff, want_e))) return rc;
+  // MD dynamics take rho and df from the lean records on every step (so a
+  // trajectory does not depend on which steps record energies); energy steps
+  // also sum the cubic records for the embedding energy
+  if ((rc = run_rho_df(c, c->md.cutoff, want_e, true))) return rc;
   if ((rc = run_ghost(c, MMD_F_DF))) return rc;
   return run_force(c, c->md.cutoff, want_e, true);
 }
@@ -684,6 +687,11 @@ struct MixHost {
 constexpr double kMixedTolValue = 1e-9;
 constexpr double kMixedTolForce = 2e-8;
 constexpr double kMixedFloor = 3e-2;
+// The lean rho records (passes that compute no energy) meet 2e-8 on rho: rho
+// reaches those passes' forces only through df = F'(rho), whose error is
+// F''(rho) times rho's, orders below the force bound; energies, which take
+// F(rho) itself, are computed from the 1e-9 value records.
+constexpr double kMixedTolLean = 2e-8;
 
 // Where the mixed tables are used: the segments from the first to the last
 // good one, minus at most kMixedHoles interior runs of failing segments (runs
@@ -814,6 +822,50 @@ static MixHost pack_mixed(const double *knots, int nfun, int n_r, double r0, dou
   return o;
 }
 
+// Lean rho records: per segment the quadratic through the cubic value record
+// c(q) = base + a1 q + a2 q^2 + a3 q^3 at q = 0, 1/2 and 1, i.e. base +
+// (a1 - a3/2) q + (a2 + 3 a3/2) q^2, whose distance from the cubic is
+// |a3 q (q - 1/2) (q - 1)| <= 0.048 |a3|.  Checked densely against the FP64
+// spline with the device's own FP32 steps; a segment the cubic does not serve
+// or that misses kMixedTolLean gets a NaN quadratic term (the FP64 spline
+// then serves its pairs).  n_miss counts served segments that missed.
+static std::vector<DerivRec> pack_lean_rho(const MixHost &mh, const double *knots, int nt, int n_r, double r0,
+                                           double dr, int n_s, double s0, double ds, const std::vector<char> &use,
+                                           int64_t &n_miss) {
+  std::vector<DerivRec> out((size_t)nt * n_s);
+  std::vector<double> c7((size_t)n_r * 7);
+  const double inv_dr = 1.0 / dr;
+  const float nan = std::numeric_limits<float>::quiet_NaN();
+  n_miss = 0;
+  for (int f = 0; f < nt; f++) {
+    spline7(knots + (size_t)f * n_r, n_r, dr, c7.data());
+    double hmax = 0.0;
+    for (int i = 0; i < n_s; i++)
+      hmax = std::max(hmax, std::fabs(spline7_value(c7.data(), n_r, r0, inv_dr, std::sqrt(s0 + ds * (double)i))));
+    for (int m = 0; m < n_s; m++) {
+      const size_t r = (size_t)f * n_s + m;
+      DerivRec &o = out[r];
+      o.base = mh.vb[2 * r];
+      bool good = use[(size_t)m] != 0;
+      if (good) {
+        const double a1 = mh.vb[2 * r + 1], a2 = mh.vc[2 * r], a3 = mh.vc[2 * r + 1];
+        o.b1 = (float)(a1 - 0.5 * a3);
+        o.b2 = (float)(a2 + 1.5 * a3);
+        for (int k = 1; k < 8 && good; k++) {
+          const double q = (double)k / 8.0;
+          const double h_ex = spline7_value(c7.data(), n_r, r0, inv_dr, std::sqrt(s0 + ds * ((double)m + q)));
+          const float c = std::fma((float)q, o.b2, o.b1);
+          const double h = std::fma(q, (double)c, o.base);
+          good = std::fabs(h - h_ex) <= kMixedTolLean * (std::fabs(h_ex) + kMixedFloor * hmax);
+        }
+        if (!good) n_miss++;
+      }
+      if (!good) o.b1 = 0.0f, o.b2 = nan;
+    }
+  }
+  return out;
+}
+
 // FP64 force records per type pair (a, b): derivative coefficients of rho_b
 // and phi_ab per knot, three double2 (see TabF64::fc).
 static size_t pack_force_f64(Arena &A, const double *rho_r, const double *phi_r, int nt, int n, double dx) {
@@ -887,6 +939,9 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
       return std::array<size_t, 3>{vb, vc, dp};
     };
     const std::array<size_t, 3> o_rho = put_mixed(m_rho), o_phi = put_mixed(m_phi);
+    int64_t lean_miss = 0;
+    const std::vector<DerivRec> lean = pack_lean_rho(m_rho, t->rho_r, nt, nr, t->r0, t->dr, n_s, s0, ds, use, lean_miss);
+    const size_t h_lean = A.put(lean.data(), lean.size() * sizeof(DerivRec));
     const size_t h_fc = pack_force_f64(A, t->rho_r, t->phi_r, nt, nr, t->dr);
     // mixed force records per type pair (a, b): rho_b and phi_ab derivative records side by side
     std::vector<DerivRec> fb((size_t)nt * nt * n_s * 2);
@@ -931,6 +986,7 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
     tm = tm_new;
     tm.rho = M(o_rho), tm.phi = M(o_phi);
     tm.fb = reinterpret_cast<const double2 *>(dev + h_fb);
+    tm.rho_lean = reinterpret_cast<const double2 *>(dev + h_lean);
     tm.nt = nt, tm.n_s = n_s, tm.s0 = s0, tm.inv_ds = 1.0 / ds;
     for (int a = 0; a < kMaxTypes; a++) {
       c->masses.m[a] = a < nt ? t->mass[a] : 1.0;
@@ -1085,7 +1141,7 @@ extern "C" int mmd_eam_rho_df(mmd_ctx *c, double cutoff) {
     clear_error();
     int rc = ctx_bind(c);
     if (rc || (rc = check_ready(c, "mmd_eam_rho_df")) || (rc = check_cutoff(c, cutoff, "mmd_eam_rho_df"))) return rc;
-    return run_rho_df(c, cutoff, false);
+    return run_rho_df(c, cutoff, false, false);
   })
 }
 
@@ -1178,7 +1234,7 @@ static int aos_pass(mmd_ctx *c, void *atoms, const mmd_aos_layout *lay, uint32_t
     c->max_type = std::min(mx, c->n_types - 1);
     if (mx >= c->n_types) return fail(MMD_E_INVALID, "atom type %d has no table (n_types = %d)", mx, c->n_types);
   }
-  if (pass == 0) rc = run_rho_df(c, cutoff, false);
+  if (pass == 0) rc = run_rho_df(c, cutoff, false, false);
   if (pass == 1) {
     HIP_TRY(launch_df(c->stream, c->g, c->type, c->rho, c->df, c->tf));
   }
@@ -1302,7 +1358,7 @@ extern "C" int mmd_md_prime(mmd_ctx *c) {
     if (rc) return rc;
     if (!c->md_ready) return fail(MMD_E_STATE, "mmd_md_prime: call mmd_md_setup first");
     if ((rc = check_ready(c, "mmd_md_prime")) || (rc = check_cutoff(c, c->md.cutoff, "mmd_md_prime"))) return rc;
-    if ((rc = run_ghost(c, MMD_F_X)) || (rc = run_rho_df(c, c->md.cutoff, false)) || (rc = run_ghost(c, MMD_F_DF)) ||
+    if ((rc = run_ghost(c, MMD_F_X)) || (rc = run_rho_df(c, c->md.cutoff, false, true)) || (rc = run_ghost(c, MMD_F_DF)) ||
         (rc = run_force(c, c->md.cutoff, false, true)))
       return rc;
     return MMD_OK;
@@ -1418,7 +1474,7 @@ extern "C" int mmd_energy(mmd_ctx *c, double *epot, double *ekin) {
     if ((rc = check_ready(c, "mmd_energy")) || (rc = check_cutoff(c, c->md.cutoff, "mmd_energy"))) return rc;
     if ((rc = ensure_energy_cap(c, 1))) return rc;
     HIP_TRY(launch_fill_i32(c->stream, c->d_counter, 1, 0));
-    if ((rc = run_ghost(c, MMD_F_X)) || (rc = run_rho_df(c, c->md.cutoff, true)) || (rc = run_ghost(c, MMD_F_DF)) ||
+    if ((rc = run_ghost(c, MMD_F_X)) || (rc = run_rho_df(c, c->md.cutoff, true, true)) || (rc = run_ghost(c, MMD_F_DF)) ||
         (rc = run_force(c, c->md.cutoff, true, true)) || (rc = run_kick(c, 0.0, false, true)) ||
         (rc = run_finalize(c, c->d_energy, c->d_counter)))
       return rc;

@@ -116,7 +116,7 @@ int ctx_bind(mmd_ctx *c);
 void ctx_drop_graphs(mmd_ctx *c);
 int timed_begin(mmd_ctx *c, int cls, hipEvent_t *ev);
 int timed_end(mmd_ctx *c, int cls, hipEvent_t start);
-int run_rho_df(mmd_ctx *c, double cutoff, bool want_e);
+int run_rho_df(mmd_ctx *c, double cutoff, bool want_e, bool lean);
 // periodic: the caller is the periodic single-domain step (the ghost shell
 // holds the local sites' images), where the half stencil may be used
 int run_force(mmd_ctx *c, double cutoff, bool want_e, bool periodic = false);

@@ -145,6 +145,12 @@ struct TabMixed {
   // The force pass reads derivatives only from fb and rho.dp, both stored
   // times -2 (the pair force is -2 dpsi/ds along the displacement).
   const double2 *fb;  // [nt*nt][n_s][2]
+  // rho(s) for the passes that compute no energy: one 16-byte record per
+  // segment holding the quadratic through the cubic at q = 0, 1/2 and 1 (FP64
+  // base, FP32 linear and quadratic terms, evaluated like a derivative
+  // record), NaN-flagged where the cubic is and where the quadratic misses
+  // its own bound.
+  const double2 *rho_lean;  // [nt][n_s]
   int32_t nt, n_s;
   double s0, inv_ds;
   // Segments that miss the accuracy bounds evaluate the FP64 r-spline instead:

@@ -229,8 +229,9 @@ __device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, uint32_t ti
 }
 
 // Phased rho of a trip (mixed): rho_{tj}(s), NaN where the pair needs the
-// FP64 spline.
-template <int U>
+// FP64 spline.  LEAN: the one-load quadratic records (passes whose rho feeds
+// only df); else the cubic value records (two loads).
+template <int U, bool LEAN>
 __device__ __forceinline__ void trip_rho_mixed(const TabMixed &tm, const uint32_t *tj, const double *d2, double *v) {
   double q[U];
   uint32_t rec[U];
@@ -239,6 +240,14 @@ __device__ __forceinline__ void trip_rho_mixed(const TabMixed &tm, const uint32_
     const SegS sg = seg_s(d2[u], tm);
     q[u] = sg.q;
     rec[u] = __umul24(tj[u], (uint32_t)tm.n_s) + sg.m;
+  }
+  if constexpr (LEAN) {
+    double2 lr[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) lr[u] = ld(tm.rho_lean, rec[u]);
+#pragma unroll
+    for (int u = 0; u < U; u++) v[u] = mix_deriv_rec(lr[u], q[u], (float)q[u]);
+    return;
   }
   double2 vb[U];
   float2 vc[U];
@@ -324,7 +333,12 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const Halo &h, const T
   __syncthreads();
 }
 
-template <int PREC, bool WANT_E>
+// LEAN (mixed, the MD steps): rho and df from the one-load quadratic records
+// on every step, so a trajectory does not depend on which steps record
+// energies; a step that records them also sums rho from the cubic value
+// records, for the embedding energy alone.  The passes whose rho reaches the
+// caller use the cubic value records.
+template <int PREC, bool WANT_E, bool LEAN = false>
 __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64_t lb,
                                              const int32_t *__restrict__ off, int32_t nk_even, int32_t nk_odd,
                                              const double4 *__restrict__ P, double *__restrict__ RHO,
@@ -343,7 +357,8 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
     if (t0 >= 0) {
       const int32_t *o = off + t.s * kMaxStencil;
       const int nk = t.s ? nk_odd : nk_even;
-      double acc = 0.0;
+      constexpr bool BOTH = PREC == 1 && LEAN && WANT_E;  // dynamics from lean records, energy from cubic ones
+      double acc = 0.0, acc_e = 0.0;
       // vacancies sit at the far point, so the cutoff test screens them;
       // their type byte (0xff) is clamped to a valid table row
       const uint8_t *ltu = reinterpret_cast<const uint8_t *>(lt);
@@ -363,22 +378,29 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
           any |= in[u];
         }
         if (__ballot(any) == 0) return;
-        double v[kUnroll];
+        double v[kUnroll], ve[kUnroll];
         if constexpr (PREC == 1) {
-          trip_rho_mixed<kUnroll>(tm, tq, d2, v);
+          trip_rho_mixed<kUnroll, LEAN>(tm, tq, d2, v);
+          if constexpr (BOTH) trip_rho_mixed<kUnroll, false>(tm, tq, d2, ve);
           uint64_t redo = 0;  // lanes with a pair to redo (scalar masks)
 #pragma unroll
-          for (int u = 0; u < kUnroll; u++) redo |= __ballot(in[u] && __builtin_isnan(v[u]));
+          for (int u = 0; u < kUnroll; u++)
+            redo |= __ballot(in[u] && (__builtin_isnan(v[u]) || (BOTH && __builtin_isnan(ve[u]))));
           if (redo != 0) {
 #pragma unroll
-            for (int u = 0; u < kUnroll; u++)
+            for (int u = 0; u < kUnroll; u++) {
               if (in[u] && __builtin_isnan(v[u])) v[u] = rho_f64(tf, (int)tq[u], d2[u]);
+              if (BOTH && in[u] && __builtin_isnan(ve[u])) ve[u] = rho_f64(tf, (int)tq[u], d2[u]);
+            }
           }
         } else {
           trip_rho_f64<kUnroll>(tf, tq, d2, v);
         }
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) acc += in[u] ? v[u] : 0.0;
+        for (int u = 0; u < kUnroll; u++) {
+          acc += in[u] ? v[u] : 0.0;
+          if (BOTH) acc_e += in[u] ? ve[u] : 0.0;
+        }
       };
       const int nk4 = nk & ~(kUnroll - 1);
       for (int k = 0; k < nk4; k += kUnroll) trip(k, false);
@@ -388,7 +410,12 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
       const Seg se = seg_f64(acc, tf.rho0, tf.inv_drho, tf.n_rho);
       const uint32_t re = (uint32_t)(t0 * tf.n_rho + se.m);
       DF[i] = spl_deriv(tf.emb, re, se.p);
-      if (WANT_E) e_loc = spl_value(tf.emb, re, se.p);
+      if (BOTH) {
+        const Seg sv = seg_f64(acc_e, tf.rho0, tf.inv_drho, tf.n_rho);
+        e_loc = spl_value(tf.emb, (uint32_t)(t0 * tf.n_rho + sv.m), sv.p);
+      } else if (WANT_E) {
+        e_loc = spl_value(tf.emb, re, se.p);
+      }
     }
   }
   if (WANT_E) {

@@ -12,10 +12,12 @@ namespace mmd {
 int64_t site_blocks(const Geom &g);
 int64_t tile_blocks(const Geom &g);
 
-// EAM passes; off = the stencil's device image (stencil_image): LDS offsets within the tile, energy masks
+// EAM passes; off = the stencil's device image (stencil_image): LDS offsets within the tile, energy masks.
+// lean (mixed, MD steps): rho and df from the quadratic records; with e_part the embedding energy
+// from a second sum over the cubic value records.
 hipError_t launch_rho_df(hipStream_t st, const Geom &g, const Halo &h, const int32_t *off, int32_t nk_even,
                          int32_t nk_odd, const double4 *P, double *RHO, double *DF, const TabF64 &tf,
-                         const TabMixed &tm, double rc2, int prec, double *e_part);
+                         const TabMixed &tm, double rc2, int prec, double *e_part, bool lean);
 hipError_t launch_df(hipStream_t st, const Geom &g, const int8_t *T, const double *RHO, double *DF,
                      const TabF64 &tf);
 hipError_t launch_force(hipStream_t st, const Geom &g, const Halo &h, const int32_t *off, int32_t nk_even,

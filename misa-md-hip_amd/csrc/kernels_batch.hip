@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void kb_ghost(BatchLists L, uint32_t fields
   }
 }
 
-template <int PREC, bool WANT_E>
+template <int PREC, bool WANT_E, bool LEAN>
 __global__ __launch_bounds__(kBlock) void kb_rho_df(BatchLists L, Halo h, TabF64 tf, TabMixed tm, double rc2) {
   extern __shared__ double2 lds_tile[];
   __shared__ double red[kWavesPerBlock];
@@ -69,8 +69,8 @@ __global__ __launch_bounds__(kBlock) void kb_rho_df(BatchLists L, Halo h, TabF64
   for (int it = r.begin; it < r.end; it += r.stride) {
     const int2 w = L.tile_items[it];
     const BatchDom &d = L.dom[w.x];
-    rho_df_block<PREC, WANT_E>(d.g, h, w.y, d.off, d.nk_even, d.nk_odd, d.pos, d.rho, d.df, tf, tm, rc2, d.e_rho,
-                               lds_tile, red);
+    rho_df_block<PREC, WANT_E, LEAN>(d.g, h, w.y, d.off, d.nk_even, d.nk_odd, d.pos, d.rho, d.df, tf, tm, rc2,
+                                     d.e_rho, lds_tile, red);
   }
 }
 
@@ -147,19 +147,23 @@ hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, int blocks, u
 }
 
 hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, int blocks, const Halo &h, const TabF64 &tf,
-                               const TabMixed &tm, double rc2, int prec, bool want_e) {
+                               const TabMixed &tm, double rc2, int prec, bool want_e, bool lean) {
   const dim3 grid(blocks), block(kBlock);
   const size_t sh = tile_lds_bytes(h, false);
   if (prec == 0) {
     if (want_e)
-      kb_rho_df<0, true><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
+      kb_rho_df<0, true, false><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
     else
-      kb_rho_df<0, false><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
+      kb_rho_df<0, false, false><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
   } else {
-    if (want_e)
-      kb_rho_df<1, true><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
+    if (want_e && lean)
+      kb_rho_df<1, true, true><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
+    else if (want_e)
+      kb_rho_df<1, true, false><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
+    else if (lean)
+      kb_rho_df<1, false, true><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
     else
-      kb_rho_df<1, false><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
+      kb_rho_df<1, false, false><<<grid, block, sh, st>>>(L, h, tf, tm, rc2);
   }
   return hipGetLastError();
 }
@@ -183,8 +187,12 @@ hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, int blocks, c
 }
 
 hipError_t batch_kernels_allow_lds(size_t bytes) {
-  const void *fns[] = {reinterpret_cast<const void *>(kb_rho_df<0, true>), reinterpret_cast<const void *>(kb_rho_df<0, false>),
-                       reinterpret_cast<const void *>(kb_rho_df<1, true>), reinterpret_cast<const void *>(kb_rho_df<1, false>),
+  const void *fns[] = {reinterpret_cast<const void *>(kb_rho_df<0, true, false>),
+                       reinterpret_cast<const void *>(kb_rho_df<0, false, false>),
+                       reinterpret_cast<const void *>(kb_rho_df<1, true, false>),
+                       reinterpret_cast<const void *>(kb_rho_df<1, false, false>),
+                       reinterpret_cast<const void *>(kb_rho_df<1, false, true>),
+                       reinterpret_cast<const void *>(kb_rho_df<1, true, true>),
                        reinterpret_cast<const void *>(kb_force<0, true>), reinterpret_cast<const void *>(kb_force<0, false>),
                        reinterpret_cast<const void *>(kb_force<1, true>), reinterpret_cast<const void *>(kb_force<1, false>)};
   for (const void *f : fns) {

@@ -7,14 +7,15 @@
 
 namespace mmd {
 
-template <int PREC, bool WANT_E>
+template <int PREC, bool WANT_E, bool LEAN>
 __global__ __launch_bounds__(kBlock) void k_rho_df(Geom g, Halo h, const int32_t *__restrict__ off, int32_t nk_even,
                                                      int32_t nk_odd, const double4 *__restrict__ P,
                                                      double *__restrict__ RHO, double *__restrict__ DF, TabF64 tf,
                                                      TabMixed tm, double rc2, double *__restrict__ e_part) {
   extern __shared__ double2 lds_tile[];
   __shared__ double red[kWavesPerBlock];
-  rho_df_block<PREC, WANT_E>(g, h, xcd_block(), off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part, lds_tile, red);
+  rho_df_block<PREC, WANT_E, LEAN>(g, h, xcd_block(), off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part, lds_tile,
+                                   red);
 }
 
 template <int PREC, bool WANT_E>
@@ -52,19 +53,23 @@ int64_t tile_blocks(const Geom &g) { return (g.n_tiles + kXcds - 1) / kXcds * kX
 
 hipError_t launch_rho_df(hipStream_t st, const Geom &g, const Halo &h, const int32_t *off, int32_t nk_even,
                          int32_t nk_odd, const double4 *P, double *RHO, double *DF, const TabF64 &tf,
-                         const TabMixed &tm, double rc2, int prec, double *e_part) {
+                         const TabMixed &tm, double rc2, int prec, double *e_part, bool lean) {
   const dim3 grid((unsigned)tile_blocks(g)), block(kBlock);
   const size_t sh = tile_lds_bytes(h, false);
   if (prec == 0) {
     if (e_part)
-      k_rho_df<0, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<0, true, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
     else
-      k_rho_df<0, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<0, false, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
   } else {
-    if (e_part)
-      k_rho_df<1, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+    if (e_part && lean)
+      k_rho_df<1, true, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+    else if (e_part)
+      k_rho_df<1, true, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+    else if (lean)
+      k_rho_df<1, false, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
     else
-      k_rho_df<1, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<1, false, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
   }
   return hipGetLastError();
 }
@@ -96,8 +101,12 @@ hipError_t launch_force(hipStream_t st, const Geom &g, const Halo &h, const int3
 
 // Dynamic LDS beyond the default 64 KB must be enabled per kernel.
 hipError_t pair_kernels_allow_lds(size_t bytes) {
-  const void *fns[] = {reinterpret_cast<const void *>(k_rho_df<0, true>), reinterpret_cast<const void *>(k_rho_df<0, false>),
-                       reinterpret_cast<const void *>(k_rho_df<1, true>), reinterpret_cast<const void *>(k_rho_df<1, false>),
+  const void *fns[] = {reinterpret_cast<const void *>(k_rho_df<0, true, false>),
+                       reinterpret_cast<const void *>(k_rho_df<0, false, false>),
+                       reinterpret_cast<const void *>(k_rho_df<1, true, false>),
+                       reinterpret_cast<const void *>(k_rho_df<1, false, false>),
+                       reinterpret_cast<const void *>(k_rho_df<1, false, true>),
+                       reinterpret_cast<const void *>(k_rho_df<1, true, true>),
                        reinterpret_cast<const void *>(k_force<0, true>), reinterpret_cast<const void *>(k_force<0, false>),
                        reinterpret_cast<const void *>(k_force<1, true>), reinterpret_cast<const void *>(k_force<1, false>)};
   for (const void *f : fns) {

@@ -462,7 +462,7 @@ int step_passes(mmd_server *S, const BatchLists &L, const ListBuf &B, bool fused
   else
     HIP_TRY(launch_batch_kick(S->cs, L, B.g_site, M, h, dt, true, false));
   HIP_TRY(launch_batch_ghost(S->cs, L, B.g_ext, MMD_F_X));
-  HIP_TRY(launch_batch_rho_df(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e));
+  HIP_TRY(launch_batch_rho_df(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e, true));
   HIP_TRY(launch_batch_ghost(S->cs, L, B.g_ext, MMD_F_DF));
   HIP_TRY(launch_batch_force(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e));
   return MMD_OK;
@@ -540,7 +540,7 @@ struct GpuExec final : Executor {
     const int prec = S->o.precision;
     const ListBuf &B = S->prime;
     HIP_TRY(launch_batch_ghost(S->cs, L, B.g_ext, MMD_F_X));
-    HIP_TRY(launch_batch_rho_df(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, true));
+    HIP_TRY(launch_batch_rho_df(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, true, true));
     HIP_TRY(launch_batch_ghost(S->cs, L, B.g_ext, MMD_F_DF));
     HIP_TRY(launch_batch_force(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, true));
     HIP_TRY(launch_batch_kick(S->cs, L, B.g_site, S->tab->masses, 0.0, S->o.dt, false, true));
