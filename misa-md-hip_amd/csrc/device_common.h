@@ -20,6 +20,10 @@ namespace mmd {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
+// Force-pass kernels: their LDS tile fits three blocks on a CU (12 waves,
+// 3 per SIMD), so their registers are capped to keep 3 waves per SIMD
+// (at most 168 VGPRs).
+constexpr int kForceWavesPerSimd = 3;
 constexpr int kMaxStencil = 512;  // entries per parity
 // Device image of a stencil for the pair passes (stencil_image): [2][kMaxStencil]
 // LDS offsets, rows zero-padded; then [2][kMaxStencil / 32] bit masks of the

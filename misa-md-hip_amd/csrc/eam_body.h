@@ -75,6 +75,48 @@ __device__ __forceinline__ SegS seg_s(double s2, const TabMixed &t) {
   return r;
 }
 
+// ------------------------------------------------------ lane-pair gathers
+
+// v of the other lane of the pair (2m, 2m + 1): DPP quad permutation
+// [1, 0, 3, 2].  Both lanes of the pair must be active.
+__device__ __forceinline__ uint32_t pair_swap(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+}
+__device__ __forceinline__ double pair_swap(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint64_t lo = pair_swap((uint32_t)b), hi = pair_swap((uint32_t)(b >> 32));
+  return __builtin_bit_cast(double, (hi << 32) | lo);
+}
+__device__ __forceinline__ double2 pair_swap(double2 v) { return make_double2(pair_swap(v.x), pair_swap(v.y)); }
+
+// A lane pair's two 32-byte records (two 16-byte halves each) gathered one
+// record per instruction: the first load brings the even lane's record (its
+// first half to the even lane, its second half to the odd lane), the second
+// load the odd lane's; pair_rec32_halves swaps the half each lane lacks.  A
+// gather then touches the cache lines of 32 records instead of 64.  Both
+// lanes of every pair must be active.
+struct PairRec32 {
+  double2 e, o;  // this lane's half of the even lane's record, of the odd lane's
+};
+__device__ __forceinline__ PairRec32 pair_rec32_load(const double2 *__restrict__ base, uint32_t rec) {
+  const uint32_t h = threadIdx.x & 1u, rq = pair_swap(rec);
+  return {ld(base, 2 * (h ? rq : rec) + h), ld(base, 2 * (h ? rec : rq) + h)};
+}
+// this lane's record: a = first half, b = second half
+__device__ __forceinline__ void pair_rec32_halves(const PairRec32 &r, double2 &a, double2 &b) {
+  const bool h = threadIdx.x & 1u;
+  const double2 y = pair_swap(h ? r.e : r.o);
+  a = h ? y : r.e;
+  b = h ? r.o : y;
+}
+
+// spl_value with the value record gathered per lane pair
+__device__ __forceinline__ double spl_value_pairs(const SplF64 &s, uint32_t rec, double p) {
+  double2 a, b;
+  pair_rec32_halves(pair_rec32_load(s.val, rec), a, b);
+  return ((a.x * p + a.y) * p + b.x) * p + b.y;
+}
+
 // ------------------------------------------------------------- pair terms
 //
 // A trip's kUnroll pairs are evaluated in three phases — table indices, then
@@ -105,15 +147,24 @@ __device__ __forceinline__ double rho_deriv_f64(const TabF64 &tf, uint32_t rec, 
 // rinv = 1/r; an unlike pair brings rho'_{ti} in rhoip_u (a like pair reuses
 // rho'_{tj}).  phi receives phi_{ti tj}(r) when WANT_E and the (wave-uniform)
 // entry counts the pair's energy.
-template <bool WANT_E>
-__device__ __forceinline__ double fpair_f64_rec(const TabF64 &tf, uint32_t rp, double p, double rinv, double dfi,
-                                                double dfj, bool unlike, double rhoip_u, bool count_e, double &phi) {
-  const double2 c0 = ld(tf.fc, 3 * rp), c1 = ld(tf.fc, 3 * rp + 1), c2 = ld(tf.fc, 3 * rp + 2);
+// c0..c2: the record's three 16-byte pieces.  PAIRED: phi's value record is
+// gathered per lane pair (every lane of the wave runs this call).
+template <bool WANT_E, bool PAIRED = false>
+__device__ __forceinline__ double fpair_f64_c(const TabF64 &tf, double2 c0, double2 c1, double2 c2, uint32_t rp,
+                                              double p, double rinv, double dfi, double dfj, bool unlike,
+                                              double rhoip_u, bool count_e, double &phi) {
   const double rhojp = (c0.x * p + c0.y) * p + c1.x;
   const double phip = (c1.y * p + c2.x) * p + c2.y;
   const double rhoip = unlike ? rhoip_u : rhojp;
-  if (WANT_E && count_e) phi = spl_value(tf.phi, rp, p);
+  if (WANT_E && count_e) phi = PAIRED ? spl_value_pairs(tf.phi, rp, p) : spl_value(tf.phi, rp, p);
   return -(dfi * rhojp + dfj * rhoip + phip) * rinv;
+}
+
+template <bool WANT_E>
+__device__ __forceinline__ double fpair_f64_rec(const TabF64 &tf, uint32_t rp, double p, double rinv, double dfi,
+                                                double dfj, bool unlike, double rhoip_u, bool count_e, double &phi) {
+  return fpair_f64_c<WANT_E>(tf, ld(tf.fc, 3 * rp), ld(tf.fc, 3 * rp + 1), ld(tf.fc, 3 * rp + 2), rp, p, rinv, dfi,
+                             dfj, unlike, rhoip_u, count_e, phi);
 }
 
 // One FP64 pair from its squared distance, rho'_{ti} loaded here (the mixed
@@ -198,12 +249,13 @@ __device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, uint32_t ti
     rp[u] = __umul24(tj[u], (uint32_t)tm.n_s) + row_i + sg.m;
     ra[u] = ra_i + sg.m;
   }
-  double2 b0[U], b1[U], ua[U], vb[U];
+  // the 32-byte force records are gathered per lane pair
+  PairRec32 fr[U];
+  double2 ua[U], vb[U];
   float2 vc[U];
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    b0[u] = ld(tm.fb, 2 * rp[u]);
-    b1[u] = ld(tm.fb, 2 * rp[u] + 1);
+    fr[u] = pair_rec32_load(tm.fb, rp[u]);
     // rho'_{ti} only on unlike lanes (like pairs reuse rho'_{tj}), into
     // registers set first so that no copy, and no wait, is needed where the
     // lanes rejoin
@@ -217,8 +269,10 @@ __device__ __forceinline__ void trip_force_mixed(const TabMixed &tm, uint32_t ti
 #pragma unroll
   for (int u = 0; u < U; u++) {
     const float qf = (float)q[u];
-    const double rhojp = mix_deriv_rec(b0[u], q[u], qf);
-    const double phip = mix_deriv_rec(b1[u], q[u], qf);
+    double2 b0, b1;
+    pair_rec32_halves(fr[u], b0, b1);
+    const double rhojp = mix_deriv_rec(b0, q[u], qf);
+    const double phip = mix_deriv_rec(b1, q[u], qf);
     const double rhoip = ti != tj[u] ? mix_deriv_rec(ua[u], q[u], qf) : rhojp;
     if (WANT_E && count_e[u]) {
       const float cc = fmaf(qf, vc[u].y, vc[u].x);
@@ -275,14 +329,16 @@ __device__ __forceinline__ void trip_rho_f64(const TabF64 &tf, const uint32_t *t
     p[u] = sg.p;
     rec[u] = __umul24(tj[u], (uint32_t)tf.n_r) + (uint32_t)sg.m;
   }
-  double2 a[U], b[U];
+  // the 32-byte value records are gathered per lane pair
+  PairRec32 vr[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) vr[u] = pair_rec32_load(tf.rho.val, rec[u]);
 #pragma unroll
   for (int u = 0; u < U; u++) {
-    a[u] = ld(tf.rho.val, 2 * rec[u]);
-    b[u] = ld(tf.rho.val, 2 * rec[u] + 1);
+    double2 a, b;
+    pair_rec32_halves(vr[u], a, b);
+    v[u] = ((a.x * p[u] + a.y) * p[u] + b.x) * p[u] + b.y;
   }
-#pragma unroll
-  for (int u = 0; u < U; u++) v[u] = ((a[u].x * p[u] + a[u].y) * p[u] + b[u].x) * p[u] + b[u].y;
 }
 
 // --------------------------------------------------------------- kernels
@@ -350,11 +406,15 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
   int8_t *lt = reinterpret_cast<int8_t *>(lz + h.n);
   stage_tile<false>(g, h, t, lb, P, nullptr, lxy, lz, lt);
   double e_loc = 0.0;
-  if (t.active) {
+  // every lane of a wave with an active site runs the stencil loop, pairs
+  // masked on lanes without an occupied site (as in force_block)
+  if (__ballot(t.active) != 0) {
     const double2 cxy = lxy[t.c];
     const double cz = lz[t.c];
-    const int t0 = lt[t.c];
-    if (t0 >= 0) {
+    const int tc = lt[t.c];
+    const bool live = t.active && tc >= 0;
+    const int t0 = live ? tc : 0;
+    {
       const int32_t *o = off + t.s * kMaxStencil;
       const int nk = t.s ? nk_odd : nk_even;
       constexpr bool BOTH = PREC == 1 && LEAN && WANT_E;  // dynamics from lean records, energy from cubic ones
@@ -374,7 +434,7 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
           tq[u] = min((uint32_t)ltu[j], tmax);
           const double dx = cxy.x - pxy.x, dy = cxy.y - pxy.y, dz = cz - lz[j];
           d2[u] = dx * dx + dy * dy + dz * dz;
-          in[u] = d2[u] < rc2 && (!tail || k + u < nk);
+          in[u] = live && d2[u] < rc2 && (!tail || k + u < nk);
           any |= in[u];
         }
         if (__ballot(any) == 0) return;
@@ -405,16 +465,18 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
       const int nk4 = nk & ~(kUnroll - 1);
       for (int k = 0; k < nk4; k += kUnroll) trip(k, false);
       if (nk4 < nk) trip(nk4, true);
-      const uint32_t i = (uint32_t)t.i;
-      RHO[i] = acc;
-      const Seg se = seg_f64(acc, tf.rho0, tf.inv_drho, tf.n_rho);
-      const uint32_t re = (uint32_t)(t0 * tf.n_rho + se.m);
-      DF[i] = spl_deriv(tf.emb, re, se.p);
-      if (BOTH) {
-        const Seg sv = seg_f64(acc_e, tf.rho0, tf.inv_drho, tf.n_rho);
-        e_loc = spl_value(tf.emb, (uint32_t)(t0 * tf.n_rho + sv.m), sv.p);
-      } else if (WANT_E) {
-        e_loc = spl_value(tf.emb, re, se.p);
+      if (live) {
+        const uint32_t i = (uint32_t)t.i;
+        RHO[i] = acc;
+        const Seg se = seg_f64(acc, tf.rho0, tf.inv_drho, tf.n_rho);
+        const uint32_t re = (uint32_t)(t0 * tf.n_rho + se.m);
+        DF[i] = spl_deriv(tf.emb, re, se.p);
+        if (BOTH) {
+          const Seg sv = seg_f64(acc_e, tf.rho0, tf.inv_drho, tf.n_rho);
+          e_loc = spl_value(tf.emb, (uint32_t)(t0 * tf.n_rho + sv.m), sv.p);
+        } else if (WANT_E) {
+          e_loc = spl_value(tf.emb, re, se.p);
+        }
       }
     }
   }
@@ -450,15 +512,19 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
   double *qx = reinterpret_cast<double *>(qw + kSlots * sizeof(uint32_t));
   stage_tile<true>(g, h, t, lb, P, DF, lxy, lzd, lt);
   double e_loc = 0.0;
-  if (t.active) {
+  // Every lane of a wave that holds an active site runs the stencil loop:
+  // lanes without an occupied site (a vacancy, a cell past a ragged box edge)
+  // take part with all their pairs masked, so the mixed path's lane-pair
+  // record exchange always finds both lanes of a pair.
+  if (__ballot(t.active) != 0) {
     const double2 cxy = lxy[t.c], czd = lzd[t.c];
-    const int t0 = lt[t.c];
+    const int tc = lt[t.c];
+    const bool live = t.active && tc >= 0;
+    const int t0 = live ? tc : 0;
     const int nk = t.s ? nk_odd : nk_even;
     const size_t nl = (size_t)g.n_local;
     const uint32_t q = HALF ? local_q(g, t.s, t.x0 + t.tx, t.y0 + t.ty, t.z0 + t.tz) : 0u;
-    if (HALF && t0 < 0)
-      for (int k = 0; k < nk; k++) SL[(size_t)k * nl + q] = 0.0;
-    if (t0 >= 0) {
+    {
       static_assert(!HALF || kHalfFwd == 0, "forward rows lead the half-stencil image");
       const int32_t *o = off + t.s * kMaxStencil;
       // entries that count the pair energy (each pair once for a symmetric stencil)
@@ -484,11 +550,11 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           dz[u] = czd.x - pzd.x;
           dfj[u] = pzd.y;
           d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-          in[u] = d2[u] < rc2 && (!tail || k + u < nk);
+          in[u] = live && d2[u] < rc2 && (!tail || k + u < nk);
           any |= in[u];
         }
         if (__ballot(any) == 0) {
-          if (HALF) {
+          if (HALF && t.active) {
 #pragma unroll
             for (int u = 0; u < kUnroll; u++)
               if (!tail || k + u < nk) SL[(size_t)(k + u) * nl + q] = 0.0;
@@ -533,9 +599,27 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           }
           unlike_queue<kUnroll>(qr, qx, ul, ra, pr, rhoq,
                                 [&](uint32_t rec, double x) { return rho_deriv_f64(tf, rec, x); });
+          // 48-byte force records gathered per lane pair: the first load
+          // brings pieces 0 and 1 of the even lane's record, the second its
+          // piece 2 and piece 0 of the odd lane's, the third the odd lane's
+          // pieces 1 and 2; a DPP swap of piece 1 completes both records
+          // (each gather touches the lines of 32 records in two of three loads)
+          const uint32_t hl = threadIdx.x & 1u;
+          double2 r0[kUnroll], r1[kUnroll], r2[kUnroll];
 #pragma unroll
-          for (int u = 0; u < kUnroll; u++)
-            fp[u] = fpair_f64_rec<WANT_E>(tf, rp[u], pr[u], rinv[u], dfi, dfj[u], tq[u] != ti, rhoq[u], ce[u], phi[u]);
+          for (int u = 0; u < kUnroll; u++) {
+            const uint32_t rq = pair_swap(rp[u]);
+            const uint32_t re = hl ? rq : rp[u], ro = hl ? rp[u] : rq;
+            r0[u] = ld(tf.fc, 3 * re + hl);
+            r1[u] = ld(tf.fc, hl ? 3 * ro : 3 * re + 2);
+            r2[u] = ld(tf.fc, 3 * ro + 1 + hl);
+          }
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) {
+            const double2 c1 = pair_swap(hl ? r0[u] : r2[u]);
+            fp[u] = fpair_f64_c<WANT_E, true>(tf, hl ? r1[u] : r0[u], c1, hl ? r2[u] : r1[u], rp[u], pr[u], rinv[u],
+                                              dfi, dfj[u], tq[u] != ti, rhoq[u], ce[u], phi[u]);
+          }
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
@@ -544,17 +628,19 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           ay = fma(dy[u], f, ay);
           az = fma(dz[u], f, az);
           if (WANT_E) ep += in[u] ? phi[u] : 0.0;
-          if (HALF && (!tail || k + u < nk)) SL[(size_t)(k + u) * nl + q] = f;
+          if (HALF && t.active && (!tail || k + u < nk)) SL[(size_t)(k + u) * nl + q] = f;
         }
       };
       const int nk4 = nk & ~(kUnroll - 1);
       for (int k = 0; k < nk4; k += kUnroll) trip(k, false);
       if (nk4 < nk) trip(nk4, true);
-      const uint32_t i = (uint32_t)t.i;
-      FX[i] = ax;
-      FY[i] = ay;
-      FZ[i] = az;
-      if (WANT_E) e_loc = ew * ep;
+      if (live) {
+        const uint32_t i = (uint32_t)t.i;
+        FX[i] = ax;
+        FY[i] = ay;
+        FZ[i] = az;
+        if (WANT_E) e_loc = ew * ep;
+      }
     }
   }
   if (WANT_E) {

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kBlock) void kb_rho_df(BatchLists L, Halo h, TabF64
 }
 
 template <int PREC, bool WANT_E>
-__global__ __launch_bounds__(kBlock) void kb_force(BatchLists L, Halo h, TabF64 tf, TabMixed tm, double rc2) {
+__global__ __launch_bounds__(kBlock, kForceWavesPerSimd) void kb_force(BatchLists L, Halo h, TabF64 tf, TabMixed tm, double rc2) {
   extern __shared__ double2 lds_tile[];
   __shared__ double red[kWavesPerBlock];
   const Range r = xcd_range(*L.n_tile_items);

@@ -19,7 +19,7 @@ __global__ __launch_bounds__(kBlock) void k_rho_df(Geom g, Halo h, const int32_t
 }
 
 template <int PREC, bool WANT_E>
-__global__ __launch_bounds__(kBlock) void k_force(Geom g, Halo h, const int32_t *__restrict__ off, int32_t nk_even,
+__global__ __launch_bounds__(kBlock, kForceWavesPerSimd) void k_force(Geom g, Halo h, const int32_t *__restrict__ off, int32_t nk_even,
                                                     int32_t nk_odd, const double4 *__restrict__ P,
                                                     const double *__restrict__ DF, double *__restrict__ FX,
                                                     double *__restrict__ FY, double *__restrict__ FZ, TabF64 tf,

@@ -33,7 +33,7 @@ __device__ __forceinline__ int32_t wrap1(int32_t v, int32_t n) { return v < 0 ? 
 
 // forward pass: force_block's HALF mode (forward entries, slot stores)
 template <int PREC, bool WANT_E>
-__global__ __launch_bounds__(kBlock) void k_force_fwd(Geom g, Halo h, const int32_t *__restrict__ img, int32_t nf_even,
+__global__ __launch_bounds__(kBlock, kForceWavesPerSimd) void k_force_fwd(Geom g, Halo h, const int32_t *__restrict__ img, int32_t nf_even,
                                                         int32_t nf_odd, const double4 *__restrict__ P,
                                                         const double *__restrict__ DF, double *__restrict__ FX,
                                                         double *__restrict__ FY, double *__restrict__ FZ,
