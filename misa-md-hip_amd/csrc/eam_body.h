@@ -19,8 +19,19 @@ namespace mmd {
 #define MMD_PAIR_UNROLL 4
 #endif
 constexpr int kUnroll = MMD_PAIR_UNROLL;  // stencil entries whose loads are in flight together
+// Stencil entries per trip of the rho pass.  Its occupancy is set by its LDS
+// tile (four blocks per CU), so the mixed MD pass (one 16-byte gather per
+// pair, under 100 registers) keeps eight entries' loads in flight; the FP64
+// pass and the passes that sum two rho records would leave four waves per
+// SIMD at eight (measured slower) and keep four.
+template <int PREC, bool LEAN, bool WANT_E>
+constexpr int rho_unroll() {
+  return PREC == 1 && LEAN && !WANT_E ? 8 : 4;
+}
 static_assert(32 % kUnroll == 0, "a trip's entries share one word of the energy masks");
-// stencil rows are read up to index nk + kUnroll - 2 (rows are zero-padded to kMaxStencil)
+// stencil rows are read up to index nk + U - 1 for U entries per trip (the
+// next trip's offsets are read ahead); rows are zero-padded to kMaxStencil and
+// mmd_set_stencil accepts at most kMaxStencil - 8 entries
 static_assert(kUnroll <= 8, "stencil padding too short for the unroll");
 
 // 1/sqrt(d2) for d2 > 0: hardware reciprocal-square-root seed refined by two
@@ -139,7 +150,8 @@ __device__ __forceinline__ double rho_f64(const TabF64 &tf, int tj, double d2) {
 // rho'_t of FP64 derivative record rec at fraction p
 __device__ __forceinline__ double rho_deriv_f64(const TabF64 &tf, uint32_t rec, double p) {
   const double2 a = ld(tf.rho.d01, rec);
-  return (a.x * p + a.y) * p + ld(tf.rho.d2, rec);
+  const double c = ld(tf.rho.d2, rec);
+  return (a.x * p + a.y) * p + c;
 }
 
 // fpair = -(df_i rho'_{tj} + df_j rho'_{ti} + phi'_{ti tj}) / r (the scalar that
@@ -400,6 +412,8 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
                                              const double4 *__restrict__ P, double *__restrict__ RHO,
                                              double *__restrict__ DF, const TabF64 &tf, const TabMixed &tm, double rc2,
                                              double *__restrict__ e_part, double2 *lds, double *red) {
+  constexpr int kUnrollRho = rho_unroll<PREC, LEAN, WANT_E>();
+  static_assert(kUnrollRho <= 8, "stencil padding too short for the unroll");
   const TileSite t = map_tile(g, h, lb);
   double2 *lxy = lds;
   double *lz = reinterpret_cast<double *>(lds + h.n);
@@ -423,13 +437,13 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
       // their type byte (0xff) is clamped to a valid table row
       const uint8_t *ltu = reinterpret_cast<const uint8_t *>(lt);
       const uint32_t tmax = (uint32_t)(tf.nt - 1);
-      auto trip = [&](int k, bool tail) {
-        uint32_t tq[kUnroll];
-        double d2[kUnroll];
-        bool in[kUnroll], any = false;
+      auto trip = [&](int k, bool tail, const int32_t *oc) {
+        uint32_t tq[kUnrollRho];
+        double d2[kUnrollRho];
+        bool in[kUnrollRho], any = false;
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-          const int32_t j = t.c + o[k + u];
+        for (int u = 0; u < kUnrollRho; u++) {
+          const int32_t j = t.c + oc[u];
           const double2 pxy = lxy[j];
           tq[u] = min((uint32_t)ltu[j], tmax);
           const double dx = cxy.x - pxy.x, dy = cxy.y - pxy.y, dz = cz - lz[j];
@@ -438,33 +452,45 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
           any |= in[u];
         }
         if (__ballot(any) == 0) return;
-        double v[kUnroll], ve[kUnroll];
+        double v[kUnrollRho], ve[kUnrollRho];
         if constexpr (PREC == 1) {
-          trip_rho_mixed<kUnroll, LEAN>(tm, tq, d2, v);
-          if constexpr (BOTH) trip_rho_mixed<kUnroll, false>(tm, tq, d2, ve);
+          trip_rho_mixed<kUnrollRho, LEAN>(tm, tq, d2, v);
+          if constexpr (BOTH) trip_rho_mixed<kUnrollRho, false>(tm, tq, d2, ve);
           uint64_t redo = 0;  // lanes with a pair to redo (scalar masks)
 #pragma unroll
-          for (int u = 0; u < kUnroll; u++)
+          for (int u = 0; u < kUnrollRho; u++)
             redo |= __ballot(in[u] && (__builtin_isnan(v[u]) || (BOTH && __builtin_isnan(ve[u]))));
           if (redo != 0) {
 #pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
+            for (int u = 0; u < kUnrollRho; u++) {
               if (in[u] && __builtin_isnan(v[u])) v[u] = rho_f64(tf, (int)tq[u], d2[u]);
               if (BOTH && in[u] && __builtin_isnan(ve[u])) ve[u] = rho_f64(tf, (int)tq[u], d2[u]);
             }
           }
         } else {
-          trip_rho_f64<kUnroll>(tf, tq, d2, v);
+          trip_rho_f64<kUnrollRho>(tf, tq, d2, v);
         }
 #pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
+        for (int u = 0; u < kUnrollRho; u++) {
           acc += in[u] ? v[u] : 0.0;
           if (BOTH) acc_e += in[u] ? ve[u] : 0.0;
         }
       };
-      const int nk4 = nk & ~(kUnroll - 1);
-      for (int k = 0; k < nk4; k += kUnroll) trip(k, false);
-      if (nk4 < nk) trip(nk4, true);
+      // a trip's stencil offsets are read one trip ahead (rows are padded
+      // far past nk + 2 kUnrollRho), so no trip waits on its scalar loads
+      const int nk4 = nk & ~(kUnrollRho - 1);
+      int32_t oc[kUnrollRho];
+#pragma unroll
+      for (int u = 0; u < kUnrollRho; u++) oc[u] = o[u];
+      for (int k = 0; k < nk4; k += kUnrollRho) {
+        int32_t on[kUnrollRho];
+#pragma unroll
+        for (int u = 0; u < kUnrollRho; u++) on[u] = o[k + kUnrollRho + u];
+        trip(k, false, oc);
+#pragma unroll
+        for (int u = 0; u < kUnrollRho; u++) oc[u] = on[u];
+      }
+      if (nk4 < nk) trip(nk4, true, oc);
       if (live) {
         const uint32_t i = (uint32_t)t.i;
         RHO[i] = acc;
@@ -536,13 +562,13 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
       const uint8_t *ltu = reinterpret_cast<const uint8_t *>(lt);
       const uint32_t tmax = (uint32_t)(tf.nt - 1), ti = (uint32_t)t0;
       const uint32_t ra_i = ti * (uint32_t)tm.n_s, row_i = ra_i * (uint32_t)tm.nt;
-      auto trip = [&](int k, bool tail) {
+      auto trip = [&](int k, bool tail, const int32_t *oc) {
         uint32_t tq[kUnroll];
         double dx[kUnroll], dy[kUnroll], dz[kUnroll], d2[kUnroll], dfj[kUnroll];
         bool in[kUnroll], any = false;
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
-          const int32_t j = t.c + o[k + u];
+          const int32_t j = t.c + oc[u];
           const double2 pxy = lxy[j], pzd = lzd[j];
           tq[u] = min((uint32_t)ltu[j], tmax);
           dx[u] = cxy.x - pxy.x;
@@ -597,13 +623,13 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
             ul[u] = in[u] && tq[u] != ti;
             rhoq[u] = 0.0;
           }
-          unlike_queue<kUnroll>(qr, qx, ul, ra, pr, rhoq,
-                                [&](uint32_t rec, double x) { return rho_deriv_f64(tf, rec, x); });
           // 48-byte force records gathered per lane pair: the first load
           // brings pieces 0 and 1 of the even lane's record, the second its
           // piece 2 and piece 0 of the odd lane's, the third the odd lane's
           // pieces 1 and 2; a DPP swap of piece 1 completes both records
-          // (each gather touches the lines of 32 records in two of three loads)
+          // (each gather touches the lines of 32 records in two of three
+          // loads).  They are issued before the unlike queue, so its round
+          // trips overlap theirs.
           const uint32_t hl = threadIdx.x & 1u;
           double2 r0[kUnroll], r1[kUnroll], r2[kUnroll];
 #pragma unroll
@@ -614,6 +640,8 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
             r1[u] = ld(tf.fc, hl ? 3 * ro : 3 * re + 2);
             r2[u] = ld(tf.fc, 3 * ro + 1 + hl);
           }
+          unlike_queue<kUnroll>(qr, qx, ul, ra, pr, rhoq,
+                                [&](uint32_t rec, double x) { return rho_deriv_f64(tf, rec, x); });
 #pragma unroll
           for (int u = 0; u < kUnroll; u++) {
             const double2 c1 = pair_swap(hl ? r0[u] : r2[u]);
@@ -631,9 +659,21 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           if (HALF && t.active && (!tail || k + u < nk)) SL[(size_t)(k + u) * nl + q] = f;
         }
       };
+      // a trip's stencil offsets are read one trip ahead (rows are padded
+      // far past nk + 2 kUnroll), so no trip waits on its scalar loads
       const int nk4 = nk & ~(kUnroll - 1);
-      for (int k = 0; k < nk4; k += kUnroll) trip(k, false);
-      if (nk4 < nk) trip(nk4, true);
+      int32_t oc[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; u++) oc[u] = o[u];
+      for (int k = 0; k < nk4; k += kUnroll) {
+        int32_t on[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) on[u] = o[k + kUnroll + u];
+        trip(k, false, oc);
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) oc[u] = on[u];
+      }
+      if (nk4 < nk) trip(nk4, true, oc);
       if (live) {
         const uint32_t i = (uint32_t)t.i;
         FX[i] = ax;
