@@ -349,8 +349,9 @@ def migration(w, state, itl, reps=5):
 def dropin_hooks(w, reps=3):
     """The reference hook sequence on host records: per step the rho pass (df fused) and the force pass, each
     copying the whole ghost-extended array of 104-byte _cuAtomElement records (pinned, as hip_create_atoms_mem
-    allocates them) to the device, as the reference plugin does; the outputs (rho and df, or f) are written
-    into the occupied local records directly (the reference copied the local layers back whole)."""
+    allocates them) to the device, as the reference plugin does, by z-chunks while earlier chunks are computed;
+    the outputs (rho and df, or f) are written into the occupied local records directly as their tiles finish
+    (the reference copied the local layers back whole)."""
     import ctypes as C
     n = w["dom"].n_ext()
     nbytes = n * mmd.REF_AOS_DTYPE.itemsize
@@ -378,7 +379,8 @@ def dropin_hooks(w, reps=3):
     finally:
         mmd.lib().mmd_host_free(p)
     atoms = n_atoms_of(w)
-    return {"what": "rho (fused df) + force hooks on pinned host _cuAtomElement records, PCIe transfers included",
+    return {"what": "rho (fused df) + force hooks on pinned host _cuAtomElement records, PCIe transfers included "
+                    "(records copied in by z-chunks overlapped with the pass and the output write-back)",
             "atom_passes_per_s": round(atoms / dt, 1), "ms_per_step": round(dt * 1e3, 3),
             "host_bytes_per_step": int(2 * nbytes + atoms * (16 + 24)),
             "reference_host_bytes_per_step": int(2 * (nbytes + w["dom"].box[2] * w["dom"].ext[0] * w["dom"].ext[1]

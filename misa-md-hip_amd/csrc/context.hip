@@ -488,6 +488,8 @@ extern "C" int mmd_ctx_destroy(mmd_ctx *c) {
     dfree(c->d_maxtype);
     for (auto &p : c->pending) c->event_pool.push_back(p.start), c->event_pool.push_back(p.stop);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    for (auto e : c->aos_events) (void)hipEventDestroy(e);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return MMD_OK;
@@ -1199,6 +1201,81 @@ static int aos_desc(const mmd_aos_layout *lay, AosDesc *d) {
 // windows a pass reads cost nearly the whole record either way (strided
 // copies of the windows and kernel reads of the mapped records both measured
 // slower than the plain copy).
+// The rho (fused df) or force pass on pinned records, pipelined by z-chunks of
+// the ghost-extended lattice: the copy stream brings the records in chunk by
+// chunk while the compute stream unpacks each chunk, runs the pass over the
+// z-rows of tiles whose halo has arrived and writes their outputs straight
+// into the records.  The outputs leave over the host link while later chunks
+// come in (the link carries both directions at once); the records a pack
+// writes lie in chunks already copied, so no copy reads a record being
+// written.  A type with no table is found after the last chunk: the call
+// fails, and the output fields of some occupied local records may already
+// hold this pass's values (every other field is untouched).
+static int aos_pass_chunked(mmd_ctx *c, const char *atoms, char *mapped, const AosDesc &d, size_t stride,
+                            uint32_t in, uint32_t out, int pass, double cutoff) {
+  const Geom &g = c->g;
+  const size_t layer = (size_t)g.ext_x * g.ext_y * stride;  // bytes of one ext z-layer of records
+  const int64_t dev_layer = (int64_t)g.ext_x * g.ext_y;     // device sites of one ext z-layer
+  const int64_t ext_z = g.ext_z;
+  // chunks of at least the stencil's z reach, about sixteen of them
+  const int64_t cz = std::max<int64_t>(std::max<int64_t>(c->halo.rz, 1), (ext_z + 15) / 16);
+  const int64_t nch = (ext_z + cz - 1) / cz;
+  if (!c->copy_stream) HIP_TRY(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+  while ((int64_t)c->aos_events.size() < nch + 1) {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    c->aos_events.push_back(e);
+  }
+  // the copies start after the compute stream's earlier work
+  HIP_TRY(hipEventRecord(c->aos_events[(size_t)nch], c->stream));
+  HIP_TRY(hipStreamWaitEvent(c->copy_stream, c->aos_events[(size_t)nch], 0));
+  for (int64_t k = 0; k < nch; k++) {
+    const int64_t z0 = k * cz, z1 = std::min(z0 + cz, ext_z);
+    HIP_TRY(hipMemcpyAsync(c->aos + (size_t)z0 * layer, atoms + (size_t)z0 * layer, (size_t)(z1 - z0) * layer,
+                           hipMemcpyHostToDevice, c->copy_stream));
+    HIP_TRY(hipEventRecord(c->aos_events[(size_t)k], c->copy_stream));
+  }
+  if (in & MMD_F_TYPE) HIP_TRY(launch_fill_i32(c->stream, c->d_maxtype, (int64_t)kMaxTypeSlots * kMaxTypeStride, -1));
+  const int64_t tiles_per_tz = (int64_t)g.ntx * g.nty;       // tiles of one z-row of tiles
+  const int64_t sites_per_z = (int64_t)g.by * 2 * g.bxh;     // local sites of one local z-plane
+  const double rc2 = cutoff * cutoff;
+  int64_t tz_done = 0;
+  for (int64_t k = 0; k < nch; k++) {
+    const int64_t z0 = k * cz, z1 = std::min(z0 + cz, ext_z);
+    HIP_TRY(hipStreamWaitEvent(c->stream, c->aos_events[(size_t)k], 0));
+    HIP_TRY(launch_aos_unpack(c->stream, g, c->aos, d, in, c->pos, c->type, c->df, c->rho, c->n_types, c->d_maxtype,
+                              z0 * dev_layer, z1 * dev_layer));
+    // tile row tz reads ext layers [gz + kTileZ tz - rz, gz + kTileZ (tz + 1) + rz): ready once they are in
+    const int64_t tz_end =
+        z1 >= ext_z ? g.ntz : std::min<int64_t>(g.ntz, std::max<int64_t>(0, (z1 - g.gz - c->halo.rz) / kTileZ));
+    if (tz_end <= tz_done) continue;
+    if (pass == 0)
+      HIP_TRY(launch_rho_df(c->stream, g, c->halo, c->d_off, c->nk_even, c->nk_odd, c->pos, c->rho, c->df, c->tf,
+                            c->tm, rc2, c->prec, nullptr, false, tz_done * tiles_per_tz, tz_end * tiles_per_tz));
+    else
+      HIP_TRY(launch_force(c->stream, g, c->halo, c->d_off, c->nk_even, c->nk_odd, c->pos, c->df, c->fx, c->fy,
+                           c->fz, c->tf, c->tm, rc2, c->prec, nullptr, tz_done * tiles_per_tz,
+                           tz_end * tiles_per_tz));
+    HIP_TRY(launch_aos_pack(c->stream, g, mapped, d, out, c->type, c->rho, c->df, c->fx, c->fy, c->fz,
+                            tz_done * kTileZ * sites_per_z,
+                            std::min<int64_t>(tz_end * kTileZ, g.bz) * sites_per_z));
+    tz_done = tz_end;
+  }
+  if (in & MMD_F_X) c->itl_stale = true;
+  if (in & MMD_F_TYPE) {
+    std::vector<int32_t> slots((size_t)kMaxTypeSlots * kMaxTypeStride);
+    HIP_TRY(hipMemcpyAsync(slots.data(), c->d_maxtype, slots.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    int32_t mx = -1;
+    for (int k = 0; k < kMaxTypeSlots; k++) mx = std::max(mx, slots[(size_t)k * kMaxTypeStride]);
+    c->max_type = std::min(mx, c->n_types - 1);
+    if (mx >= c->n_types) return fail(MMD_E_INVALID, "atom type %d has no table (n_types = %d)", mx, c->n_types);
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return MMD_OK;
+}
+
 static int aos_pass(mmd_ctx *c, void *atoms, const mmd_aos_layout *lay, uint32_t in, uint32_t out, int pass,
                     double cutoff) {
   if (!atoms) return fail(MMD_E_INVALID, "null atom array");
@@ -1219,6 +1296,8 @@ static int aos_pass(mmd_ctx *c, void *atoms, const mmd_aos_layout *lay, uint32_t
     if ((rc = dalloc(&c->aos, bytes))) return rc;
     c->aos_bytes = bytes;
   }
+  if (mapped && c->n_itl == 0 && (pass == 0 || pass == 2))
+    return aos_pass_chunked(c, static_cast<const char *>(atoms), mapped, d, lay->stride, in, out, pass, cutoff);
   HIP_TRY(hipMemcpyAsync(c->aos, atoms, bytes, hipMemcpyHostToDevice, c->stream));
   if (in & MMD_F_TYPE) HIP_TRY(launch_fill_i32(c->stream, c->d_maxtype, (int64_t)kMaxTypeSlots * kMaxTypeStride, -1));
   HIP_TRY(launch_aos_unpack(c->stream, c->g, c->aos, d, in, c->pos, c->type, c->df, c->rho, c->n_types,

@@ -13,6 +13,10 @@
 struct mmd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // drop-in passes on pinned records: the copy stream brings the records in
+  // by z-chunks (one event per chunk) while `stream` computes
+  hipStream_t copy_stream = nullptr;
+  std::vector<hipEvent_t> aos_events;
   bool has_domain = false, has_stencil = false, has_tables = false, md_ready = false;
   mmd_domain dom{};
   mmd::Geom g{};

@@ -225,8 +225,8 @@ struct Site {
 // lb = logical block (the single-domain kernels use xcd_block(); the batched
 // serving kernels take it from their work list, so a replica's sites map to
 // blocks identically in both)
-__device__ __forceinline__ bool map_site_lb(const Geom &g, int64_t lb, Site &st) {
-  const int64_t q = lb * kBlock + threadIdx.x;
+// local site q (device order) -> its device index; false past the last
+__device__ __forceinline__ bool map_site_q(const Geom &g, int64_t q, Site &st) {
   if (q >= g.n_local) return false;
   const uint32_t u = (uint32_t)q, bxh = (uint32_t)g.bxh;
   const uint32_t r = u / bxh, xh = u - r * bxh;  // r = (z * by + y) * 2 + parity
@@ -234,6 +234,10 @@ __device__ __forceinline__ bool map_site_lb(const Geom &g, int64_t lb, Site &st)
   st.active = true;
   st.i = dev_index(g, (int32_t)(r & 1u), (int32_t)xh + g.gxh, (int32_t)yl + g.gy, (int32_t)zl + g.gz);
   return true;
+}
+
+__device__ __forceinline__ bool map_site_lb(const Geom &g, int64_t lb, Site &st) {
+  return map_site_q(g, lb * kBlock + threadIdx.x, st);
 }
 
 __device__ __forceinline__ bool map_site(const Geom &g, Site &st) { return map_site_lb(g, xcd_block(), st); }

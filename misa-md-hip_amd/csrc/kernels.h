@@ -15,14 +15,17 @@ int64_t tile_blocks(const Geom &g);
 // EAM passes; off = the stencil's device image (stencil_image): LDS offsets within the tile, energy masks.
 // lean (mixed, MD steps): rho and df from the quadratic records; with e_part the embedding energy
 // from a second sum over the cubic value records.
+// pair passes over tiles [lb0, lb1) (lb1 < 0: to the last tile)
 hipError_t launch_rho_df(hipStream_t st, const Geom &g, const Halo &h, const int32_t *off, int32_t nk_even,
                          int32_t nk_odd, const double4 *P, double *RHO, double *DF, const TabF64 &tf,
-                         const TabMixed &tm, double rc2, int prec, double *e_part, bool lean);
+                         const TabMixed &tm, double rc2, int prec, double *e_part, bool lean, int64_t lb0 = 0,
+                         int64_t lb1 = -1);
 hipError_t launch_df(hipStream_t st, const Geom &g, const int8_t *T, const double *RHO, double *DF,
                      const TabF64 &tf);
 hipError_t launch_force(hipStream_t st, const Geom &g, const Halo &h, const int32_t *off, int32_t nk_even,
                         int32_t nk_odd, const double4 *P, const double *DF, double *FX, double *FY, double *FZ,
-                        const TabF64 &tf, const TabMixed &tm, double rc2, int prec, double *e_part);
+                        const TabF64 &tf, const TabMixed &tm, double rc2, int prec, double *e_part, int64_t lb0 = 0,
+                        int64_t lb1 = -1);
 hipError_t pair_kernels_allow_lds(size_t bytes);
 // Newton's-third-law half stencil (periodic step only): forward pass (own
 // force + pair-force slots SL[nf_max][n_local]) then backward pass (adds the
@@ -52,11 +55,14 @@ struct AosDesc {
 };
 // largest type unpacked: the maximum over kMaxTypeSlots ints kMaxTypeStride apart
 constexpr int kMaxTypeSlots = 64, kMaxTypeStride = 32;
+// device-layout ext sites [i0, i1) (i1 < 0: to the last)
 hipError_t launch_aos_unpack(hipStream_t st, const Geom &g, const char *aos, AosDesc d, uint32_t fields, double4 *P,
-                             int8_t *T, double *DF, double *RHO, int32_t n_types, int32_t *max_type);
+                             int8_t *T, double *DF, double *RHO, int32_t n_types, int32_t *max_type, int64_t i0 = 0,
+                             int64_t i1 = -1);
+// local sites [q0, q1) in device order (q1 < 0: to the last)
 hipError_t launch_aos_pack(hipStream_t st, const Geom &g, char *aos, AosDesc d, uint32_t fields, const int8_t *T,
                            const double *RHO, const double *DF, const double *FX, const double *FY,
-                           const double *FZ);
+                           const double *FZ, int64_t q0 = 0, int64_t q1 = -1);
 
 // periodic single-domain ghost refresh
 hipError_t launch_ghost(hipStream_t st, const Geom &g, double Lx, double Ly, double Lz, double4 *P, int8_t *T,

@@ -11,11 +11,12 @@ template <int PREC, bool WANT_E, bool LEAN>
 __global__ __launch_bounds__(kBlock) void k_rho_df(Geom g, Halo h, const int32_t *__restrict__ off, int32_t nk_even,
                                                      int32_t nk_odd, const double4 *__restrict__ P,
                                                      double *__restrict__ RHO, double *__restrict__ DF, TabF64 tf,
-                                                     TabMixed tm, double rc2, double *__restrict__ e_part) {
+                                                     TabMixed tm, double rc2, double *__restrict__ e_part,
+                                                     int64_t lb0) {
   extern __shared__ double2 lds_tile[];
   __shared__ double red[kWavesPerBlock];
-  rho_df_block<PREC, WANT_E, LEAN>(g, h, xcd_block(), off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part, lds_tile,
-                                   red);
+  rho_df_block<PREC, WANT_E, LEAN>(g, h, lb0 + xcd_block(), off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part,
+                                   lds_tile, red);
 }
 
 template <int PREC, bool WANT_E>
@@ -23,11 +24,12 @@ __global__ __launch_bounds__(kBlock, kForceWavesPerSimd) void k_force(Geom g, Ha
                                                     int32_t nk_odd, const double4 *__restrict__ P,
                                                     const double *__restrict__ DF, double *__restrict__ FX,
                                                     double *__restrict__ FY, double *__restrict__ FZ, TabF64 tf,
-                                                    TabMixed tm, double rc2, double *__restrict__ e_part) {
+                                                    TabMixed tm, double rc2, double *__restrict__ e_part,
+                                                    int64_t lb0) {
   extern __shared__ double2 lds_tile[];
   __shared__ double red[kWavesPerBlock];
-  force_block<PREC, WANT_E>(g, h, xcd_block(), off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part, lds_tile,
-                            red);
+  force_block<PREC, WANT_E>(g, h, lb0 + xcd_block(), off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part,
+                            lds_tile, red);
 }
 
 // standalone df (src/kernels/hip_kernels.hip.cpp:21-40)
@@ -51,25 +53,31 @@ int64_t site_blocks(const Geom &g) {
 
 int64_t tile_blocks(const Geom &g) { return (g.n_tiles + kXcds - 1) / kXcds * kXcds; }
 
-hipError_t launch_rho_df(hipStream_t st, const Geom &g, const Halo &h, const int32_t *off, int32_t nk_even,
+hipError_t launch_rho_df(hipStream_t st, const Geom &g0, const Halo &h, const int32_t *off, int32_t nk_even,
                          int32_t nk_odd, const double4 *P, double *RHO, double *DF, const TabF64 &tf,
-                         const TabMixed &tm, double rc2, int prec, double *e_part, bool lean) {
-  const dim3 grid((unsigned)tile_blocks(g)), block(kBlock);
+                         const TabMixed &tm, double rc2, int prec, double *e_part, bool lean, int64_t lb0,
+                         int64_t lb1) {
+  // tiles [lb0, lb1): the bodies take g.n_tiles as the end of the launch's range
+  Geom g = g0;
+  if (lb1 < 0 || lb1 > g0.n_tiles) lb1 = g0.n_tiles;
+  if (lb0 >= lb1) return hipSuccess;
+  g.n_tiles = lb1;
+  const dim3 grid((unsigned)((lb1 - lb0 + kXcds - 1) / kXcds * kXcds)), block(kBlock);
   const size_t sh = tile_lds_bytes(h, false);
   if (prec == 0) {
     if (e_part)
-      k_rho_df<0, true, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<0, true, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part, lb0);
     else
-      k_rho_df<0, false, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<0, false, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part, lb0);
   } else {
     if (e_part && lean)
-      k_rho_df<1, true, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<1, true, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part, lb0);
     else if (e_part)
-      k_rho_df<1, true, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<1, true, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part, lb0);
     else if (lean)
-      k_rho_df<1, false, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<1, false, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part, lb0);
     else
-      k_rho_df<1, false, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part);
+      k_rho_df<1, false, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, RHO, DF, tf, tm, rc2, e_part, lb0);
   }
   return hipGetLastError();
 }
@@ -80,21 +88,26 @@ hipError_t launch_df(hipStream_t st, const Geom &g, const int8_t *T, const doubl
   return hipGetLastError();
 }
 
-hipError_t launch_force(hipStream_t st, const Geom &g, const Halo &h, const int32_t *off, int32_t nk_even,
+hipError_t launch_force(hipStream_t st, const Geom &g0, const Halo &h, const int32_t *off, int32_t nk_even,
                         int32_t nk_odd, const double4 *P, const double *DF, double *FX, double *FY, double *FZ,
-                        const TabF64 &tf, const TabMixed &tm, double rc2, int prec, double *e_part) {
-  const dim3 grid((unsigned)tile_blocks(g)), block(kBlock);
+                        const TabF64 &tf, const TabMixed &tm, double rc2, int prec, double *e_part, int64_t lb0,
+                        int64_t lb1) {
+  Geom g = g0;
+  if (lb1 < 0 || lb1 > g0.n_tiles) lb1 = g0.n_tiles;
+  if (lb0 >= lb1) return hipSuccess;
+  g.n_tiles = lb1;
+  const dim3 grid((unsigned)((lb1 - lb0 + kXcds - 1) / kXcds * kXcds)), block(kBlock);
   const size_t sh = tile_lds_bytes(h, true);
   if (prec == 0) {
     if (e_part)
-      k_force<0, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
+      k_force<0, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part, lb0);
     else
-      k_force<0, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
+      k_force<0, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part, lb0);
   } else {
     if (e_part)
-      k_force<1, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
+      k_force<1, true><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part, lb0);
     else
-      k_force<1, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part);
+      k_force<1, false><<<grid, block, sh, st>>>(g, h, off, nk_even, nk_odd, P, DF, FX, FY, FZ, tf, tm, rc2, e_part, lb0);
   }
   return hipGetLastError();
 }

@@ -55,9 +55,9 @@ __global__ __launch_bounds__(kBlock) void k_aos_unpack(Geom g, const char *__res
                                                          uint32_t fields, double4 *__restrict__ P,
                                                          int8_t *__restrict__ T, double *__restrict__ DF,
                                                          double *__restrict__ RHO, int32_t n_types,
-                                                         int32_t *__restrict__ max_type) {
-  const uint32_t idx = blockIdx.x * kBlock + threadIdx.x;
-  const bool live = idx < (uint32_t)g.n_ext;
+                                                         int32_t *__restrict__ max_type, uint32_t i0, uint32_t i1) {
+  const uint32_t idx = i0 + blockIdx.x * kBlock + threadIdx.x;
+  const bool live = idx < i1;
   const char *rec = aos + (live ? ref_of(g, ext_of(g, idx)) * d.stride : 0);
   int32_t t = -1;
   if (live && (fields & 0x20u)) {
@@ -102,9 +102,11 @@ typedef double DPair __attribute__((ext_vector_type(2), aligned(8)));
 __global__ __launch_bounds__(kBlock) void k_aos_pack(Geom g, char *__restrict__ aos, AosDesc d, uint32_t fields,
                                                        const int8_t *__restrict__ T, const double *__restrict__ RHO,
                                                        const double *__restrict__ DF, const double *__restrict__ FX,
-                                                       const double *__restrict__ FY, const double *__restrict__ FZ) {
+                                                       const double *__restrict__ FY, const double *__restrict__ FZ,
+                                                       int64_t q0, int64_t q1) {
   Site st;
-  if (!map_site(g, st) || !st.active) return;
+  const int64_t q = q0 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= q1 || !map_site_q(g, q, st)) return;
   const int64_t i = st.i;
   if (T[i] < 0) return;
   ExtPos p;
@@ -229,15 +231,22 @@ hipError_t launch_dev2ref_i8(hipStream_t st, const Geom &g, const int8_t *dev, i
 }
 
 hipError_t launch_aos_unpack(hipStream_t st, const Geom &g, const char *aos, AosDesc d, uint32_t fields, double4 *P,
-                             int8_t *T, double *DF, double *RHO, int32_t n_types, int32_t *max_type) {
-  k_aos_unpack<<<grid_ext(g), kBlock, 0, st>>>(g, aos, d, fields, P, T, DF, RHO, n_types, max_type);
+                             int8_t *T, double *DF, double *RHO, int32_t n_types, int32_t *max_type, int64_t i0,
+                             int64_t i1) {
+  if (i1 < 0 || i1 > g.n_ext) i1 = g.n_ext;
+  if (i0 >= i1) return hipSuccess;
+  k_aos_unpack<<<dim3((unsigned)((i1 - i0 + kBlock - 1) / kBlock)), kBlock, 0, st>>>(
+      g, aos, d, fields, P, T, DF, RHO, n_types, max_type, (uint32_t)i0, (uint32_t)i1);
   return hipGetLastError();
 }
 
 hipError_t launch_aos_pack(hipStream_t st, const Geom &g, char *aos, AosDesc d, uint32_t fields, const int8_t *T,
                            const double *RHO, const double *DF, const double *FX, const double *FY,
-                           const double *FZ) {
-  k_aos_pack<<<dim3((unsigned)site_blocks(g)), kBlock, 0, st>>>(g, aos, d, fields, T, RHO, DF, FX, FY, FZ);
+                           const double *FZ, int64_t q0, int64_t q1) {
+  if (q1 < 0 || q1 > g.n_local) q1 = g.n_local;
+  if (q0 >= q1) return hipSuccess;
+  k_aos_pack<<<dim3((unsigned)((q1 - q0 + kBlock - 1) / kBlock)), kBlock, 0, st>>>(g, aos, d, fields, T, RHO, DF, FX,
+                                                                                   FY, FZ, q0, q1);
   return hipGetLastError();
 }
 

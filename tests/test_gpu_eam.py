@@ -205,8 +205,10 @@ class _PinnedRecords:
 def test_dropin_aos_hooks_match_oracle(memory):
     """hip_eam_rho_calc / hip_eam_force_calc contract: host AoS records in,
     rho/df (or f) of local records out, everything else untouched.  Pageable
-    records take the whole-array copies; pinned ones move only the fields a
-    pass reads (strided copies) and writes (straight into the records)."""
+    records take the whole-array copies; pinned ones are copied in by
+    z-chunks while earlier chunks are computed, and the outputs are written
+    straight into the records.  The 7x9x5 box's 11 ext z-layers make six
+    chunks of two layers."""
     c = W.case(**CASES["ragged_7x9x5"])
     n = c["st"].x.size
     if memory == "pinned":
@@ -273,6 +275,37 @@ def test_dropin_rejects_type_without_table():
             ctx.rho_calc_aos(bad, RC)
         assert e.value.code == mmd.E_INVALID
         assert np.all(bad["rho"] == -3.0)
+        recs["rho"] = 0.0
+        ctx.rho_calc_aos(recs, RC)
+        assert np.array_equal(recs["rho"][occ], good[occ])
+
+
+def test_dropin_pinned_rejects_type_without_table():
+    """Pinned records run the pass pipelined by z-chunks, so a type with no
+    table is reported after the last chunk: the hook fails with
+    MMD_E_INVALID, only output fields may have changed, and the next valid
+    call computes as before."""
+    c = W.case(**CASES["ragged_7x9x5"])
+    st = c["st"]
+    n = st.x.size
+    li = st.local_index()
+    occ = li[st.type[li] >= 0]
+    n_types = int(np.asarray(c["tab"]["rho_r"]).shape[0])
+    with _PinnedRecords(n) as recs, _ctx(c) as ctx:
+        recs["id"] = np.arange(n)
+        recs["type"] = st.type
+        recs["x"] = np.stack([st.x, st.y, st.z], 1)
+        ctx.rho_calc_aos(recs, RC)
+        good = recs["rho"].copy()
+        recs["type"][occ[len(occ) // 2]] = n_types
+        recs["rho"] = -3.0
+        before = recs.copy()
+        with pytest.raises(mmd.MmdError) as e:
+            ctx.rho_calc_aos(recs, RC)
+        assert e.value.code == mmd.E_INVALID
+        for f in ("id", "type", "x", "v", "f"):
+            assert np.array_equal(recs[f], before[f])
+        recs["type"] = st.type
         recs["rho"] = 0.0
         ctx.rho_calc_aos(recs, RC)
         assert np.array_equal(recs["rho"][occ], good[occ])
