@@ -628,20 +628,26 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           // piece 2 and piece 0 of the odd lane's, the third the odd lane's
           // pieces 1 and 2; a DPP swap of piece 1 completes both records
           // (each gather touches the lines of 32 records in two of three
-          // loads).  They are issued before the unlike queue, so its round
-          // trips overlap theirs.
+          // loads).  The full stencil issues them before the unlike queue,
+          // so its round trips overlap theirs; the half stencil's forward
+          // pass, whose slot stores hold more registers, issues them after
+          // (issued first they spill there).
           const uint32_t hl = threadIdx.x & 1u;
           double2 r0[kUnroll], r1[kUnroll], r2[kUnroll];
+          auto gather_records = [&] {
 #pragma unroll
-          for (int u = 0; u < kUnroll; u++) {
-            const uint32_t rq = pair_swap(rp[u]);
-            const uint32_t re = hl ? rq : rp[u], ro = hl ? rp[u] : rq;
-            r0[u] = ld(tf.fc, 3 * re + hl);
-            r1[u] = ld(tf.fc, hl ? 3 * ro : 3 * re + 2);
-            r2[u] = ld(tf.fc, 3 * ro + 1 + hl);
-          }
+            for (int u = 0; u < kUnroll; u++) {
+              const uint32_t rq = pair_swap(rp[u]);
+              const uint32_t re = hl ? rq : rp[u], ro = hl ? rp[u] : rq;
+              r0[u] = ld(tf.fc, 3 * re + hl);
+              r1[u] = ld(tf.fc, hl ? 3 * ro : 3 * re + 2);
+              r2[u] = ld(tf.fc, 3 * ro + 1 + hl);
+            }
+          };
+          if (!HALF) gather_records();
           unlike_queue<kUnroll>(qr, qx, ul, ra, pr, rhoq,
                                 [&](uint32_t rec, double x) { return rho_deriv_f64(tf, rec, x); });
+          if (HALF) gather_records();
 #pragma unroll
           for (int u = 0; u < kUnroll; u++) {
             const double2 c1 = pair_swap(hl ? r0[u] : r2[u]);
