@@ -31,7 +31,8 @@ The same run also measures
     10..64 BCC cells, 3-type alloy, 300-1500 K, 200 steps each, mixed
     precision) arriving as a Poisson stream at 75% of the capacity measured
     by a closed-loop burst just before; throughput over the whole trace and
-    p50/p99 per-replica latency (arrival to results);
+    p50/p99 per-replica latency (arrival to results); the same burst served
+    in FP64 gives the mixed-over-FP64 capacity ratio of the serving form;
   * the CPU oracle (FP64 C restatement of the reference's EAM algorithm,
     OpenMP on the host cores) on the headline box, one thread on the 20^3-cell
     single-element box, and the per-atom check of the GPU's final forces
@@ -397,25 +398,34 @@ def replica_trace(n, seed, rate, t0=0.0, steps=200):
                  arrival_s=float(arr[i])) for i in range(n)]
 
 
+def burst_capacity(srv, burst, seed, steps):
+    """Atom-steps/s of a burst of the replica mix, all present at t = 0, measured while the admission queue is
+    non-empty (from the first replica resident to the last admission), each replica's atom-steps spread evenly
+    over its residency."""
+    res, st = srv.run(replica_trace(burst, seed, 0.0, steps=steps))
+    t0 = min(r["t_ready"] for r in res)
+    t1 = max(r["t_admit"] for r in res)
+    work = 0.0
+    for r in res:
+        span = r["t_done"] - r["t_ready"]
+        overlap = max(0.0, min(r["t_done"], t1) - max(r["t_ready"], t0))
+        work += r["n_atoms"] * steps * (overlap / span if span > 0 else 0.0)
+    return work / (t1 - t0)
+
+
 def serving(n_replicas, load, seed=77, burst=512, steps=200, use_graph=True, ghost_cells=2):
     """Replicas carry no interstitials, so their ghost shell only needs the
-    stencil's reach (2 cells at rc + skin = 5.9 A)."""
+    stencil's reach (2 cells at rc + skin = 5.9 A).  The same burst is also
+    served in FP64, so the mixed serving form is compared with the FP64 one
+    on the same replicas (their temperatures and 200-step trajectories, which
+    the synthetic forms disorder, slow both precisions alike)."""
     tab = mmd.synth_tables(n_types=3, rc=RC, jitter=0.1, seed=seed)
-    with mmd.Server(tab, precision=mmd.PREC_MIXED, a=A, cutoff=RC, skin=SKIN, dt=DT, sigma=0.05, max_replicas=64,
-                    hbm_fraction=0.5, chunk_steps=10, workers=16, use_graph=use_graph,
-                    ghost_cells=ghost_cells) as srv:
-        # capacity: a burst of the same replica mix, all present at t = 0, measured while the admission queue
-        # is non-empty (from the first replica resident to the last admission), each replica's atom-steps
-        # spread evenly over its residency
-        res, st = srv.run(replica_trace(burst, seed + 1, 0.0, steps=steps))
-        t0 = min(r["t_ready"] for r in res)
-        t1 = max(r["t_admit"] for r in res)
-        work = 0.0
-        for r in res:
-            span = r["t_done"] - r["t_ready"]
-            overlap = max(0.0, min(r["t_done"], t1) - max(r["t_ready"], t0))
-            work += r["n_atoms"] * steps * (overlap / span if span > 0 else 0.0)
-        capacity = work / (t1 - t0)
+    opts = dict(a=A, cutoff=RC, skin=SKIN, dt=DT, sigma=0.05, max_replicas=64, hbm_fraction=0.5, chunk_steps=10,
+                workers=16, use_graph=use_graph, ghost_cells=ghost_cells)
+    with mmd.Server(tab, precision=mmd.PREC_FP64, **opts) as srv:
+        capacity_fp64 = burst_capacity(srv, burst, seed + 1, steps)
+    with mmd.Server(tab, precision=mmd.PREC_MIXED, **opts) as srv:
+        capacity = burst_capacity(srv, burst, seed + 1, steps)
         reps = replica_trace(n_replicas, seed, 1.0, steps=steps)
         mean_work = np.mean([2 * r["cells"] ** 3 * r["steps"] for r in reps])
         rate = load * capacity / mean_work
@@ -431,7 +441,9 @@ def serving(n_replicas, load, seed=77, burst=512, steps=200, use_graph=True, gho
                         "to the work lists' capacities" + (", replayed from one captured HIP graph for every mix"
                                                             if use_graph else ", launched directly"),
             "value": round(st["atom_steps_per_s"], 1), "unit": "atom-steps/s",
-            "capacity_atom_steps_per_s": round(capacity, 1), "arrival_rate_per_s": round(rate, 3),
+            "capacity_atom_steps_per_s": round(capacity, 1),
+            "capacity_fp64_serving_atom_steps_per_s": round(capacity_fp64, 1),
+            "capacity_vs_fp64_serving": round(capacity / capacity_fp64, 3), "arrival_rate_per_s": round(rate, 3),
             "offered_load": load, "latency_p50_s": round(st["latency_p50_s"], 4),
             "latency_p99_s": round(st["latency_p99_s"], 4), "latency_mean_s": round(st["latency_mean_s"], 4),
             "wait_mean_s": round(st["wait_mean_s"], 4), "wall_s": round(st["wall_s"], 3),
