@@ -983,7 +983,7 @@ extern "C" int mmd_set_tables(mmd_ctx *c, const mmd_eam_tables *t) {
     tf.rho = F(h_rho), tf.emb = F(h_emb), tf.phi = F(h_phi);
     tf.fc = reinterpret_cast<const double2 *>(dev + h_fc);
     tf.nt = nt, tf.n_r = nr, tf.n_rho = nrho;
-    tf.r0 = t->r0, tf.inv_dr = 1.0 / t->dr, tf.rho0 = t->rho0, tf.inv_drho = 1.0 / t->drho;
+    tf.r0 = t->r0, tf.inv_dr = 1.0 / t->dr, tf.rho0 = t->rho0, tf.inv_drho = 1.0 / t->drho, tf.dr = t->dr;
     TabMixed &tm = c->tm;
     tm = tm_new;
     tm.rho = M(o_rho), tm.phi = M(o_phi);

@@ -115,6 +115,7 @@ struct TabF64 {
   const double2 *fc;  // [nt*nt][n_r][3]
   int32_t nt, n_r, n_rho;
   double r0, inv_dr, rho0, inv_drho;
+  double dr;  // knot spacing of the r tables
 };
 
 __device__ __forceinline__ double spl_value(const SplF64 &s, uint32_t rec, double p) {

@@ -121,12 +121,6 @@ __device__ __forceinline__ void pair_rec32_halves(const PairRec32 &r, double2 &a
   b = h ? r.o : y;
 }
 
-// spl_value with the value record gathered per lane pair
-__device__ __forceinline__ double spl_value_pairs(const SplF64 &s, uint32_t rec, double p) {
-  double2 a, b;
-  pair_rec32_halves(pair_rec32_load(s.val, rec), a, b);
-  return ((a.x * p + a.y) * p + b.x) * p + b.y;
-}
 
 // ------------------------------------------------------------- pair terms
 //
@@ -154,21 +148,32 @@ __device__ __forceinline__ double rho_deriv_f64(const TabF64 &tf, uint32_t rec, 
   return (a.x * p + a.y) * p + c;
 }
 
+// phi_ab at fraction p of record rp from the derivative coefficients of its
+// force record (e0 p + e1) p + e2 and the knot value: the spline's value
+// coefficients are (e0 dr / 3, e1 dr / 2, e2 dr, phi(r_m)), so the value needs
+// one 8-byte gather (the knot value, the last word of the value record)
+// instead of the 32-byte value record.
+__device__ __forceinline__ double phi_from_deriv(const TabF64 &tf, uint32_t rp, double p, double e0, double e1,
+                                                 double e2) {
+  const double c6 = ld(reinterpret_cast<const double *>(tf.phi.val), 4 * rp + 3);
+  const double t = fma(p, e0 * (1.0 / 3.0), 0.5 * e1);
+  return fma(tf.dr * p, fma(p, t, e2), c6);
+}
+
 // fpair = -(df_i rho'_{tj} + df_j rho'_{ti} + phi'_{ti tj}) / r (the scalar that
 // multiplies del = x_i - x_j) from the pair's force record rp at fraction p,
 // rinv = 1/r; an unlike pair brings rho'_{ti} in rhoip_u (a like pair reuses
 // rho'_{tj}).  phi receives phi_{ti tj}(r) when WANT_E and the (wave-uniform)
 // entry counts the pair's energy.
-// c0..c2: the record's three 16-byte pieces.  PAIRED: phi's value record is
-// gathered per lane pair (every lane of the wave runs this call).
-template <bool WANT_E, bool PAIRED = false>
+// c0..c2: the record's three 16-byte pieces.
+template <bool WANT_E>
 __device__ __forceinline__ double fpair_f64_c(const TabF64 &tf, double2 c0, double2 c1, double2 c2, uint32_t rp,
                                               double p, double rinv, double dfi, double dfj, bool unlike,
                                               double rhoip_u, bool count_e, double &phi) {
   const double rhojp = (c0.x * p + c0.y) * p + c1.x;
   const double phip = (c1.y * p + c2.x) * p + c2.y;
   const double rhoip = unlike ? rhoip_u : rhojp;
-  if (WANT_E && count_e) phi = PAIRED ? spl_value_pairs(tf.phi, rp, p) : spl_value(tf.phi, rp, p);
+  if (WANT_E && count_e) phi = phi_from_deriv(tf, rp, p, c1.y, c2.x, c2.y);
   return -(dfi * rhojp + dfj * rhoip + phip) * rinv;
 }
 
@@ -628,10 +633,10 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           // piece 2 and piece 0 of the odd lane's, the third the odd lane's
           // pieces 1 and 2; a DPP swap of piece 1 completes both records
           // (each gather touches the lines of 32 records in two of three
-          // loads).  The full stencil issues them before the unlike queue,
-          // so its round trips overlap theirs; the half stencil's forward
-          // pass, whose slot stores hold more registers, issues them after
-          // (issued first they spill there).
+          // loads).  The plain full-stencil pass issues them before the
+          // unlike queue, so its round trips overlap theirs; the passes that
+          // hold more registers (pair energies, the half stencil's slot
+          // stores) issue them after (issued first they spill there).
           const uint32_t hl = threadIdx.x & 1u;
           double2 r0[kUnroll], r1[kUnroll], r2[kUnroll];
           auto gather_records = [&] {
@@ -644,14 +649,15 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
               r2[u] = ld(tf.fc, 3 * ro + 1 + hl);
             }
           };
-          if (!HALF) gather_records();
+          constexpr bool kEarly = !HALF && !WANT_E;
+          if (kEarly) gather_records();
           unlike_queue<kUnroll>(qr, qx, ul, ra, pr, rhoq,
                                 [&](uint32_t rec, double x) { return rho_deriv_f64(tf, rec, x); });
-          if (HALF) gather_records();
+          if (!kEarly) gather_records();
 #pragma unroll
           for (int u = 0; u < kUnroll; u++) {
             const double2 c1 = pair_swap(hl ? r0[u] : r2[u]);
-            fp[u] = fpair_f64_c<WANT_E, true>(tf, hl ? r1[u] : r0[u], c1, hl ? r2[u] : r1[u], rp[u], pr[u], rinv[u],
+            fp[u] = fpair_f64_c<WANT_E>(tf, hl ? r1[u] : r0[u], c1, hl ? r2[u] : r1[u], rp[u], pr[u], rinv[u],
                                               dfi, dfj[u], tq[u] != ti, rhoq[u], ce[u], phi[u]);
           }
         }
