@@ -442,16 +442,30 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
       // their type byte (0xff) is clamped to a valid table row
       const uint8_t *ltu = reinterpret_cast<const uint8_t *>(lt);
       const uint32_t tmax = (uint32_t)(tf.nt - 1);
-      auto trip = [&](int k, bool tail, const int32_t *oc) {
+      // a trip's neighbour records, read from the LDS tile
+      struct Nbr {
+        double2 xy;
+        double z;
+        uint32_t t;
+      };
+      auto fetch = [&](const int32_t *oc, Nbr *nb) {
+#pragma unroll
+        for (int u = 0; u < kUnrollRho; u++) {
+          const int32_t j = t.c + oc[u];
+          nb[u].xy = lxy[j];
+          nb[u].z = lz[j];
+          nb[u].t = ltu[j];
+        }
+      };
+      auto trip = [&](int k, bool tail, const Nbr *nb) {
         uint32_t tq[kUnrollRho];
         double d2[kUnrollRho];
         bool in[kUnrollRho], any = false;
 #pragma unroll
         for (int u = 0; u < kUnrollRho; u++) {
-          const int32_t j = t.c + oc[u];
-          const double2 pxy = lxy[j];
-          tq[u] = min((uint32_t)ltu[j], tmax);
-          const double dx = cxy.x - pxy.x, dy = cxy.y - pxy.y, dz = cz - lz[j];
+          const double2 pxy = nb[u].xy;
+          tq[u] = min(nb[u].t, tmax);
+          const double dx = cxy.x - pxy.x, dy = cxy.y - pxy.y, dz = cz - nb[u].z;
           d2[u] = dx * dx + dy * dy + dz * dz;
           in[u] = live && d2[u] < rc2 && (!tail || k + u < nk);
           any |= in[u];
@@ -481,21 +495,25 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
           if (BOTH) acc_e += in[u] ? ve[u] : 0.0;
         }
       };
-      // a trip's stencil offsets are read one trip ahead (rows are padded
-      // far past nk + 2 kUnrollRho), so no trip waits on its scalar loads
+      // A trip's stencil offsets are read one trip ahead (rows are padded
+      // far past nk + 2 kUnrollRho), so no trip waits on its scalar loads;
+      // the next trip's neighbour records are read from LDS as a trip ends.
+      // (Reading them before a trip's gathers, to overlap their latency,
+      // measured 3 % slower in the FP64 pass.)
       const int nk4 = nk & ~(kUnrollRho - 1);
       int32_t oc[kUnrollRho];
 #pragma unroll
       for (int u = 0; u < kUnrollRho; u++) oc[u] = o[u];
+      Nbr cur[kUnrollRho];
+      fetch(oc, cur);
       for (int k = 0; k < nk4; k += kUnrollRho) {
         int32_t on[kUnrollRho];
 #pragma unroll
         for (int u = 0; u < kUnrollRho; u++) on[u] = o[k + kUnrollRho + u];
-        trip(k, false, oc);
-#pragma unroll
-        for (int u = 0; u < kUnrollRho; u++) oc[u] = on[u];
+        trip(k, false, cur);
+        fetch(on, cur);
       }
-      if (nk4 < nk) trip(nk4, true, oc);
+      if (nk4 < nk) trip(nk4, true, cur);
       if (live) {
         const uint32_t i = (uint32_t)t.i;
         RHO[i] = acc;
