@@ -280,6 +280,48 @@ def test_dropin_rejects_type_without_table():
         assert np.array_equal(recs["rho"][occ], good[occ])
 
 
+PIPE_CASES = {
+    "ragged_7x9x5": CASES["ragged_7x9x5"],
+    # 168-entry stencil: z reach 3, a 92 KB LDS tile
+    "wide_stencil": dict(cells=(10, 9, 8), n_types=3, sigma=0.05, temperature=900.0, vacancy_frac=0.01, seed=5,
+                         skin=1.9),
+}
+
+
+@pytest.mark.parametrize("prec", [mmd.PREC_FP64, mmd.PREC_MIXED])
+@pytest.mark.parametrize("name", list(PIPE_CASES))
+def test_dropin_pinned_chunks_equal_resident_passes(name, prec):
+    """Pinned records take the z-chunk pipeline (records copied in by
+    chunks, the pass run over the tile rows whose halo has arrived, outputs
+    written as their rows finish).  Its rho, df and f equal the
+    device-resident passes' bit for bit, for the default stencil (z reach 2)
+    and a 168-entry one (z reach 3), in both precisions."""
+    c = W.case(**PIPE_CASES[name])
+    st = c["st"]
+    n = st.x.size
+    li = st.local_index()
+    occ = li[st.type[li] >= 0]
+    with _ctx(c, prec) as ctx:
+        ctx.rho_df(RC)
+        res = st.copy()
+        ctx.download(res, mmd.F_RHO | mmd.F_DF)
+        ctx.md_setup(c["box"], A, 1.0, RC)
+        ctx.ghost_refresh(mmd.F_DF)
+        ctx.force(RC)
+        ctx.download(res, mmd.F_F | mmd.F_DF)
+    with _PinnedRecords(n) as recs, _ctx(c, prec) as ctx:
+        recs["id"] = np.arange(n)
+        recs["type"] = st.type
+        recs["x"] = np.stack([st.x, st.y, st.z], 1)
+        recs["v"] = np.stack([st.vx, st.vy, st.vz], 1)
+        ctx.rho_calc_aos(recs, RC)
+        assert np.array_equal(recs["rho"][occ], res.rho[occ])
+        assert np.array_equal(recs["df"][occ], res.df[occ])
+        recs["df"] = res.df  # every ext site's df, as the core's ghost exchange leaves it
+        ctx.force_calc_aos(recs, RC)
+        assert np.array_equal(recs["f"][occ], np.stack([res.fx[occ], res.fy[occ], res.fz[occ]], 1))
+
+
 def test_dropin_pinned_rejects_type_without_table():
     """Pinned records run the pass pipelined by z-chunks, so a type with no
     table is reported after the last chunk: the hook fails with
