@@ -467,6 +467,8 @@ def main():
     ap.add_argument("--mixed-steps", type=int, default=200)
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3-cell large-box line")
     ap.add_argument("--no-half", action="store_true", help="skip the half-stencil comparison")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the drop-in hook measurement (its pipelined passes add small pair-kernel launches)")
     ap.add_argument("--half-steps", type=int, default=50)
     ap.add_argument("--large-steps", type=int, default=100)
     ap.add_argument("--serve-replicas", type=int, default=2000)
@@ -531,8 +533,9 @@ def main():
         log(f"bench: Newton's-third-law half stencil vs full stencil, FP64 and mixed, {args.half_steps} steps")
         out["half_stencil"] = half_stencil(w, args.half_steps, args.warmup)
 
-    log("bench: drop-in hook path on host records (PCIe included)")
-    out["dropin_pcie"] = dropin_hooks(w)
+    if not args.no_dropin:
+        log("bench: drop-in hook path on host records (PCIe included)")
+        out["dropin_pcie"] = dropin_hooks(w)
 
     if not args.no_mixed:
         log(f"bench: mixed vs FP64 on the defect box (0.5% vacancies, 0.2% interstitials, 1200 K), "
