@@ -272,6 +272,13 @@ __device__ __forceinline__ TileSite map_tile(const Geom &g, const Halo &h, int64
   return t;
 }
 
+// device index of the site thread tid takes in tile t (map_tile's t.i)
+__device__ __forceinline__ int64_t tile_site_index(const Geom &g, const TileSite &t, uint32_t tid) {
+  const uint32_t lane = tid & (kWave - 1), w = tid >> 6;
+  return dev_index(g, t.s, t.x0 + (int32_t)(lane & 7) + g.gxh, t.y0 + (int32_t)((lane >> 3) & 3) + g.gy,
+                   t.z0 + (int32_t)((w >> 1) * 2 + (lane >> 5)) + g.gz);
+}
+
 // LDS record idx -> device index of the ext site it mirrors, or -1 outside
 // the ghost-extended lattice.
 __device__ __forceinline__ int64_t tile_record_src(const Geom &g, const Halo &h, const TileSite &t, int32_t idx) {
