@@ -103,22 +103,28 @@ __device__ __forceinline__ double2 pair_swap(double2 v) { return make_double2(pa
 // A lane pair's two 32-byte records (two 16-byte halves each) gathered one
 // record per instruction: the first load brings the even lane's record (its
 // first half to the even lane, its second half to the odd lane), the second
-// load the odd lane's; pair_rec32_halves swaps the half each lane lacks.  A
-// gather then touches the cache lines of 32 records instead of 64.  Both
-// lanes of every pair must be active.
+// the odd lane's with the halves crossed (its second half to the even lane,
+// its first half to the odd lane).  Each lane then holds its own first half
+// in one of the two registers and the half its partner lacks in the other,
+// so a select and one swap complete both records.  A gather touches the
+// cache lines of 32 records instead of 64, and the two lanes of a pair read
+// adjacent 16-byte halves (pairing lanes l and l + 32 instead, with
+// v_permlane32_swap and no selects, measured 8-13 % slower: the partner
+// lanes' reads no longer fall together).  Both lanes of every pair must be
+// active.
 struct PairRec32 {
-  double2 e, o;  // this lane's half of the even lane's record, of the odd lane's
+  double2 e;  // the even lane's record: first half in the even lane, second half in the odd lane
+  double2 o;  // the odd lane's record: second half in the even lane, first half in the odd lane
 };
 __device__ __forceinline__ PairRec32 pair_rec32_load(const double2 *__restrict__ base, uint32_t rec) {
   const uint32_t h = threadIdx.x & 1u, rq = pair_swap(rec);
-  return {ld(base, 2 * (h ? rq : rec) + h), ld(base, 2 * (h ? rec : rq) + h)};
+  return {ld(base, 2 * (h ? rq : rec) + h), ld(base, 2 * (h ? rec : rq) + 1 - h)};
 }
 // this lane's record: a = first half, b = second half
 __device__ __forceinline__ void pair_rec32_halves(const PairRec32 &r, double2 &a, double2 &b) {
   const bool h = threadIdx.x & 1u;
-  const double2 y = pair_swap(h ? r.e : r.o);
-  a = h ? y : r.e;
-  b = h ? r.o : y;
+  a = h ? r.o : r.e;
+  b = pair_swap(h ? r.e : r.o);
 }
 
 
@@ -646,15 +652,16 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
             ul[u] = in[u] && tq[u] != ti;
             rhoq[u] = 0.0;
           }
-          // 48-byte force records gathered per lane pair: the first load
-          // brings pieces 0 and 1 of the even lane's record, the second its
-          // piece 2 and piece 0 of the odd lane's, the third the odd lane's
-          // pieces 1 and 2; a DPP swap of piece 1 completes both records
-          // (each gather touches the lines of 32 records in two of three
-          // loads).  The plain full-stencil pass issues them before the
-          // unlike queue, so its round trips overlap theirs; the passes that
-          // hold more registers (pair energies, the half stencil's slot
-          // stores) issue them after (issued first they spill there).
+          // 48-byte force records (pieces 0-2) gathered per lane pair: the
+          // first load brings each lane its own piece 0, the second pieces 1
+          // and 2 of the even lane's record (to the even and the odd lane),
+          // the third pieces 2 and 1 of the odd lane's; a select and one
+          // swap complete pieces 1 and 2 (each gather touches the lines of
+          // 32 records in two of three loads).  The plain full-stencil pass
+          // issues them before the unlike queue, so its round trips overlap
+          // theirs; the passes that hold more registers (pair energies, the
+          // half stencil's slot stores) issue them after (issued first they
+          // spill there).
           const uint32_t hl = threadIdx.x & 1u;
           double2 r0[kUnroll], r1[kUnroll], r2[kUnroll];
           auto gather_records = [&] {
@@ -662,9 +669,9 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
             for (int u = 0; u < kUnroll; u++) {
               const uint32_t rq = pair_swap(rp[u]);
               const uint32_t re = hl ? rq : rp[u], ro = hl ? rp[u] : rq;
-              r0[u] = ld(tf.fc, 3 * re + hl);
-              r1[u] = ld(tf.fc, hl ? 3 * ro : 3 * re + 2);
-              r2[u] = ld(tf.fc, 3 * ro + 1 + hl);
+              r0[u] = ld(tf.fc, 3 * rp[u]);
+              r1[u] = ld(tf.fc, 3 * re + 1 + hl);
+              r2[u] = ld(tf.fc, 3 * ro + 2 - hl);
             }
           };
           constexpr bool kEarly = !HALF && !WANT_E;
@@ -674,9 +681,9 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           if (!kEarly) gather_records();
 #pragma unroll
           for (int u = 0; u < kUnroll; u++) {
-            const double2 c1 = pair_swap(hl ? r0[u] : r2[u]);
-            fp[u] = fpair_f64_c<WANT_E>(tf, hl ? r1[u] : r0[u], c1, hl ? r2[u] : r1[u], rp[u], pr[u], rinv[u],
-                                              dfi, dfj[u], tq[u] != ti, rhoq[u], ce[u], phi[u]);
+            const double2 c2 = pair_swap(hl ? r1[u] : r2[u]);
+            fp[u] = fpair_f64_c<WANT_E>(tf, r0[u], hl ? r2[u] : r1[u], c2, rp[u], pr[u], rinv[u], dfi, dfj[u],
+                                        tq[u] != ti, rhoq[u], ce[u], phi[u]);
           }
         }
 #pragma unroll
