@@ -615,6 +615,11 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
         uint32_t tq[kUnroll];
         double dx[kUnroll], dy[kUnroll], dz[kUnroll], d2[kUnroll], dfj[kUnroll];
         bool in[kUnroll], any = false;
+        // entry k + u exists: tested on every FP64 trip (its loop's last trip
+        // is the partial one) and on the mixed pass's tail trip, as a plain
+        // scalar comparison (a select on the run-time tail flag was
+        // materialised per lane)
+        auto entry = [&](int u) { return PREC == 0 || tail ? k + u < nk : true; };
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
           const int32_t j = t.c + oc[u];
@@ -625,14 +630,14 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           dz[u] = czd.x - pzd.x;
           dfj[u] = pzd.y;
           d2[u] = dx[u] * dx[u] + dy[u] * dy[u] + dz[u] * dz[u];
-          in[u] = live && d2[u] < rc2 && (!tail || k + u < nk);
+          in[u] = live && d2[u] < rc2 && entry(u);
           any |= in[u];
         }
         if (__ballot(any) == 0) {
           if (HALF && t.active) {
 #pragma unroll
             for (int u = 0; u < kUnroll; u++)
-              if (!tail || k + u < nk) SL[(size_t)(k + u) * nl + q] = 0.0;
+              if (entry(u)) SL[(size_t)(k + u) * nl + q] = 0.0;
           }
           return;
         }
@@ -689,7 +694,10 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
             for (int u = 0; u < kUnroll; u++) {
               const uint32_t rq = pair_swap(rp[u]);
               const uint32_t re = hl ? rq : rp[u], ro = hl ? rp[u] : rq;
-              r0[u] = ld(tf.fc, 3 * rp[u]);
+              // (the 24-bit multiply keeps 3 rp off the quarter-rate 32-bit
+              // one; r1 and r2 stay in the form that keeps every load on a
+              // scalar base plus a 32-bit lane offset)
+              r0[u] = ld(tf.fc, __umul24(rp[u], 3u));
               r1[u] = ld(tf.fc, 3 * re + 1 + hl);
               r2[u] = ld(tf.fc, 3 * ro + 2 - hl);
             }
@@ -713,7 +721,7 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           ay = fma(dy[u], f, ay);
           az = fma(dz[u], f, az);
           if (WANT_E) ep += in[u] ? phi[u] : 0.0;
-          if (HALF && t.active && (!tail || k + u < nk)) SL[(size_t)(k + u) * nl + q] = f;
+          if (HALF && t.active && entry(u)) SL[(size_t)(k + u) * nl + q] = f;
         }
       };
       // a trip's stencil offsets are read one trip ahead (rows are padded
