@@ -677,16 +677,21 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
             ul[u] = in[u] && tq[u] != ti;
             rhoq[u] = 0.0;
           }
-          // 48-byte force records (pieces 0-2) gathered per lane pair: the
-          // first load brings each lane its own piece 0, the second pieces 1
-          // and 2 of the even lane's record (to the even and the odd lane),
-          // the third pieces 2 and 1 of the odd lane's; a select and one
-          // swap complete pieces 1 and 2 (each gather touches the lines of
-          // 32 records in two of three loads).  The plain full-stencil pass
-          // issues them before the unlike queue, so its round trips overlap
-          // theirs; the passes that hold more registers (pair energies, the
-          // half stencil's slot stores) issue them after (issued first they
-          // spill there).
+          // 48-byte force records (pieces 0-2) gathered per lane pair, each
+          // gather touching the lines of 32 records in two of three loads.
+          // Full stencil: the first load brings each lane its own piece 0,
+          // the second pieces 1 and 2 of the even lane's record (to the even
+          // and the odd lane), the third pieces 2 and 1 of the odd lane's; a
+          // select and one swap complete pieces 1 and 2.  Half stencil: the
+          // first load brings pieces 0 and 1 of the even lane's record, the
+          // second its piece 2 and piece 0 of the odd lane's, the third the
+          // odd lane's pieces 1 and 2, and a swap of piece 1 completes both
+          // (one select more per record, but the forward pass, which holds
+          // its slot stores, measured 4 % slower with the first form).  The
+          // plain full-stencil pass issues them before the unlike queue, so
+          // its round trips overlap theirs; the passes that hold more
+          // registers (pair energies, the half stencil's slot stores) issue
+          // them after (issued first they spill there).
           const uint32_t hl = threadIdx.x & 1u;
           double2 r0[kUnroll], r1[kUnroll], r2[kUnroll];
           auto gather_records = [&] {
@@ -694,12 +699,18 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
             for (int u = 0; u < kUnroll; u++) {
               const uint32_t rq = pair_swap(rp[u]);
               const uint32_t re = hl ? rq : rp[u], ro = hl ? rp[u] : rq;
-              // (the 24-bit multiply keeps 3 rp off the quarter-rate 32-bit
-              // one; r1 and r2 stay in the form that keeps every load on a
-              // scalar base plus a 32-bit lane offset)
-              r0[u] = ld(tf.fc, __umul24(rp[u], 3u));
-              r1[u] = ld(tf.fc, 3 * re + 1 + hl);
-              r2[u] = ld(tf.fc, 3 * ro + 2 - hl);
+              if constexpr (!HALF) {
+                // (the 24-bit multiply keeps 3 rp off the quarter-rate 32-bit
+                // one; r1 and r2 stay in the form that keeps every load on a
+                // scalar base plus a 32-bit lane offset)
+                r0[u] = ld(tf.fc, __umul24(rp[u], 3u));
+                r1[u] = ld(tf.fc, 3 * re + 1 + hl);
+                r2[u] = ld(tf.fc, 3 * ro + 2 - hl);
+              } else {
+                r0[u] = ld(tf.fc, 3 * re + hl);
+                r1[u] = ld(tf.fc, hl ? 3 * ro : 3 * re + 2);
+                r2[u] = ld(tf.fc, 3 * ro + 1 + hl);
+              }
             }
           };
           constexpr bool kEarly = !HALF && !WANT_E;
@@ -709,9 +720,18 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           if (!kEarly) gather_records();
 #pragma unroll
           for (int u = 0; u < kUnroll; u++) {
-            const double2 c2 = pair_swap(hl ? r1[u] : r2[u]);
-            fp[u] = fpair_f64_c<WANT_E>(tf, r0[u], hl ? r2[u] : r1[u], c2, rp[u], pr[u], rinv[u], dfi, dfj[u],
-                                        tq[u] != ti, rhoq[u], ce[u], phi[u]);
+            double2 c0, c1, c2;
+            if constexpr (!HALF) {
+              c0 = r0[u];
+              c1 = hl ? r2[u] : r1[u];
+              c2 = pair_swap(hl ? r1[u] : r2[u]);
+            } else {
+              c0 = hl ? r1[u] : r0[u];
+              c1 = pair_swap(hl ? r0[u] : r2[u]);
+              c2 = hl ? r2[u] : r1[u];
+            }
+            fp[u] = fpair_f64_c<WANT_E>(tf, c0, c1, c2, rp[u], pr[u], rinv[u], dfi, dfj[u], tq[u] != ti, rhoq[u], ce[u],
+                                        phi[u]);
           }
         }
 #pragma unroll
