@@ -161,7 +161,12 @@ int mmd_sync(mmd_ctx *ctx);
  * force pass) from a host AoS array of ext records, run the pass, and write
  * rho/df (or f) back into the local records, as the reference's streamed
  * passes do (src/double-buffer/rho_double_buffer_imp.cpp:81-121,
- * force_double_buffer_imp.cpp:78-126).  Synchronous. */
+ * force_double_buffer_imp.cpp:78-126).  Synchronous.  A record whose type
+ * has no table fails the call with MMD_E_INVALID; on pinned records
+ * (mmd_host_alloc), which are processed in z-chunks with the outputs written
+ * as each chunk's rows finish, some local records' output fields may already
+ * hold this pass's values when it fails (other host memory is left
+ * unchanged). */
 int mmd_eam_rho_calc_aos(mmd_ctx *ctx, void *atoms, const mmd_aos_layout *lay, double cutoff);
 int mmd_eam_df_calc_aos(mmd_ctx *ctx, void *atoms, const mmd_aos_layout *lay);
 int mmd_eam_force_calc_aos(mmd_ctx *ctx, void *atoms, const mmd_aos_layout *lay, double cutoff);
