@@ -69,8 +69,7 @@ struct Seg {
 // capped at n - 1 before the conversion, which keeps any x -- the far point of
 // a vacancy included -- in the converter's range and the fraction p - m at
 // most 1, since m = n - 2 wherever p > n - 2.
-__device__ __forceinline__ Seg seg_f64(double x, double x0, double inv_dx, int n) {
-  double p = (x - x0) * inv_dx;
+__device__ __forceinline__ Seg seg_clamp(double p, int n) {
   p = fmin(fmax(p, 0.0), (double)(n - 1));
   int m = (int)p;
   m = min(m, n - 2);
@@ -78,6 +77,17 @@ __device__ __forceinline__ Seg seg_f64(double x, double x0, double inv_dx, int n
   s.p = p - (double)m;
   s.m = m;
   return s;
+}
+__device__ __forceinline__ Seg seg_f64(double x, double x0, double inv_dx, int n) {
+  return seg_clamp((x - x0) * inv_dx, n);
+}
+// The rho passes' form: p = (x - x0) / dx from one FMA with the start folded
+// into the addend, exactly the two-step rounding where x0 = 0 (the tables'
+// usual start) and within an ulp of it otherwise; one instruction fewer per
+// pair (the force pass keeps the two-step form: the folded constant's
+// register made it spill).
+__device__ __forceinline__ Seg seg_f64_fma(double x, double x0, double inv_dx, int n) {
+  return seg_clamp(fma(x, inv_dx, -x0 * inv_dx), n);
 }
 
 struct SegS {
@@ -156,7 +166,7 @@ __device__ __forceinline__ void pair_rec32_halves(const PairRec32 &r, double2 &a
 // FP64 r-keyed spline: rho_{tj}(r), r^2 = d2
 __device__ __forceinline__ double rho_f64(const TabF64 &tf, int tj, double d2) {
   const double r = sqrt64(d2);
-  const Seg sg = seg_f64(r, tf.r0, tf.inv_dr, tf.n_r);
+  const Seg sg = seg_f64_fma(r, tf.r0, tf.inv_dr, tf.n_r);
   return spl_value(tf.rho, (uint32_t)(tj * tf.n_r + sg.m), sg.p);
 }
 
@@ -361,7 +371,7 @@ __device__ __forceinline__ void trip_rho_f64(const TabF64 &tf, const uint32_t *t
 #pragma unroll
   for (int u = 0; u < U; u++) {
     const double r = sqrt64(d2[u]);
-    const Seg sg = seg_f64(r, tf.r0, tf.inv_dr, tf.n_r);
+    const Seg sg = seg_f64_fma(r, tf.r0, tf.inv_dr, tf.n_r);
     p[u] = sg.p;
     rec[u] = __umul24(tj[u], (uint32_t)tf.n_r) + (uint32_t)sg.m;
   }
@@ -396,13 +406,16 @@ __device__ __forceinline__ void trip_rho_f64(const TabF64 &tf, const uint32_t *t
 // LDS tile layout (n records): lxy = (x, y) [n], then lzd = (z, df) [n] for
 // the force pass or lz = z [n] for the rho pass, then the int8 types [n].
 // A vacancy's staged coordinates: any occupied site lies beyond the cutoff
-// of it (coordinates are within 1e6 A of the origin).
+// of it (coordinates are within 1e6 A of the origin), so a staged record is
+// occupied exactly when its x is not kFarPos.  Types are staged as table
+// rows, clamped to [0, tmax] (a vacancy's -1 to 0), so the pair loops index
+// the tables with them directly; the cutoff test screens the vacancies.
 constexpr double kFarPos = 1e7;
 
 template <bool WITH_DF>
 __device__ __forceinline__ void stage_tile(const Geom &g, const Halo &h, const TileSite &t, int64_t lb,
                                            const double4 *__restrict__ P, const double *__restrict__ DF,
-                                           double2 *lxy, void *lz_or_lzd, int8_t *lt) {
+                                           double2 *lxy, void *lz_or_lzd, int8_t *lt, int tmax) {
   __syncthreads();
   if (lb < g.n_tiles) {
     for (int32_t idx = threadIdx.x; idx < h.n; idx += kBlock) {
@@ -419,7 +432,7 @@ __device__ __forceinline__ void stage_tile(const Geom &g, const Halo &h, const T
         static_cast<double2 *>(lz_or_lzd)[idx] = make_double2(q.z, f);
       else
         static_cast<double *>(lz_or_lzd)[idx] = q.z;
-      lt[idx] = (int8_t)q.w;
+      lt[idx] = (int8_t)min(max((int)q.w, 0), tmax);
     }
   }
   __syncthreads();
@@ -442,25 +455,23 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
   double2 *lxy = lds;
   double *lz = reinterpret_cast<double *>(lds + h.n);
   int8_t *lt = reinterpret_cast<int8_t *>(lz + h.n);
-  stage_tile<false>(g, h, t, lb, P, nullptr, lxy, lz, lt);
+  stage_tile<false>(g, h, t, lb, P, nullptr, lxy, lz, lt, tf.nt - 1);
   double e_loc = 0.0;
   // every lane of a wave with an active site runs the stencil loop, pairs
   // masked on lanes without an occupied site (as in force_block)
   if (__ballot(t.active) != 0) {
     const double2 cxy = lxy[t.c];
     const double cz = lz[t.c];
-    const int tc = lt[t.c];
-    const bool live = t.active && tc >= 0;
-    const int t0 = live ? tc : 0;
+    const bool live = t.active && cxy.x != kFarPos;
+    const int t0 = lt[t.c];
     {
       const int32_t *o = off + t.s * kMaxStencil;
       const int nk = t.s ? nk_odd : nk_even;
       constexpr bool BOTH = PREC == 1 && LEAN && WANT_E;  // dynamics from lean records, energy from cubic ones
       double acc = 0.0, acc_e = 0.0;
-      // vacancies sit at the far point, so the cutoff test screens them;
-      // their type byte (0xff) is clamped to a valid table row
+      // vacancies sit at the far point, so the cutoff test screens them
+      // (their staged type is row 0)
       const uint8_t *ltu = reinterpret_cast<const uint8_t *>(lt);
-      const uint32_t tmax = (uint32_t)(tf.nt - 1);
       // a trip's neighbour records, read from the LDS tile
       struct Nbr {
         double2 xy;
@@ -483,7 +494,7 @@ __device__ __forceinline__ void rho_df_block(const Geom &g, const Halo &h, int64
 #pragma unroll
         for (int u = 0; u < kUnrollRho; u++) {
           const double2 pxy = nb[u].xy;
-          tq[u] = min(nb[u].t, tmax);
+          tq[u] = nb[u].t;
           const double dx = cxy.x - pxy.x, dy = cxy.y - pxy.y, dz = cz - nb[u].z;
           d2[u] = dx * dx + dy * dy + dz * dz;
           in[u] = live && d2[u] < rc2 && (!tail || k + u < nk);
@@ -585,17 +596,16 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
   char *qw = reinterpret_cast<char *>(lds) + tile_queue_offset(h) + (threadIdx.x >> 6) * kSlotWaveBytes;
   uint32_t *qr = reinterpret_cast<uint32_t *>(qw);
   double *qx = reinterpret_cast<double *>(qw + kSlots * sizeof(uint32_t));
-  stage_tile<true>(g, h, t, lb, P, DF, lxy, lzd, lt);
+  stage_tile<true>(g, h, t, lb, P, DF, lxy, lzd, lt, tf.nt - 1);
   double e_loc = 0.0;
   // Every lane of a wave that holds an active site runs the stencil loop:
   // lanes without an occupied site (a vacancy, a cell past a ragged box edge)
-  // take part with all their pairs masked, so the mixed path's lane-pair
-  // record exchange always finds both lanes of a pair.
+  // take part with all their pairs masked, so the lane-pair record exchanges
+  // always find both lanes of a pair.
   if (__ballot(t.active) != 0) {
     const double2 cxy = lxy[t.c], czd = lzd[t.c];
-    const int tc = lt[t.c];
-    const bool live = t.active && tc >= 0;
-    const int t0 = live ? tc : 0;
+    const bool live = t.active && cxy.x != kFarPos;
+    const int t0 = lt[t.c];
     const int nk = t.s ? nk_odd : nk_even;
     const size_t nl = (size_t)g.n_local;
     const uint32_t q = HALF ? local_q(g, t.s, t.x0 + t.tx, t.y0 + t.ty, t.z0 + t.tz) : 0u;
@@ -609,7 +619,7 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
       double ax = 0.0, ay = 0.0, az = 0.0, ep = 0.0;
       // vacancies sit at the far point (see rho_df_block)
       const uint8_t *ltu = reinterpret_cast<const uint8_t *>(lt);
-      const uint32_t tmax = (uint32_t)(tf.nt - 1), ti = (uint32_t)t0;
+      const uint32_t ti = (uint32_t)t0;
       const uint32_t ra_i = ti * (uint32_t)tm.n_s, row_i = ra_i * (uint32_t)tm.nt;
       auto trip = [&](int k, bool tail, const int32_t *oc) {
         uint32_t tq[kUnroll];
@@ -624,7 +634,7 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
         for (int u = 0; u < kUnroll; u++) {
           const int32_t j = t.c + oc[u];
           const double2 pxy = lxy[j], pzd = lzd[j];
-          tq[u] = min((uint32_t)ltu[j], tmax);
+          tq[u] = ltu[j];
           dx[u] = cxy.x - pxy.x;
           dy[u] = cxy.y - pxy.y;
           dz[u] = czd.x - pzd.x;
