@@ -20,6 +20,24 @@ struct Range {
   int begin, end, stride;
 };
 
+// Work items and replica descriptors are read through the constant address
+// space: the host writes them before a launch and no kernel writes them, so
+// the compiler may load them with scalar loads and reload them where it
+// needs them, as it does kernel arguments.  Through plain global pointers an
+// item's loads follow the stores of the block's previous item, so the
+// compiler cannot prove them unclobbered and keeps the ~40 descriptor words
+// in vector registers, which pushed the force bodies into register spills.
+template <class T>
+__device__ __forceinline__ T load_const(const T *p) {
+  static_assert(sizeof(T) % 4 == 0, "read as 32-bit words");
+  const uint32_t __attribute__((address_space(4))) *q = (const uint32_t __attribute__((address_space(4))) *)p;
+  T out;
+  uint32_t *o = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); k++) o[k] = q[k];
+  return out;
+}
+
 __device__ __forceinline__ Range xcd_range(int n) {
   const int xcd = blockIdx.x % kXcds, per_block = gridDim.x / kXcds;
   const int per = (n + kXcds - 1) / kXcds;
@@ -35,9 +53,11 @@ __global__ __launch_bounds__(kBlock) void kb_kick(BatchLists L, Masses M, double
   __shared__ double lds[kWavesPerBlock];
   const Range r = xcd_range(*L.n_site_items);
   for (int it = r.begin; it < r.end; it += r.stride) {
-    const int2 w = L.site_items[it];
-    const BatchDom &d = L.dom[w.x];
-    kick_block<DRIFT, WANT_KE>(d.g, w.y, M, h, dt, d.pos, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.type, d.e_ke, lds);
+    const int2 w = load_const(L.site_items + it);
+    const BatchDom *d = L.dom + w.x;
+    kick_block<DRIFT, WANT_KE>(load_const(&d->g), w.y, M, h, dt, load_const(&d->pos), load_const(&d->vx),
+                               load_const(&d->vy), load_const(&d->vz), load_const(&d->fx), load_const(&d->fy),
+                               load_const(&d->fz), load_const(&d->type), load_const(&d->e_ke), lds);
   }
 }
 
@@ -46,18 +66,21 @@ __global__ __launch_bounds__(kBlock) void kb_kick_fused(BatchLists L, Masses M, 
   __shared__ double lds[kWavesPerBlock];
   const Range r = xcd_range(*L.n_site_items);
   for (int it = r.begin; it < r.end; it += r.stride) {
-    const int2 w = L.site_items[it];
-    const BatchDom &d = L.dom[w.x];
-    kick_fused_block<WANT_KE>(d.g, w.y, M, h, dt, d.pos, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz, d.type, d.e_ke, lds);
+    const int2 w = load_const(L.site_items + it);
+    const BatchDom *d = L.dom + w.x;
+    kick_fused_block<WANT_KE>(load_const(&d->g), w.y, M, h, dt, load_const(&d->pos), load_const(&d->vx),
+                              load_const(&d->vy), load_const(&d->vz), load_const(&d->fx), load_const(&d->fy),
+                              load_const(&d->fz), load_const(&d->type), load_const(&d->e_ke), lds);
   }
 }
 
 __global__ __launch_bounds__(kBlock) void kb_ghost(BatchLists L, uint32_t fields) {
   const Range r = xcd_range(*L.n_ext_items);
   for (int it = r.begin; it < r.end; it += r.stride) {
-    const int2 w = L.ext_items[it];
-    const BatchDom &d = L.dom[w.x];
-    ghost_site(d.g, (uint32_t)w.y * kBlock + threadIdx.x, d.L[0], d.L[1], d.L[2], d.pos, d.type, d.df, fields);
+    const int2 w = load_const(L.ext_items + it);
+    const BatchDom *d = L.dom + w.x;
+    ghost_site(load_const(&d->g), (uint32_t)w.y * kBlock + threadIdx.x, load_const(&d->L[0]), load_const(&d->L[1]),
+               load_const(&d->L[2]), load_const(&d->pos), load_const(&d->type), load_const(&d->df), fields);
   }
 }
 
@@ -67,10 +90,11 @@ __global__ __launch_bounds__(kBlock) void kb_rho_df(BatchLists L, Halo h, TabF64
   __shared__ double red[kWavesPerBlock];
   const Range r = xcd_range(*L.n_tile_items);
   for (int it = r.begin; it < r.end; it += r.stride) {
-    const int2 w = L.tile_items[it];
-    const BatchDom &d = L.dom[w.x];
-    rho_df_block<PREC, WANT_E, LEAN>(d.g, h, w.y, d.off, d.nk_even, d.nk_odd, d.pos, d.rho, d.df, tf, tm, rc2,
-                                     d.e_rho, lds_tile, red);
+    const int2 w = load_const(L.tile_items + it);
+    const BatchDom *d = L.dom + w.x;
+    rho_df_block<PREC, WANT_E, LEAN>(load_const(&d->g), h, w.y, load_const(&d->off), load_const(&d->nk_even),
+                                     load_const(&d->nk_odd), load_const(&d->pos), load_const(&d->rho),
+                                     load_const(&d->df), tf, tm, rc2, load_const(&d->e_rho), lds_tile, red);
   }
 }
 
@@ -80,10 +104,12 @@ __global__ __launch_bounds__(kBlock, kForceWavesPerSimd) void kb_force(BatchList
   __shared__ double red[kWavesPerBlock];
   const Range r = xcd_range(*L.n_tile_items);
   for (int it = r.begin; it < r.end; it += r.stride) {
-    const int2 w = L.tile_items[it];
-    const BatchDom &d = L.dom[w.x];
-    force_block<PREC, WANT_E>(d.g, h, w.y, d.off, d.nk_even, d.nk_odd, d.pos, d.df, d.fx, d.fy, d.fz, tf, tm, rc2,
-                              d.e_force, lds_tile, red);
+    const int2 w = load_const(L.tile_items + it);
+    const BatchDom *d = L.dom + w.x;
+    force_block<PREC, WANT_E>(load_const(&d->g), h, w.y, load_const(&d->off), load_const(&d->nk_even),
+                              load_const(&d->nk_odd), load_const(&d->pos), load_const(&d->df), load_const(&d->fx),
+                              load_const(&d->fy), load_const(&d->fz), tf, tm, rc2, load_const(&d->e_force), lds_tile,
+                              red);
   }
 }
 
