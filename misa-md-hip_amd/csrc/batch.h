@@ -42,20 +42,26 @@ struct BatchLists {
   const int *n_reps;
 };
 
-// blocks per launch: a multiple of the 8 XCDs; one block per item when it covers the list
+// blocks per launch: a multiple of the 8 XCDs, one block per item
 inline int batch_blocks(int64_t items) { return (int)((std::max<int64_t>(items, 1) + 7) / 8 * 8); }
 
-hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, int blocks, const Masses &M, double h, double dt,
+// The launchers take the number of items their grid covers, one block per
+// item (batch_blocks(items) blocks): at least the count of the list they
+// walk, which the kernels read from device memory.  Serving step launches
+// pass the lists' capacities, so one captured graph serves every mix that
+// fits them.
+
+hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, int64_t items, const Masses &M, double h, double dt,
                              bool drift, bool want_ke);
 // a step's closing half kick fused with the next step's opening kick and drift
-hipError_t launch_batch_kick_fused(hipStream_t st, const BatchLists &L, int blocks, const Masses &M, double h,
+hipError_t launch_batch_kick_fused(hipStream_t st, const BatchLists &L, int64_t items, const Masses &M, double h,
                                    double dt, bool want_ke);
-hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, int blocks, uint32_t fields);
+hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, int64_t items, uint32_t fields);
 // lean (mixed, MD steps): rho and df from the quadratic records; with want_e the embedding energy
 // from a second sum over the cubic value records
-hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, int blocks, const Halo &h, const TabF64 &tf,
+hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, int64_t items, const Halo &h, const TabF64 &tf,
                                const TabMixed &tm, double rc2, int prec, bool want_e, bool lean);
-hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, int blocks, const Halo &h, const TabF64 &tf,
+hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, int64_t items, const Halo &h, const TabF64 &tf,
                               const TabMixed &tm, double rc2, int prec, bool want_e);
 hipError_t batch_kernels_allow_lds(size_t bytes);
 // per-replica {potential, kinetic} -> energy[2 * which], which = 0 (step) or 1 (admission)

@@ -3,10 +3,10 @@
 // round-robin, so blockIdx.x % 8 names the XCD), so the blocks of an XCD take
 // neighbouring blocks of the same replica and share its halo in L2.  The
 // item counts are read from device memory, so a grid of fixed size serves
-// any list: the serving loop's step launches size their grids to the lists'
-// capacities (blocks past the count exit at once) and its priming launches
-// one block per item; a grid smaller than the list makes each block walk its
-// XCD's range with a stride.
+// any list up to the item capacity it was sized for: the launchers give one
+// block per item of that capacity (the serving loop's step launches size it
+// to the lists' capacities, and blocks past the count exit at once; its
+// priming launches to the item count).
 #include "batch.h"
 #include "eam_body.h"
 #include "kernels.h"
@@ -16,17 +16,22 @@ namespace mmd {
 
 namespace {
 
-struct Range {
-  int begin, end, stride;
-};
+// This block's item of a list of n: the XCD's contiguous range holds
+// ceil(n / 8) items and the XCD gridDim.x / 8 >= ceil(n / 8) blocks (the
+// launchers' grids cover the lists' capacities), so every item has a block
+// and no block has two; -1 for a block past the count.
+__device__ __forceinline__ int xcd_item(int n) {
+  const int xcd = blockIdx.x % kXcds, per = (n + kXcds - 1) / kXcds;
+  const int it = xcd * per + (int)(blockIdx.x / kXcds);
+  return it < min(n, (xcd + 1) * per) ? it : -1;
+}
 
 // Work items and replica descriptors are read through the constant address
 // space: the host writes them before a launch and no kernel writes them, so
-// the compiler may load them with scalar loads and reload them where it
-// needs them, as it does kernel arguments.  Through plain global pointers an
-// item's loads follow the stores of the block's previous item, so the
-// compiler cannot prove them unclobbered and keeps the ~40 descriptor words
-// in vector registers, which pushed the force bodies into register spills.
+// the compiler loads them with scalar loads.  Through plain global pointers
+// it cannot prove them unwritten by the kernel's own stores and keeps the
+// ~40 descriptor words in vector registers, which pushed the force bodies
+// into register spills.
 template <class T>
 __device__ __forceinline__ T load_const(const T *p) {
   static_assert(sizeof(T) % 4 == 0, "read as 32-bit words");
@@ -38,21 +43,11 @@ __device__ __forceinline__ T load_const(const T *p) {
   return out;
 }
 
-__device__ __forceinline__ Range xcd_range(int n) {
-  const int xcd = blockIdx.x % kXcds, per_block = gridDim.x / kXcds;
-  const int per = (n + kXcds - 1) / kXcds;
-  Range r;
-  r.begin = xcd * per + (int)(blockIdx.x / kXcds);
-  r.end = min(n, (xcd + 1) * per);
-  r.stride = per_block;
-  return r;
-}
-
 template <bool DRIFT, bool WANT_KE>
 __global__ __launch_bounds__(kBlock) void kb_kick(BatchLists L, Masses M, double h, double dt) {
   __shared__ double lds[kWavesPerBlock];
-  const Range r = xcd_range(*L.n_site_items);
-  for (int it = r.begin; it < r.end; it += r.stride) {
+  const int it = xcd_item(*L.n_site_items);
+  if (it >= 0) {
     const int2 w = load_const(L.site_items + it);
     const BatchDom *d = L.dom + w.x;
     kick_block<DRIFT, WANT_KE>(load_const(&d->g), w.y, M, h, dt, load_const(&d->pos), load_const(&d->vx),
@@ -64,8 +59,8 @@ __global__ __launch_bounds__(kBlock) void kb_kick(BatchLists L, Masses M, double
 template <bool WANT_KE>
 __global__ __launch_bounds__(kBlock) void kb_kick_fused(BatchLists L, Masses M, double h, double dt) {
   __shared__ double lds[kWavesPerBlock];
-  const Range r = xcd_range(*L.n_site_items);
-  for (int it = r.begin; it < r.end; it += r.stride) {
+  const int it = xcd_item(*L.n_site_items);
+  if (it >= 0) {
     const int2 w = load_const(L.site_items + it);
     const BatchDom *d = L.dom + w.x;
     kick_fused_block<WANT_KE>(load_const(&d->g), w.y, M, h, dt, load_const(&d->pos), load_const(&d->vx),
@@ -75,8 +70,8 @@ __global__ __launch_bounds__(kBlock) void kb_kick_fused(BatchLists L, Masses M, 
 }
 
 __global__ __launch_bounds__(kBlock) void kb_ghost(BatchLists L, uint32_t fields) {
-  const Range r = xcd_range(*L.n_ext_items);
-  for (int it = r.begin; it < r.end; it += r.stride) {
+  const int it = xcd_item(*L.n_ext_items);
+  if (it >= 0) {
     const int2 w = load_const(L.ext_items + it);
     const BatchDom *d = L.dom + w.x;
     ghost_site(load_const(&d->g), (uint32_t)w.y * kBlock + threadIdx.x, load_const(&d->L[0]), load_const(&d->L[1]),
@@ -88,8 +83,8 @@ template <int PREC, bool WANT_E, bool LEAN>
 __global__ __launch_bounds__(kBlock) void kb_rho_df(BatchLists L, Halo h, TabF64 tf, TabMixed tm, double rc2) {
   extern __shared__ double2 lds_tile[];
   __shared__ double red[kWavesPerBlock];
-  const Range r = xcd_range(*L.n_tile_items);
-  for (int it = r.begin; it < r.end; it += r.stride) {
+  const int it = xcd_item(*L.n_tile_items);
+  if (it >= 0) {
     const int2 w = load_const(L.tile_items + it);
     const BatchDom *d = L.dom + w.x;
     rho_df_block<PREC, WANT_E, LEAN>(load_const(&d->g), h, w.y, load_const(&d->off), load_const(&d->nk_even),
@@ -102,8 +97,8 @@ template <int PREC, bool WANT_E>
 __global__ __launch_bounds__(kBlock, kForceWavesPerSimd) void kb_force(BatchLists L, Halo h, TabF64 tf, TabMixed tm, double rc2) {
   extern __shared__ double2 lds_tile[];
   __shared__ double red[kWavesPerBlock];
-  const Range r = xcd_range(*L.n_tile_items);
-  for (int it = r.begin; it < r.end; it += r.stride) {
+  const int it = xcd_item(*L.n_tile_items);
+  if (it >= 0) {
     const int2 w = load_const(L.tile_items + it);
     const BatchDom *d = L.dom + w.x;
     force_block<PREC, WANT_E>(load_const(&d->g), h, w.y, load_const(&d->off), load_const(&d->nk_even),
@@ -140,9 +135,9 @@ __global__ __launch_bounds__(kBlock) void kb_energy(BatchLists L, int which) {
 
 }  // namespace
 
-hipError_t launch_batch_kick_fused(hipStream_t st, const BatchLists &L, int blocks, const Masses &M, double h,
+hipError_t launch_batch_kick_fused(hipStream_t st, const BatchLists &L, int64_t items, const Masses &M, double h,
                                    double dt, bool want_ke) {
-  const dim3 grid(blocks), block(kBlock);
+  const dim3 grid(batch_blocks(items)), block(kBlock);
   if (want_ke)
     kb_kick_fused<true><<<grid, block, 0, st>>>(L, M, h, dt);
   else
@@ -150,9 +145,9 @@ hipError_t launch_batch_kick_fused(hipStream_t st, const BatchLists &L, int bloc
   return hipGetLastError();
 }
 
-hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, int blocks, const Masses &M, double h, double dt,
+hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, int64_t items, const Masses &M, double h, double dt,
                              bool drift, bool want_ke) {
-  const dim3 grid(blocks), block(kBlock);
+  const dim3 grid(batch_blocks(items)), block(kBlock);
   if (drift) {
     if (want_ke)
       kb_kick<true, true><<<grid, block, 0, st>>>(L, M, h, dt);
@@ -167,14 +162,14 @@ hipError_t launch_batch_kick(hipStream_t st, const BatchLists &L, int blocks, co
   return hipGetLastError();
 }
 
-hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, int blocks, uint32_t fields) {
-  kb_ghost<<<dim3(blocks), dim3(kBlock), 0, st>>>(L, fields);
+hipError_t launch_batch_ghost(hipStream_t st, const BatchLists &L, int64_t items, uint32_t fields) {
+  kb_ghost<<<dim3(batch_blocks(items)), dim3(kBlock), 0, st>>>(L, fields);
   return hipGetLastError();
 }
 
-hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, int blocks, const Halo &h, const TabF64 &tf,
+hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, int64_t items, const Halo &h, const TabF64 &tf,
                                const TabMixed &tm, double rc2, int prec, bool want_e, bool lean) {
-  const dim3 grid(blocks), block(kBlock);
+  const dim3 grid(batch_blocks(items)), block(kBlock);
   const size_t sh = tile_lds_bytes(h, false);
   if (prec == 0) {
     if (want_e)
@@ -194,9 +189,9 @@ hipError_t launch_batch_rho_df(hipStream_t st, const BatchLists &L, int blocks, 
   return hipGetLastError();
 }
 
-hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, int blocks, const Halo &h, const TabF64 &tf,
+hipError_t launch_batch_force(hipStream_t st, const BatchLists &L, int64_t items, const Halo &h, const TabF64 &tf,
                               const TabMixed &tm, double rc2, int prec, bool want_e) {
-  const dim3 grid(blocks), block(kBlock);
+  const dim3 grid(batch_blocks(items)), block(kBlock);
   const size_t sh = tile_lds_bytes(h, true);
   if (prec == 0) {
     if (want_e)

@@ -228,7 +228,7 @@ struct ListBuf {
   size_t cap_site = 0, cap_tile = 0, cap_ext = 0;
   int *h_reps = nullptr, *d_reps = nullptr;  // [max_replicas]
   int *h_n = nullptr, *d_n = nullptr;        // {n_site, n_tile, n_ext, n_reps}
-  int g_site = 8, g_tile = 8, g_ext = 8;     // blocks per launch: one per item
+  int64_t k_site = 0, k_tile = 0, k_ext = 0;  // items each launch covers, one block per item
 
   BatchLists lists(const BatchDom *dom) const {
     return {dom, d_site, d_n, d_tile, d_n + 1, d_ext, d_n + 2, d_reps, d_n + 3};
@@ -435,11 +435,9 @@ int fill_lists(mmd_server *S, ListBuf &b, const std::vector<int> &reps, bool is_
   }
   b.h_n[0] = (int)n_site, b.h_n[1] = (int)n_tile, b.h_n[2] = (int)n_ext, b.h_n[3] = (int)reps.size();
   if (is_step) {  // grids of the lists' capacities: the captured graphs stay valid for any mix that fits them
-    b.g_site = batch_blocks((int64_t)b.cap_site), b.g_tile = batch_blocks((int64_t)b.cap_tile),
-    b.g_ext = batch_blocks((int64_t)b.cap_ext);
-  } else {  // priming: direct launches, one block per item
-    b.g_site = batch_blocks((int64_t)n_site), b.g_tile = batch_blocks((int64_t)n_tile),
-    b.g_ext = batch_blocks((int64_t)n_ext);
+    b.k_site = (int64_t)b.cap_site, b.k_tile = (int64_t)b.cap_tile, b.k_ext = (int64_t)b.cap_ext;
+  } else {  // priming: direct launches over the items
+    b.k_site = (int64_t)n_site, b.k_tile = (int64_t)n_tile, b.k_ext = (int64_t)n_ext;
   }
   if (n_site) HIP_TRY(hipMemcpyAsync(b.d_site, b.h_site, n_site * sizeof(int2), hipMemcpyHostToDevice, S->cs));
   if (n_tile) HIP_TRY(hipMemcpyAsync(b.d_tile, b.h_tile, n_tile * sizeof(int2), hipMemcpyHostToDevice, S->cs));
@@ -458,19 +456,19 @@ int step_passes(mmd_server *S, const BatchLists &L, const ListBuf &B, bool fused
   const Masses &M = S->tab->masses;
   const int prec = S->o.precision;
   if (fused)
-    HIP_TRY(launch_batch_kick_fused(S->cs, L, B.g_site, M, h, dt, false));
+    HIP_TRY(launch_batch_kick_fused(S->cs, L, B.k_site, M, h, dt, false));
   else
-    HIP_TRY(launch_batch_kick(S->cs, L, B.g_site, M, h, dt, true, false));
-  HIP_TRY(launch_batch_ghost(S->cs, L, B.g_ext, MMD_F_X));
-  HIP_TRY(launch_batch_rho_df(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e, true));
-  HIP_TRY(launch_batch_ghost(S->cs, L, B.g_ext, MMD_F_DF));
-  HIP_TRY(launch_batch_force(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e));
+    HIP_TRY(launch_batch_kick(S->cs, L, B.k_site, M, h, dt, true, false));
+  HIP_TRY(launch_batch_ghost(S->cs, L, B.k_ext, MMD_F_X));
+  HIP_TRY(launch_batch_rho_df(S->cs, L, B.k_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e, true));
+  HIP_TRY(launch_batch_ghost(S->cs, L, B.k_ext, MMD_F_DF));
+  HIP_TRY(launch_batch_force(S->cs, L, B.k_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, want_e));
   return MMD_OK;
 }
 
 // the chunk's closing half kick and the energies of its last step
 int step_close(mmd_server *S, const BatchLists &L, const ListBuf &B) {
-  HIP_TRY(launch_batch_kick(S->cs, L, B.g_site, S->tab->masses, 0.5 * S->o.dt, S->o.dt, false, true));
+  HIP_TRY(launch_batch_kick(S->cs, L, B.k_site, S->tab->masses, 0.5 * S->o.dt, S->o.dt, false, true));
   HIP_TRY(launch_batch_energy(S->cs, L, 0));
   return MMD_OK;
 }
@@ -539,11 +537,11 @@ struct GpuExec final : Executor {
     const double rc2 = S->o.cutoff * S->o.cutoff;
     const int prec = S->o.precision;
     const ListBuf &B = S->prime;
-    HIP_TRY(launch_batch_ghost(S->cs, L, B.g_ext, MMD_F_X));
-    HIP_TRY(launch_batch_rho_df(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, true, true));
-    HIP_TRY(launch_batch_ghost(S->cs, L, B.g_ext, MMD_F_DF));
-    HIP_TRY(launch_batch_force(S->cs, L, B.g_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, true));
-    HIP_TRY(launch_batch_kick(S->cs, L, B.g_site, S->tab->masses, 0.0, S->o.dt, false, true));
+    HIP_TRY(launch_batch_ghost(S->cs, L, B.k_ext, MMD_F_X));
+    HIP_TRY(launch_batch_rho_df(S->cs, L, B.k_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, true, true));
+    HIP_TRY(launch_batch_ghost(S->cs, L, B.k_ext, MMD_F_DF));
+    HIP_TRY(launch_batch_force(S->cs, L, B.k_tile, S->halo, S->tab->tf, S->tab->tm, rc2, prec, true));
+    HIP_TRY(launch_batch_kick(S->cs, L, B.k_site, S->tab->masses, 0.0, S->o.dt, false, true));
     HIP_TRY(launch_batch_energy(S->cs, L, 1));
     return fetch_energies(reps);
   }
