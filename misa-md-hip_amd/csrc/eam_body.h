@@ -122,6 +122,14 @@ __device__ __forceinline__ double pair_swap(double v) {
   return __builtin_bit_cast(double, (hi << 32) | lo);
 }
 __device__ __forceinline__ double2 pair_swap(double2 v) { return make_double2(pair_swap(v.x), pair_swap(v.y)); }
+// v of the pair's even / odd lane, in both lanes: quad permutations
+// [0, 0, 2, 2] / [1, 1, 3, 3] (a lane-dependent pick with no select)
+__device__ __forceinline__ uint32_t pair_even(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xA0, 0xF, 0xF, false);
+}
+__device__ __forceinline__ uint32_t pair_odd(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xF5, 0xF, 0xF, false);
+}
 
 // A lane pair's two 32-byte records (two 16-byte halves each) gathered one
 // record per instruction: the first load brings the even lane's record (its
@@ -140,8 +148,8 @@ struct PairRec32 {
   double2 o;  // the odd lane's record: second half in the even lane, first half in the odd lane
 };
 __device__ __forceinline__ PairRec32 pair_rec32_load(const double2 *__restrict__ base, uint32_t rec) {
-  const uint32_t h = threadIdx.x & 1u, rq = pair_swap(rec);
-  return {ld(base, 2 * (h ? rq : rec) + h), ld(base, 2 * (h ? rec : rq) + 1 - h)};
+  const uint32_t h = threadIdx.x & 1u;
+  return {ld(base, 2 * pair_even(rec) + h), ld(base, 2 * pair_odd(rec) + 1 - h)};
 }
 // this lane's record: a = first half, b = second half
 __device__ __forceinline__ void pair_rec32_halves(const PairRec32 &r, double2 &a, double2 &b) {
@@ -707,8 +715,7 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           auto gather_records = [&] {
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) {
-              const uint32_t rq = pair_swap(rp[u]);
-              const uint32_t re = hl ? rq : rp[u], ro = hl ? rp[u] : rq;
+              const uint32_t re = pair_even(rp[u]), ro = pair_odd(rp[u]);
               if constexpr (!HALF) {
                 // (the 24-bit multiply keeps 3 rp off the quarter-rate 32-bit
                 // one; r1 and r2 stay in the form that keeps every load on a
