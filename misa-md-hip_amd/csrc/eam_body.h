@@ -185,16 +185,19 @@ __device__ __forceinline__ double rho_deriv_f64(const TabF64 &tf, uint32_t rec, 
   return (a.x * p + a.y) * p + c;
 }
 
-// phi_ab at fraction p of record rp from the derivative coefficients of its
-// force record (e0 p + e1) p + e2 and the knot value: the spline's value
+// phi_ab at fraction p of a record from the derivative coefficients of its
+// force record (e0 p + e1) p + e2 and the knot value c6: the spline's value
 // coefficients are (e0 dr / 3, e1 dr / 2, e2 dr, phi(r_m)), so the value needs
 // one 8-byte gather (the knot value, the last word of the value record)
 // instead of the 32-byte value record.
-__device__ __forceinline__ double phi_from_deriv(const TabF64 &tf, uint32_t rp, double p, double e0, double e1,
+__device__ __forceinline__ double phi_from_deriv(const TabF64 &tf, double c6, double p, double e0, double e1,
                                                  double e2) {
-  const double c6 = ld(reinterpret_cast<const double *>(tf.phi.val), 4 * rp + 3);
   const double t = fma(p, e0 * (1.0 / 3.0), 0.5 * e1);
   return fma(tf.dr * p, fma(p, t, e2), c6);
+}
+// the knot value phi(r_m) of record rp (the last word of its value record)
+__device__ __forceinline__ double phi_knot(const TabF64 &tf, uint32_t rp) {
+  return ld(reinterpret_cast<const double *>(tf.phi.val), 4 * rp + 3);
 }
 
 // fpair = -(df_i rho'_{tj} + df_j rho'_{ti} + phi'_{ti tj}) / r (the scalar that
@@ -202,22 +205,24 @@ __device__ __forceinline__ double phi_from_deriv(const TabF64 &tf, uint32_t rp, 
 // rinv = 1/r; an unlike pair brings rho'_{ti} in rhoip_u (a like pair reuses
 // rho'_{tj}).  phi receives phi_{ti tj}(r) when WANT_E and the (wave-uniform)
 // entry counts the pair's energy.
-// c0..c2: the record's three 16-byte pieces.
+// c0..c2: the record's three 16-byte pieces; c6 its knot value (phi_knot,
+// read only when the entry counts the pair energy).
 template <bool WANT_E>
-__device__ __forceinline__ double fpair_f64_c(const TabF64 &tf, double2 c0, double2 c1, double2 c2, uint32_t rp,
+__device__ __forceinline__ double fpair_f64_c(const TabF64 &tf, double2 c0, double2 c1, double2 c2, double c6,
                                               double p, double rinv, double dfi, double dfj, bool unlike,
                                               double rhoip_u, bool count_e, double &phi) {
   const double rhojp = (c0.x * p + c0.y) * p + c1.x;
   const double phip = (c1.y * p + c2.x) * p + c2.y;
   const double rhoip = unlike ? rhoip_u : rhojp;
-  if (WANT_E && count_e) phi = phi_from_deriv(tf, rp, p, c1.y, c2.x, c2.y);
+  if (WANT_E && count_e) phi = phi_from_deriv(tf, c6, p, c1.y, c2.x, c2.y);
   return -(dfi * rhojp + dfj * rhoip + phip) * rinv;
 }
 
 template <bool WANT_E>
 __device__ __forceinline__ double fpair_f64_rec(const TabF64 &tf, uint32_t rp, double p, double rinv, double dfi,
                                                 double dfj, bool unlike, double rhoip_u, bool count_e, double &phi) {
-  return fpair_f64_c<WANT_E>(tf, ld(tf.fc, 3 * rp), ld(tf.fc, 3 * rp + 1), ld(tf.fc, 3 * rp + 2), rp, p, rinv, dfi,
+  const double c6 = WANT_E && count_e ? phi_knot(tf, rp) : 0.0;
+  return fpair_f64_c<WANT_E>(tf, ld(tf.fc, 3 * rp), ld(tf.fc, 3 * rp + 1), ld(tf.fc, 3 * rp + 2), c6, p, rinv, dfi,
                              dfj, unlike, rhoip_u, count_e, phi);
 }
 
@@ -712,9 +717,16 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
           // them after (issued first they spill there).
           const uint32_t hl = threadIdx.x & 1u;
           double2 r0[kUnroll], r1[kUnroll], r2[kUnroll];
+          double kv[kUnroll];  // knot values of the entries that count the pair energy
           auto gather_records = [&] {
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) {
+              // issued with the records (read one by one where they are
+              // used, each waited for in turn)
+              if constexpr (WANT_E) {
+                kv[u] = 0.0;
+                if (ce[u]) kv[u] = phi_knot(tf, rp[u]);
+              }
               const uint32_t re = pair_even(rp[u]), ro = pair_odd(rp[u]);
               if constexpr (!HALF) {
                 // (the 24-bit multiply keeps 3 rp off the quarter-rate 32-bit
@@ -747,8 +759,8 @@ __device__ __forceinline__ void force_block(const Geom &g, const Halo &h, int64_
               c1 = pair_swap(hl ? r0[u] : r2[u]);
               c2 = hl ? r2[u] : r1[u];
             }
-            fp[u] = fpair_f64_c<WANT_E>(tf, c0, c1, c2, rp[u], pr[u], rinv[u], dfi, dfj[u], tq[u] != ti, rhoq[u], ce[u],
-                                        phi[u]);
+            fp[u] = fpair_f64_c<WANT_E>(tf, c0, c1, c2, WANT_E ? kv[u] : 0.0, pr[u], rinv[u], dfi, dfj[u], tq[u] != ti,
+                                        rhoq[u], ce[u], phi[u]);
           }
         }
 #pragma unroll
