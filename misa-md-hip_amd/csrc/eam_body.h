@@ -20,13 +20,15 @@ namespace mmd {
 #endif
 constexpr int kUnroll = MMD_PAIR_UNROLL;  // stencil entries whose loads are in flight together
 // Stencil entries per trip of the rho pass.  Its occupancy is set by its LDS
-// tile (four blocks per CU), so the mixed MD pass (one 16-byte gather per
-// pair, under 100 registers) keeps eight entries' loads in flight; the FP64
-// pass and the passes that sum two rho records would leave four waves per
-// SIMD at eight (measured slower) and keep four.
+// tile (four blocks per CU), so a pass that stays within 128 registers at
+// eight entries keeps eight entries' loads in flight: the mixed MD pass (one
+// 16-byte gather per pair, under 100 registers) and the FP64 pass (106; half
+// the trips' waits and ballots, 0.3-1.5 % faster than at four).  The mixed
+// passes that sum two rho records, or that keep the cubic records for
+// energies, measured slower at eight and keep four.
 template <int PREC, bool LEAN, bool WANT_E>
 constexpr int rho_unroll() {
-  return PREC == 1 && LEAN && !WANT_E ? 8 : 4;
+  return PREC == 0 || (PREC == 1 && LEAN && !WANT_E) ? 8 : 4;
 }
 static_assert(32 % kUnroll == 0, "a trip's entries share one word of the energy masks");
 // stencil rows are read up to index nk + U - 1 for U entries per trip (the
